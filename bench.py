@@ -1,0 +1,157 @@
+"""Throughput benchmark of the MI355X path tracer (one JSON line on stdout).
+
+A step is one dispatch of the plain tracer over one frame of the BVH-stress workload:
+65,536 random triangles + one emissive sphere, 320x180, 4 spp, 4 bounces, median-split
+BVH (<= 4 primitives per leaf), black background, frame seed 0 (the reference's static
+benchmark mode, path_tracer.rs:602-609), per-pixel PCG streams as in the reference
+kernel.  Scene data is resident in HBM before timing; value = paths traced per second.
+
+roofline.achieved uses algorithmic bytes per launch: the reference-layout bytes the
+reference traversal reads for this frame (48 B per node visited, 8 B per primitive
+reference, 64/32 B per triangle/sphere test, 48+80 B per closest hit, 96 B per light
+record scanned, 16 B per output pixel), counted by the oracle on the same seeds, divided
+by the kernel's mean duration from HIP events around each launch.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(c, pixels, accumulated=False):
+    b = (48 * c["nodes_visited"] + 8 * c["primref_tests"] + 64 * c["triangle_tests"] +
+         32 * c["sphere_tests"] + 48 * c["closest_hits"] + 80 * c["triangle_hits"] +
+         96 * c["lights_scanned"] + 4 * 16 * c["env_lookups"] + 16 * 19 * c["env_samples"] +
+         16 * pixels)
+    if accumulated:
+        b += 16 * pixels
+    return b
+
+
+def cpu_baseline(sc, budget_s=8.0):
+    """The oracle (C++ restatement, pcg mode) on this host's cores: full frames until the
+    budget is spent; also returns its counters for the algorithmic-byte figure."""
+    import oracle
+    oc = oracle.OracleScene(sc)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    _, counters = oc.render(threads=threads, counters=True)
+    frames, t0 = 0, time.perf_counter()
+    while True:
+        oc.render(threads=threads)
+        frames += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    # single-thread figure on a row band (reference CPU renderer is single-threaded)
+    rows = max(1, sc.height // 8)
+    t1 = time.perf_counter()
+    oc.render(threads=1, rows=(0, rows))
+    dt1 = time.perf_counter() - t1
+    oc.close()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    paths = sc.paths_per_frame
+    return {
+        "value": round(frames * paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads,
+        "kind": "port",
+        "sample": f"{frames} full frames of the same workload (oracle, pcg mode, OpenMP over pixels)",
+        "single_thread_mpaths_s": round(rows * sc.width * sc.samples / dt1 / 1e6, 4),
+        "cpu_model": model, "nproc": os.cpu_count(),
+    }, counters
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--triangles", type=int, default=65536)
+    ap.add_argument("--traversal", choices=["reference", "ordered"], default="ordered")
+    ap.add_argument("--rng", choices=["reference", "per_sample"], default="reference")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
+    sc = scenes.bvh_stress(args.triangles)
+    trav = A.TH_TRAVERSE_ORDERED if args.traversal == "ordered" else A.TH_TRAVERSE_REFERENCE
+    rng = A.TH_RNG_REFERENCE if args.rng == "reference" else A.TH_RNG_PER_SAMPLE
+    opts = RenderOptions(rng_mode=rng, traversal=trav)
+    ctx = GpuContext(0)
+    gs = GpuScene.from_scene(ctx, sc)
+    # one counting pass (outside the timed region) for rays per frame
+    gs.render(options=RenderOptions(rng_mode=rng, traversal=trav, collect_stats=True))
+    st = ctx.stats()
+    rays_per_frame = st["rays"]
+    for _ in range(args.warmup):
+        gs.render(options=opts, frame_index=0, accumulated_samples=0)
+    ctx.synchronize()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gs.render(options=opts, frame_index=0, accumulated_samples=0)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    st = ctx.stats()
+    gs.close()
+    ctx.close()
+    dt = t1 - t0
+    paths = sc.paths_per_frame
+    ms_step = dt * 1e3 / args.steps
+    kernel_ms = st["total_kernel_ms"] / max(1, st["launches"])
+
+    cpu, counters = (None, None)
+    if not args.no_cpu:
+        cpu, counters = cpu_baseline(sc)
+    roofline = None
+    if counters is not None:
+        bytes_launch = algorithmic_bytes(counters, sc.width * sc.height)
+        achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_bvh_stress_{args.triangles}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                    "algorithmic_bytes_per_launch": bytes_launch,
+                    "kernel": "k_render_pixel", "kernel_ms": round(kernel_ms, 5)}
+    line = {
+        "metric": "Mpaths/s (plain HIP tracer, BVH-stress workload)",
+        "value": round(args.steps * paths / dt / 1e6, 3),
+        "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"bvh_stress_{args.triangles}: {args.triangles} random triangles + "
+                               "1 emissive sphere, 320x180, 4 spp, 4 bounces, median-split BVH",
+                   "triangles": args.triangles, "width": sc.width, "height": sc.height,
+                   "spp": sc.samples, "max_bounces": sc.max_bounces, "paths_per_step": paths,
+                   "rng": args.rng, "traversal": args.traversal, "kernel": "megakernel"},
+        "rays_per_s": round(args.steps * rays_per_frame / dt, 1),
+        "kernel_ms_mean": round(kernel_ms, 5),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "host": platform.node(),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,340 @@
+/*
+ * toaster_hip.h -- C ABI of the MI355X path-tracing library (libtoaster_hip.so).
+ *
+ * The library replaces the wgpu device runtime and WGSL megakernel of the
+ * reference renderer (crates/toaster-gpu + shaders/pathtrace_spheres.wgsl) behind
+ * a plain C boundary: plain pointers, sizes and int32 status codes, no C++ or torch
+ * types.  Every function returns TH_OK (0) on success or a negative TH_E* code;
+ * the message of the last failure is returned by th_last_error().
+ *
+ * Two groups of entry points:
+ *   1. host packing  -- th_pack_scene() turns a renderer-neutral scene into the
+ *      flat GPU records (reference: crates/toaster-gpu/src/scene_upload.rs:60-76,
+ *      scene_to_gpu).  It needs no GPU.
+ *   2. device runtime -- contexts, scene upload/update, render, readback, stats
+ *      (reference: crates/toaster-gpu/src/{device,buffers,dispatch,readback}.rs
+ *      and path_tracer.rs:518-849).
+ *
+ * Record layouts are byte-identical to crates/toaster-gpu/src/gpu_types.rs:8-195
+ * (sizes asserted below exactly as gpu_types.rs:200-211 asserts them).
+ */
+#ifndef TOASTER_HIP_H
+#define TOASTER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TH_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+#define TH_OK 0
+#define TH_E_INVALID (-1)     /* null pointer, bad size, bad count, bad enum   */
+#define TH_E_DEVICE (-2)      /* HIP runtime failure                          */
+#define TH_E_NO_DEVICE (-3)   /* no GPU visible                               */
+#define TH_E_MISMATCH (-4)    /* counts changed after create, size mismatch   */
+#define TH_E_CAPACITY (-5)    /* node count above the allocated capacity      */
+#define TH_E_UNSUPPORTED (-6) /* option not supported by this build           */
+
+/* ------------------------------------------------------ GPU record layouts */
+/* gpu_types.rs:8-56 GpuRenderParams (96 B) */
+typedef struct TH_GpuRenderParams {
+    uint32_t width, height, samples, max_bounces;
+    uint32_t sphere_count, triangle_count, material_count, frame_index;
+    uint32_t background_kind; /* 0 sky, 1 black, 2 environment (scene_upload.rs:257-261) */
+    uint32_t light_count;
+    float total_light_weight;
+    uint32_t _pad0;
+    uint32_t environment_width, environment_height;
+    float environment_intensity, environment_rotation_degrees;
+    uint32_t accumulated_samples;
+    uint32_t _pad1[3];
+    uint32_t bvh_node_count, bvh_primitive_count;
+    uint32_t _pad2[2];
+} TH_GpuRenderParams;
+
+/* gpu_types.rs:58-70 GpuCamera (64 B) */
+typedef struct TH_GpuCamera {
+    float origin[4];
+    float lower_left_corner[4];
+    float horizontal[4];
+    float vertical[4];
+} TH_GpuCamera;
+
+/* gpu_types.rs:72-83 GpuSphere (32 B) */
+typedef struct TH_GpuSphere {
+    float center_radius[4];
+    uint32_t material_index;
+    uint32_t _pad0[3];
+} TH_GpuSphere;
+
+/* gpu_types.rs:85-100 GpuTriangle (64 B) */
+typedef struct TH_GpuTriangle {
+    float v0[4], v1[4], v2[4];
+    uint32_t material_index;
+    uint32_t _pad0[3];
+} TH_GpuTriangle;
+
+/* gpu_types.rs:102-123 GpuTriangleAttributes (80 B); flags bit0 normals, bit1 uvs */
+typedef struct TH_GpuTriangleAttributes {
+    float n0[4], n1[4], n2[4];
+    float uv0[2], uv1[2], uv2[2];
+    uint32_t flags;
+    uint32_t _pad0;
+} TH_GpuTriangleAttributes;
+
+/* gpu_types.rs:125-143 GpuMaterial (48 B); kind 0 diffuse 1 metal 2 dielectric 3 emissive */
+typedef struct TH_GpuMaterial {
+    uint32_t kind, texture_offset, texture_width, texture_height;
+    float albedo[4];
+    float params[4]; /* roughness, ior, emission strength, unused */
+} TH_GpuMaterial;
+
+/* gpu_types.rs:145-167 GpuLight (96 B); kind 0 triangle, 1 sphere */
+typedef struct TH_GpuLight {
+    uint32_t kind, material_index;
+    uint32_t _pad0[2];
+    float v0[4], v1[4], v2[4];
+    float center_radius[4];
+    float area_weight_cumulative[4];
+} TH_GpuLight;
+
+/* gpu_types.rs:169-184 GpuBvhNode (48 B); primitive_count > 0 <=> leaf */
+typedef struct TH_GpuBvhNode {
+    float min[4];
+    float max[4];
+    uint32_t first_or_right;
+    uint32_t primitive_count;
+    uint32_t _pad[2];
+} TH_GpuBvhNode;
+
+/* gpu_types.rs:186-194 GpuPrimitiveRef (8 B); kind 0 sphere, 1 triangle */
+typedef struct TH_GpuPrimitiveRef {
+    uint32_t kind;
+    uint32_t index;
+} TH_GpuPrimitiveRef;
+
+/* ------------------------------------------------ renderer-neutral scene */
+/* Mirrors toaster-scene's Scene (crates/toaster-scene/src/scene.rs:19-90),
+ * Material (material.rs:7-38), Sphere/Triangle/TriangleAttributes (object.rs:7-36)
+ * and EnvironmentMap (environment.rs:9-24). */
+#define TH_MAT_DIFFUSE 0
+#define TH_MAT_METAL 1
+#define TH_MAT_DIELECTRIC 2
+#define TH_MAT_EMISSIVE 3
+#define TH_MAT_TEXTURED_DIFFUSE 4
+
+#define TH_BG_SKY 0
+#define TH_BG_BLACK 1
+#define TH_BG_ENVIRONMENT 2
+
+typedef struct th_material {
+    uint32_t kind;          /* TH_MAT_* */
+    float color[3];         /* albedo (diffuse/metal/textured) or emission color */
+    float roughness;        /* metal */
+    float ior;              /* dielectric */
+    float strength;         /* emissive */
+    uint32_t texture_index; /* textured diffuse */
+} th_material;
+
+typedef struct th_sphere {
+    float center[3];
+    float radius;
+    uint32_t material_index;
+} th_sphere;
+
+typedef struct th_triangle {
+    float v[3][3];
+    uint32_t material_index;
+} th_triangle;
+
+typedef struct th_triangle_attributes {
+    uint32_t has_normals, has_tex_coords;
+    float normals[3][3];
+    float tex_coords[3][2];
+} th_triangle_attributes;
+
+typedef struct th_texture {
+    uint32_t width, height;
+    const uint8_t* rgba8; /* width*height*4 bytes, sRGB */
+} th_texture;
+
+typedef struct th_environment {
+    uint32_t width, height;
+    const float* rgb; /* width*height*3 linear radiance, row-major */
+    float intensity;
+    float rotation_degrees; /* normalised with rem_euclid(360) by the packer */
+} th_environment;
+
+typedef struct th_scene_input {
+    float camera_position[3], camera_look_at[3], camera_up[3];
+    float fov_degrees;
+    uint32_t width, height, samples, max_bounces, background; /* TH_BG_* */
+    const th_material* materials;
+    uint32_t material_count;
+    const th_sphere* spheres;
+    uint32_t sphere_count;
+    const th_triangle* triangles;
+    uint32_t triangle_count;
+    const th_triangle_attributes* triangle_attributes; /* NULL or triangle_count entries */
+    const th_texture* textures;
+    uint32_t texture_count;
+    const th_environment* environment; /* NULL when absent */
+} th_scene_input;
+
+/* ------------------------------------------------------ packed GPU scene */
+/* SceneGpuData (scene_upload.rs:17-40): params, camera and 9 flat arrays. */
+typedef struct th_scene_desc {
+    TH_GpuRenderParams params;
+    TH_GpuCamera camera;
+    const TH_GpuSphere* spheres;
+    const TH_GpuTriangle* triangles;
+    const TH_GpuTriangleAttributes* triangle_attributes;
+    const TH_GpuMaterial* materials;
+    const TH_GpuLight* lights;
+    const TH_GpuBvhNode* bvh_nodes;
+    const TH_GpuPrimitiveRef* bvh_primitives;
+    const uint32_t* texture_pixels;
+    const float* environment_pixels; /* float4 per texel: rgb + CDF in w */
+    uint32_t sphere_count, triangle_count, material_count, light_count;
+    uint32_t bvh_node_count, bvh_primitive_count, texture_pixel_count, environment_pixel_count;
+} th_scene_desc;
+
+typedef struct th_packed_scene th_packed_scene; /* opaque, owns the arrays */
+
+/* Packs a scene: validation (scene_upload.rs:84-92), camera (352-368), spheres,
+ * triangles, attributes, texture atlas, materials (371-404), power-weighted light
+ * list (412-475), median-split BVH (296-340 + crates/toaster-bvh) and the
+ * environment importance CDF (environment.rs:245-278 via 231-246).
+ * include_static_pixels = 0 mirrors scene_to_gpu_frame (texture/env omitted). */
+int32_t th_pack_scene(const th_scene_input* scene, int32_t include_static_pixels,
+                      th_packed_scene** out);
+int32_t th_packed_view(const th_packed_scene* packed, th_scene_desc* out);
+void th_packed_free(th_packed_scene* packed);
+
+/* Camera basis exactly as make_camera (scene_upload.rs:352-368). */
+int32_t th_make_camera(const float position[3], const float look_at[3], const float up[3],
+                       float fov_degrees, float aspect, TH_GpuCamera* out);
+
+/* Rebuilds only the BVH of a packed scene from replaced geometry (per-frame
+ * rebuild of path_tracer.rs:611-638); counts must be unchanged. */
+int32_t th_build_bvh(const TH_GpuSphere* spheres, uint32_t sphere_count,
+                     const TH_GpuTriangle* triangles, uint32_t triangle_count,
+                     TH_GpuBvhNode* nodes_out, uint32_t node_capacity, uint32_t* node_count,
+                     TH_GpuPrimitiveRef* prims_out, uint32_t prim_capacity);
+
+/* ------------------------------------------------------ device runtime */
+typedef struct th_context th_context;
+typedef struct th_scene th_scene;
+
+/* Random-number mode of a render.
+ * TH_RNG_REFERENCE: one PCG stream per pixel shared by all samples of a dispatch,
+ *   seeded pcg_hash(idx ^ frame_index*9781) (pathtrace_spheres.wgsl:519); output is
+ *   per-pixel comparable with the reference estimator.
+ * TH_RNG_PER_SAMPLE: an independent stream per (pixel, sample); same estimator,
+ *   statistically equivalent, more parallel work per dispatch. */
+#define TH_RNG_REFERENCE 0
+#define TH_RNG_PER_SAMPLE 1
+
+/* Kernel organisation of a render. */
+#define TH_KERNEL_MEGA 0      /* one fused kernel, one lane per pixel              */
+#define TH_KERNEL_WAVEFRONT 1 /* extend / shade / shadow stages with compaction   */
+
+/* Traversal order for closest-hit queries. */
+#define TH_TRAVERSE_REFERENCE 0 /* visit order of wgsl:746-792 (right child first) */
+#define TH_TRAVERSE_ORDERED 1   /* nearer child first, reference tie-breaking      */
+
+typedef struct th_render_options {
+    uint32_t rng_mode;       /* TH_RNG_*                                         */
+    uint32_t kernel;         /* TH_KERNEL_*                                      */
+    uint32_t traversal;      /* TH_TRAVERSE_*                                    */
+    uint32_t block_size;     /* 0 = autotuned / default                          */
+    uint32_t collect_stats;  /* 1 = count rays / nodes / tests (slower launch)   */
+    uint32_t reserved[3];
+} th_render_options;
+
+typedef struct th_stats {
+    double last_kernel_ms;    /* device time of the last th_render (hipEvents)    */
+    double total_kernel_ms;   /* summed device time of renders since th_reset_stats */
+    uint64_t launches;        /* renders timed into total_kernel_ms               */
+    uint64_t paths;           /* camera paths traced by the last render           */
+    uint64_t rays;            /* closest-hit + shadow rays cast                   */
+    uint64_t closest_rays;
+    uint64_t shadow_rays;
+    uint64_t nodes_visited;   /* reference-layout nodes whose box was tested      */
+    uint64_t primref_tests;   /* leaf primitive slots tested                      */
+    uint64_t triangle_tests;
+    uint64_t sphere_tests;
+    uint64_t closest_hits;    /* closest-hit rays that hit a surface              */
+    uint64_t triangle_hits;   /* ... of which on a triangle                       */
+    uint64_t lights_scanned;  /* light records read by emitter selection          */
+    uint64_t env_lookups;     /* bilinear environment evaluations                 */
+    uint64_t env_samples;     /* environment CDF searches                         */
+} th_stats;
+
+int32_t th_device_count(int32_t* count);
+int32_t th_context_create(int32_t device, th_context** out);
+void th_context_destroy(th_context* ctx);
+/* Message of the last failure on this thread (never NULL). */
+const char* th_last_error(void);
+
+/* Uploads a packed scene (buffers.rs:117-271).  The library copies every array;
+ * the caller keeps ownership.  node_capacity = 0 selects 2N-1 (buffers.rs:205-211). */
+int32_t th_scene_create(th_context* ctx, const th_scene_desc* desc, uint32_t node_capacity,
+                        th_scene** out);
+/* Per-frame update of the arrays whose bit is set (path_tracer.rs:676-720). */
+#define TH_CHANGE_CAMERA 1u
+#define TH_CHANGE_SPHERES 2u
+#define TH_CHANGE_TRIANGLES 4u
+#define TH_CHANGE_LIGHTS 8u
+#define TH_CHANGE_BVH 16u
+#define TH_CHANGE_MATERIALS 32u
+int32_t th_scene_update(th_scene* scene, uint32_t change_mask, const th_scene_desc* desc);
+void th_scene_destroy(th_scene* scene);
+
+/* One dispatch: params.samples samples per pixel folded into the progressive
+ * average with params.accumulated_samples (wgsl:538-547).  Asynchronous. */
+int32_t th_render(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
+                  const TH_GpuCamera* camera, const th_render_options* options);
+/* Blocks, then copies width*height float4 linear-HDR pixels (readback.rs:8-25). */
+int32_t th_readback(th_context* ctx, th_scene* scene, float* rgba, size_t bytes);
+/* Overwrites the accumulation buffer (progressive resume from a checkpoint). */
+int32_t th_upload_accumulation(th_context* ctx, th_scene* scene, const float* rgba, size_t bytes);
+int32_t th_synchronize(th_context* ctx);
+/* Waits for pending renders, then reports timing and (if collected) counters. */
+int32_t th_get_stats(const th_context* ctx, th_stats* out);
+int32_t th_reset_stats(th_context* ctx);
+
+/* Closest hit of every pixel's first camera ray (first two draws of the pixel's
+ * reference stream): per pixel the leaf slot index (UINT32_MAX = miss) and the hit
+ * distance (3.4028235e38 = miss).  Parity query for traversal + intersection. */
+int32_t th_primary_hits(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
+                        const TH_GpuCamera* camera, uint32_t traversal, uint32_t* slot_out,
+                        float* distance_out, size_t count);
+
+/* Linear HDR float4 -> RGBA8 with exposure, ACES fitted and sRGB OETF
+ * (image_output.rs:59-92, color.rs:23-75).  Host implementation. */
+int32_t th_display_rgba8(const float* rgba, uint32_t width, uint32_t height, float exposure_stops,
+                         uint8_t* out, size_t out_bytes);
+/* The same conversion on the device from a scene's accumulation buffer. */
+int32_t th_display_rgba8_device(th_context* ctx, th_scene* scene, float exposure_stops,
+                                uint8_t* out, size_t out_bytes);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+static_assert(sizeof(TH_GpuRenderParams) == 96, "GpuRenderParams");
+static_assert(sizeof(TH_GpuCamera) == 64, "GpuCamera");
+static_assert(sizeof(TH_GpuSphere) == 32, "GpuSphere");
+static_assert(sizeof(TH_GpuTriangle) == 64, "GpuTriangle");
+static_assert(sizeof(TH_GpuTriangleAttributes) == 80, "GpuTriangleAttributes");
+static_assert(sizeof(TH_GpuMaterial) == 48, "GpuMaterial");
+static_assert(sizeof(TH_GpuLight) == 96, "GpuLight");
+static_assert(sizeof(TH_GpuBvhNode) == 48, "GpuBvhNode");
+static_assert(sizeof(TH_GpuPrimitiveRef) == 8, "GpuPrimitiveRef");
+#endif
+
+#endif /* TOASTER_HIP_H */

@@ -1,0 +1,832 @@
+// th_runtime.hip -- device runtime of libtoaster_hip: contexts, scene upload, prepare
+// kernels, the path-tracing megakernel and the C ABI declared in include/toaster_hip.h.
+//
+// Replaces crates/toaster-gpu/src/{device.rs:49-100, buffers.rs:117-271,
+// dispatch.rs:28-60, readback.rs:8-25} and the WGSL entry point
+// shaders/pathtrace_spheres.wgsl:508-548.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "th_common.h"
+#include "th_integrator.h"
+
+using namespace thd;
+
+// ====================================================================== kernels
+// Reference nodes -> child-pair records (one per reference node index).
+__global__ void k_prepare_cnodes(const TH_GpuBvhNode* __restrict__ nodes, uint32_t count,
+                                 CNode* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const TH_GpuBvhNode& n = nodes[i];
+  CNode c;
+  if (n.primitive_count > 0u) {
+    c.a = make_float4(0, 0, 0, 0);
+    c.b = c.a;
+    c.c = c.a;
+    c.d = make_uint4(0, 0, 0, 0);
+  } else {
+    uint32_t li = i + 1u, ri = n.first_or_right;
+    const TH_GpuBvhNode& L = nodes[li];
+    const TH_GpuBvhNode& R = nodes[ri];
+    c.a = make_float4(L.min[0], L.min[1], L.min[2], L.max[0]);
+    c.b = make_float4(L.max[1], L.max[2], R.min[0], R.min[1]);
+    c.c = make_float4(R.min[2], R.max[0], R.max[1], R.max[2]);
+    c.d = make_uint4(th_entry_of(L, li), th_entry_of(R, ri), 0u, 0u);
+  }
+  out[i] = c;
+}
+
+// Reference primitive refs -> 48-byte leaf-ordered slots.
+__global__ void k_prepare_slots(const TH_GpuPrimitiveRef* __restrict__ prims, uint32_t count,
+                                const TH_GpuSphere* __restrict__ spheres,
+                                const TH_GpuTriangle* __restrict__ tris,
+                                const TH_GpuTriangleAttributes* __restrict__ attrs,
+                                float4* __restrict__ slots) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  TH_GpuPrimitiveRef r = prims[i];
+  float4 a, b, c;
+  if (r.kind == 1u) {
+    const TH_GpuTriangle& t = tris[r.index];
+    uint32_t tag = THD_KIND_TRI | ((attrs[r.index].flags & 3u) << 29) | (r.index & 0x1fffffffu);
+    a = make_float4(t.v0[0], t.v0[1], t.v0[2], __uint_as_float(t.material_index));
+    b = make_float4(t.v1[0] - t.v0[0], t.v1[1] - t.v0[1], t.v1[2] - t.v0[2], __uint_as_float(tag));
+    c = make_float4(t.v2[0] - t.v0[0], t.v2[1] - t.v0[1], t.v2[2] - t.v0[2],
+                    __uint_as_float(t.material_index));
+  } else {
+    const TH_GpuSphere& s = spheres[r.index];
+    a = make_float4(s.center_radius[0], s.center_radius[1], s.center_radius[2], s.center_radius[3]);
+    b = make_float4(0, 0, 0, __uint_as_float(r.index & 0x1fffffffu));
+    c = make_float4(0, 0, 0, __uint_as_float(s.material_index));
+  }
+  slots[3u * i] = a;
+  slots[3u * i + 1u] = b;
+  slots[3u * i + 2u] = c;
+}
+
+// Pixel of a lane: each 64-lane wave covers one 8x8 tile (coherent primary rays).
+__device__ __forceinline__ bool tile_pixel(uint32_t W, uint32_t H, uint32_t* x, uint32_t* y) {
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint32_t tiles_x = (W + 7u) >> 3;
+  *x = (tile % tiles_x) * 8u + (lane & 7u);
+  *y = (tile / tiles_x) * 8u + (lane >> 3);
+  return *x < W && *y < H;
+}
+
+// The megakernel: one lane per pixel, params.samples samples from the pixel's
+// reference PCG stream, then the progressive running average (wgsl:508-548).
+template <int TMODE, bool COUNT>
+__global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderParams P,
+                                                     TH_GpuCamera cam, float4* __restrict__ accum,
+                                                     const float* __restrict__ srgb_lut,
+                                                     unsigned long long* counters) {
+  extern __shared__ uint2 lds_stack[];
+  uint32_t x, y;
+  if (!tile_pixel(P.width, P.height, &x, &y)) return;
+  Stack st{lds_stack + threadIdx.x, (int)blockDim.x, 0};
+  LaneCounters<COUNT> lc;
+  Integrator<TMODE, COUNT> it{S, P, srgb_lut, st, lc};
+  uint32_t idx = y * P.width + x;
+  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
+  for (uint32_t i = 0; i < P.samples; ++i) {
+    v3 o, d;
+    camera_ray(cam, P, x, y, &s, &o, &d);
+    v3 L = it.ray_color(o, d, &s);
+    cr = cr + L.x;
+    cg = cg + L.y;
+    cb = cb + L.z;
+    ca = ca + 1.0f;
+  }
+  if (COUNT) lc.paths += P.samples;
+  float ns = (float)P.samples;
+  float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
+  if (P.accumulated_samples == 0u) {
+    accum[idx] = b;
+  } else {
+    float prev = (float)P.accumulated_samples;
+    float tot = prev + ns;
+    float4 o = accum[idx];
+    accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                             (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+  }
+  lc.flush(counters);
+}
+
+// Per-sample RNG variant: G consecutive lanes share one pixel and split its samples
+// (lane j takes samples j, j+G, ...); sample s of pixel idx draws from the stream
+// seeded pcg_hash((idx ^ frame*9781) + s*0x9E3779B9), which for s = 0 is the
+// reference seed.  The G partial sums are combined in a fixed tree order.
+template <int TMODE, bool COUNT, int G>
+__global__ void __launch_bounds__(64) k_render_sample(DevScene S, TH_GpuRenderParams P,
+                                                      TH_GpuCamera cam, float4* __restrict__ accum,
+                                                      const float* __restrict__ srgb_lut,
+                                                      unsigned long long* counters) {
+  extern __shared__ uint2 lds_stack[];
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;  // pixel slot
+  uint32_t j = lane % G;
+  // pixels per wave = 64/G, laid out as tiles of (8 x 8/G)
+  uint32_t ppw = 64u / G;
+  uint32_t wave = gid / ppw, pw = gid % ppw;
+  uint32_t th_ = 8u / (G >= 8 ? 8u : (uint32_t)G);  // tile height
+  uint32_t tw = ppw / th_;
+  uint32_t tiles_x = (P.width + tw - 1u) / tw;
+  uint32_t x = (wave % tiles_x) * tw + pw % tw;
+  uint32_t y = (wave / tiles_x) * th_ + pw / tw;
+  bool active = x < P.width && y < P.height;
+  Stack st{lds_stack + threadIdx.x, (int)blockDim.x, 0};
+  LaneCounters<COUNT> lc;
+  Integrator<TMODE, COUNT> it{S, P, srgb_lut, st, lc};
+  uint32_t idx = y * P.width + x;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
+  if (active) {
+    uint32_t base = idx ^ (P.frame_index * 9781u);
+    for (uint32_t smp = j; smp < P.samples; smp += G) {
+      uint32_t s = th::pcg_hash(base + smp * 0x9E3779B9u);
+      v3 o, d;
+      camera_ray(cam, P, x, y, &s, &o, &d);
+      v3 L = it.ray_color(o, d, &s);
+      cr = cr + L.x;
+      cg = cg + L.y;
+      cb = cb + L.z;
+      ca = ca + 1.0f;
+      if (COUNT) lc.paths++;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    cr = cr + __shfl_xor(cr, off, 64);
+    cg = cg + __shfl_xor(cg, off, 64);
+    cb = cb + __shfl_xor(cb, off, 64);
+    ca = ca + __shfl_xor(ca, off, 64);
+  }
+  if (active && j == 0u) {
+    float ns = (float)P.samples;
+    float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
+    if (P.accumulated_samples == 0u) {
+      accum[idx] = b;
+    } else {
+      float prev = (float)P.accumulated_samples;
+      float tot = prev + ns;
+      float4 o = accum[idx];
+      accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                               (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+    }
+  }
+  lc.flush(counters);
+}
+
+// Closest hit of each pixel's first camera ray (parity query).
+template <int TMODE>
+__global__ void __launch_bounds__(64) k_primary(DevScene S, TH_GpuRenderParams P, TH_GpuCamera cam,
+                                                uint32_t* __restrict__ slot_out,
+                                                float* __restrict__ t_out) {
+  extern __shared__ uint2 lds_stack[];
+  uint32_t x, y;
+  if (!tile_pixel(P.width, P.height, &x, &y)) return;
+  Stack st{lds_stack + threadIdx.x, (int)blockDim.x, 0};
+  uint32_t idx = y * P.width + x;
+  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  v3 o, d;
+  camera_ray(cam, P, x, y, &s, &o, &d);
+  RayStats<false> rs;
+  float t;
+  uint32_t slot;
+  if (S.node_count == 0u)
+    t = closest_linear<false>(S, P.sphere_count, P.triangle_count, o, d, TH_FLT_MAX, rs, &slot);
+  else
+    traverse<TMODE, false>(S, o, d, TH_FLT_MAX, st, rs, &t, &slot);
+  slot_out[idx] = slot;
+  t_out[idx] = t;
+}
+
+// Display conversion on the device (color.rs:42-55, image_output.rs:59-92).
+__global__ void k_display(const float4* __restrict__ accum, uint32_t n, float scale,
+                          const float* __restrict__ thr, uchar4* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = accum[i];
+  out[i] = make_uchar4(th_display_channel(p.x, scale, thr), th_display_channel(p.y, scale, thr),
+                       th_display_channel(p.z, scale, thr), th_u8(p.w));
+}
+
+// ====================================================================== host side
+struct th_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // unused spare pair
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool, ev_pending;
+  unsigned long long* d_counters = nullptr;
+  bool counters_pending = false;
+  th_stats stats;
+  float* d_srgb_lut = nullptr;
+};
+
+struct th_scene {
+  th_context* ctx = nullptr;
+  TH_GpuRenderParams params;  // counts fixed at create
+  uint32_t node_capacity = 0;
+  uint32_t width = 0, height = 0;
+  TH_GpuBvhNode* d_nodes = nullptr;
+  TH_GpuPrimitiveRef* d_prims = nullptr;
+  TH_GpuSphere* d_spheres = nullptr;
+  TH_GpuTriangle* d_tris = nullptr;
+  TH_GpuTriangleAttributes* d_attrs = nullptr;
+  TH_GpuMaterial* d_mats = nullptr;
+  TH_GpuLight* d_lights = nullptr;
+  float4* d_env = nullptr;
+  uint32_t* d_tex = nullptr;
+  CNode* d_cnodes = nullptr;
+  float4* d_slots = nullptr;
+  float4* d_accum = nullptr;
+  uint32_t node_count = 0, prim_count = 0, env_count = 0, tex_count = 0;
+  uint32_t root_entry = 0;
+  float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
+  uint32_t max_depth = 0;
+};
+
+namespace {
+
+#define TH_CHECK(expr)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return th_set_error(TH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int32_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);  // 1-element dummies (buffers.rs)
+  TH_CHECK(hipMalloc((void**)p, bytes));
+  TH_CHECK(hipMemset(*p, 0, bytes));
+  return TH_OK;
+}
+
+template <class T>
+int32_t upload(th_context* ctx, T* dst, const T* src, size_t count) {
+  if (count == 0) return TH_OK;
+  if (!src) return th_set_error(TH_E_INVALID, "null array with nonzero count");
+  TH_CHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return TH_OK;
+}
+
+void free_scene_buffers(th_scene* s) {
+  void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
+                  s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+// Tree depth (for the traversal stack size) from the reference layout.
+uint32_t bvh_depth(const TH_GpuBvhNode* nodes, uint32_t count) {
+  if (count == 0) return 0;
+  std::vector<uint32_t> depth(count, 0);
+  uint32_t maxd = 1;
+  depth[0] = 1;
+  for (uint32_t i = 0; i < count; ++i) {  // pre-order: parents precede children
+    if (nodes[i].primitive_count == 0) {
+      uint32_t l = i + 1, r = nodes[i].first_or_right;
+      if (l < count) depth[l] = depth[i] + 1;
+      if (r < count) depth[r] = depth[i] + 1;
+    }
+    maxd = std::max(maxd, depth[i]);
+  }
+  return maxd;
+}
+
+// Structural validation of a reference-layout BVH before any kernel reads it.
+int32_t validate_bvh(const th_scene_desc* d) {
+  uint32_t n = d->bvh_node_count, np = d->bvh_primitive_count;
+  if (np != d->sphere_count + d->triangle_count && n > 0)
+    return th_set_error(TH_E_INVALID, "bvh primitive count must equal sphere + triangle count");
+  for (uint32_t i = 0; i < n; ++i) {
+    const TH_GpuBvhNode& b = d->bvh_nodes[i];
+    if (b.primitive_count > 0) {
+      if ((uint64_t)b.first_or_right + b.primitive_count > np)
+        return th_set_error(TH_E_INVALID, "bvh leaf range out of bounds");
+      if (i >= (1u << 25) && (b.first_or_right >= (1u << 25) || b.primitive_count >= THD_BIG_LEAF))
+        return th_set_error(TH_E_UNSUPPORTED, "leaf beyond the 2^25 traversal-entry range");
+    } else {
+      if (i + 1 >= n || b.first_or_right >= n || b.first_or_right <= i + 1)
+        return th_set_error(TH_E_INVALID, "bvh interior child index out of bounds");
+    }
+  }
+  for (uint32_t i = 0; i < np; ++i) {
+    const TH_GpuPrimitiveRef& r = d->bvh_primitives[i];
+    if (r.kind > 1u) return th_set_error(TH_E_INVALID, "bvh primitive kind must be 0 or 1");
+    if ((r.kind == 0u && r.index >= d->sphere_count) || (r.kind == 1u && r.index >= d->triangle_count))
+      return th_set_error(TH_E_INVALID, "bvh primitive index out of range");
+  }
+  return TH_OK;
+}
+
+int32_t validate_refs(const th_scene_desc* d) {
+  for (uint32_t i = 0; i < d->sphere_count; ++i)
+    if (d->spheres[i].material_index >= d->material_count)
+      return th_set_error(TH_E_INVALID, "sphere material index out of range");
+  for (uint32_t i = 0; i < d->triangle_count; ++i)
+    if (d->triangles[i].material_index >= d->material_count)
+      return th_set_error(TH_E_INVALID, "triangle material index out of range");
+  for (uint32_t i = 0; i < d->light_count; ++i)
+    if (d->lights[i].material_index >= d->material_count || d->lights[i].kind > 1u)
+      return th_set_error(TH_E_INVALID, "light record out of range");
+  for (uint32_t i = 0; i < d->material_count; ++i) {
+    const TH_GpuMaterial& m = d->materials[i];
+    if (m.texture_width && m.texture_height &&
+        (uint64_t)m.texture_offset + (uint64_t)m.texture_width * m.texture_height > d->texture_pixel_count)
+      return th_set_error(TH_E_INVALID, "material texture out of range");
+  }
+  const TH_GpuRenderParams& p = d->params;
+  if (p.background_kind > 2u) return th_set_error(TH_E_INVALID, "unknown background kind");
+  if (p.background_kind == 2u &&
+      (p.environment_width == 0 || p.environment_height == 0 ||
+       (uint64_t)p.environment_width * p.environment_height != d->environment_pixel_count))
+    return th_set_error(TH_E_INVALID, "environment background requires an environment map");
+  if (p.light_count != d->light_count) return th_set_error(TH_E_MISMATCH, "light count mismatch");
+  return TH_OK;
+}
+
+int32_t validate_desc(const th_scene_desc* d) {
+  if (!d) return th_set_error(TH_E_INVALID, "null scene descriptor");
+  if (d->sphere_count + (uint64_t)d->triangle_count == 0)
+    return th_set_error(TH_E_INVALID, "GPU renderer requires at least one object");
+  if (d->material_count == 0 || !d->materials)
+    return th_set_error(TH_E_INVALID, "GPU renderer requires at least one material");
+  if ((d->sphere_count && !d->spheres) || (d->triangle_count && (!d->triangles || !d->triangle_attributes)) ||
+      (d->light_count && !d->lights) || (d->bvh_node_count && (!d->bvh_nodes || !d->bvh_primitives)) ||
+      (d->texture_pixel_count && !d->texture_pixels) ||
+      (d->environment_pixel_count && !d->environment_pixels))
+    return th_set_error(TH_E_INVALID, "null array with nonzero count");
+  if (d->params.width == 0 || d->params.height == 0)
+    return th_set_error(TH_E_INVALID, "render dimensions must be nonzero");
+  if (d->params.sphere_count != d->sphere_count || d->params.triangle_count != d->triangle_count ||
+      d->params.material_count != d->material_count || d->params.bvh_node_count != d->bvh_node_count ||
+      d->params.bvh_primitive_count != d->bvh_primitive_count)
+    return th_set_error(TH_E_MISMATCH, "params counts disagree with the arrays");
+  int32_t rc = validate_refs(d);
+  if (rc) return rc;
+  return validate_bvh(d);
+}
+
+int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
+  th_context* ctx = s->ctx;
+  s->node_count = d->bvh_node_count;
+  s->prim_count = d->bvh_primitive_count;
+  if (s->node_count) {
+    const TH_GpuBvhNode& r = d->bvh_nodes[0];
+    s->root_entry = th_entry_of(r, 0u);
+    for (int k = 0; k < 3; ++k) {
+      s->root_min[k] = r.min[k];
+      s->root_max[k] = r.max[k];
+    }
+    s->max_depth = bvh_depth(d->bvh_nodes, d->bvh_node_count);
+    hipLaunchKernelGGL(k_prepare_cnodes, dim3((s->node_count + 255) / 256), dim3(256), 0, ctx->stream,
+                       s->d_nodes, s->node_count, s->d_cnodes);
+    TH_CHECK(hipGetLastError());
+  }
+  if (s->prim_count) {
+    hipLaunchKernelGGL(k_prepare_slots, dim3((s->prim_count + 255) / 256), dim3(256), 0, ctx->stream,
+                       s->d_prims, s->prim_count, s->d_spheres, s->d_tris, s->d_attrs, s->d_slots);
+    TH_CHECK(hipGetLastError());
+  }
+  return TH_OK;
+}
+
+DevScene dev_scene(const th_scene* s) {
+  DevScene S;
+  S.cnodes = s->d_cnodes;
+  S.slots = s->d_slots;
+  S.nodes = s->d_nodes;
+  S.spheres = s->d_spheres;
+  S.triangles = s->d_tris;
+  S.attrs = s->d_attrs;
+  S.materials = s->d_mats;
+  S.lights = s->d_lights;
+  S.env = s->d_env;
+  S.tex = s->d_tex;
+  S.node_count = s->node_count;
+  S.root_entry = s->root_entry;
+  for (int k = 0; k < 3; ++k) {
+    S.root_min[k] = s->root_min[k];
+    S.root_max[k] = s->root_max[k];
+  }
+  return S;
+}
+
+// Per-launch event pair from a pool (every launch is timed; th_get_stats sums them).
+int32_t begin_timed(th_context* ctx, std::pair<hipEvent_t, hipEvent_t>* pr) {
+  if (ctx->ev_pending.size() >= 4096) {  // bound the pending list: fold the oldest
+    TH_CHECK(hipEventSynchronize(ctx->ev_pending.front().second));
+  }
+  if (ctx->ev_pool.empty()) {
+    hipEvent_t a, b;
+    TH_CHECK(hipEventCreate(&a));
+    TH_CHECK(hipEventCreate(&b));
+    ctx->ev_pool.push_back({a, b});
+  }
+  *pr = ctx->ev_pool.back();
+  ctx->ev_pool.pop_back();
+  TH_CHECK(hipEventRecord(pr->first, ctx->stream));
+  return TH_OK;
+}
+
+int32_t end_timed(th_context* ctx, std::pair<hipEvent_t, hipEvent_t> pr) {
+  hipError_t e = hipEventRecord(pr.second, ctx->stream);
+  if (e != hipSuccess) {
+    ctx->ev_pool.push_back(pr);
+    return th_set_error(TH_E_DEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
+  }
+  ctx->ev_pending.push_back(pr);
+  return TH_OK;
+}
+
+int32_t fold_timing(th_context* ctx) {
+  for (auto& pr : ctx->ev_pending) {
+    TH_CHECK(hipEventSynchronize(pr.second));
+    float ms = 0.0f;
+    TH_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    ctx->stats.last_kernel_ms = ms;
+    ctx->stats.total_kernel_ms += ms;
+    ctx->stats.launches += 1;
+    ctx->ev_pool.push_back(pr);
+  }
+  ctx->ev_pending.clear();
+  return TH_OK;
+}
+
+int32_t check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
+  if (!p) return th_set_error(TH_E_INVALID, "null params");
+  if (p->width != s->width || p->height != s->height)
+    return th_set_error(TH_E_MISMATCH, "render dimensions differ from the scene's output buffer");
+  if (p->samples == 0) return th_set_error(TH_E_INVALID, "samples must be nonzero");
+  if (p->sphere_count != s->params.sphere_count || p->triangle_count != s->params.triangle_count ||
+      p->material_count != s->params.material_count || p->bvh_node_count != s->node_count ||
+      p->bvh_primitive_count != s->prim_count || p->light_count != s->params.light_count)
+    return th_set_error(TH_E_MISMATCH, "params counts differ from the uploaded scene");
+  if (p->background_kind > 2u) return th_set_error(TH_E_INVALID, "unknown background kind");
+  if (p->background_kind == 2u &&
+      ((uint64_t)p->environment_width * p->environment_height != s->env_count || s->env_count == 0))
+    return th_set_error(TH_E_MISMATCH, "environment dimensions differ from the uploaded map");
+  return TH_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int32_t th_device_count(int32_t* count) {
+  if (!count) return th_set_error(TH_E_INVALID, "null argument");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return TH_OK;
+}
+
+int32_t th_context_create(int32_t device, th_context** out) {
+  if (!out) return th_set_error(TH_E_INVALID, "null argument");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return th_set_error(TH_E_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= n) return th_set_error(TH_E_INVALID, "device index out of range");
+  th_context* c = new th_context();
+  c->device = device;
+  memset(&c->stats, 0, sizeof c->stats);
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_counters, 16 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_srgb_lut, 512 * sizeof(float));
+  if (e == hipSuccess) {
+    float lut[512];
+    th_srgb_table(lut);
+    th_srgb_thresholds(lut + 256);
+    e = hipMemcpy(c->d_srgb_lut, lut, sizeof lut, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    th_context_destroy(c);
+    return th_set_error(TH_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return TH_OK;
+}
+
+void th_context_destroy(th_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_counters) (void)hipFree(c->d_counters);
+  if (c->d_srgb_lut) (void)hipFree(c->d_srgb_lut);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (auto* v : {&c->ev_pool, &c->ev_pending})
+    for (auto& pr : *v) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_capacity,
+                        th_scene** out) {
+  if (!ctx || !out) return th_set_error(TH_E_INVALID, "null argument");
+  *out = nullptr;
+  int32_t rc = validate_desc(d);
+  if (rc) return rc;
+  uint32_t nprim = d->sphere_count + d->triangle_count;
+  uint32_t cap = node_capacity ? node_capacity : (nprim ? 2 * nprim - 1 : 1);  // buffers.rs:205-211
+  if (d->bvh_node_count > cap) return th_set_error(TH_E_CAPACITY, "bvh node count exceeds capacity");
+  TH_CHECK(hipSetDevice(ctx->device));
+  th_scene* s = new th_scene();
+  s->ctx = ctx;
+  s->params = d->params;
+  s->node_capacity = cap;
+  s->width = d->params.width;
+  s->height = d->params.height;
+  s->env_count = d->environment_pixel_count;
+  s->tex_count = d->texture_pixel_count;
+  size_t npix = (size_t)s->width * s->height;
+  if ((rc = dalloc(&s->d_nodes, cap)) || (rc = dalloc(&s->d_prims, nprim)) ||
+      (rc = dalloc(&s->d_spheres, d->sphere_count)) || (rc = dalloc(&s->d_tris, d->triangle_count)) ||
+      (rc = dalloc(&s->d_attrs, d->triangle_count)) || (rc = dalloc(&s->d_mats, d->material_count)) ||
+      (rc = dalloc(&s->d_lights, d->light_count)) || (rc = dalloc(&s->d_env, d->environment_pixel_count)) ||
+      (rc = dalloc(&s->d_tex, d->texture_pixel_count)) || (rc = dalloc(&s->d_cnodes, cap)) ||
+      (rc = dalloc(&s->d_slots, 3 * (size_t)nprim)) || (rc = dalloc(&s->d_accum, npix))) {
+    free_scene_buffers(s);
+    delete s;
+    return rc;
+  }
+  if ((rc = upload(ctx, s->d_nodes, d->bvh_nodes, d->bvh_node_count)) ||
+      (rc = upload(ctx, s->d_prims, d->bvh_primitives, d->bvh_primitive_count)) ||
+      (rc = upload(ctx, s->d_spheres, d->spheres, d->sphere_count)) ||
+      (rc = upload(ctx, s->d_tris, d->triangles, d->triangle_count)) ||
+      (rc = upload(ctx, s->d_attrs, d->triangle_attributes, d->triangle_count)) ||
+      (rc = upload(ctx, s->d_mats, d->materials, d->material_count)) ||
+      (rc = upload(ctx, s->d_lights, d->lights, d->light_count)) ||
+      (rc = upload(ctx, (float*)s->d_env, d->environment_pixels, 4 * (size_t)d->environment_pixel_count)) ||
+      (rc = upload(ctx, s->d_tex, d->texture_pixels, d->texture_pixel_count)) ||
+      (rc = prepare_bvh(s, d))) {
+    (void)hipStreamSynchronize(ctx->stream);
+    free_scene_buffers(s);
+    delete s;
+    return rc;
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free_scene_buffers(s);
+    delete s;
+    return th_set_error(TH_E_DEVICE, std::string("scene upload: ") + hipGetErrorString(e));
+  }
+  *out = s;
+  return TH_OK;
+}
+
+int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
+  if (!s || !d) return th_set_error(TH_E_INVALID, "null argument");
+  th_context* ctx = s->ctx;
+  // path_tracer.rs:639-653: counts are fixed after setup; nodes within capacity.
+  if (d->sphere_count != s->params.sphere_count || d->triangle_count != s->params.triangle_count ||
+      d->material_count != s->params.material_count || d->light_count != s->params.light_count ||
+      d->bvh_primitive_count != s->prim_count || d->params.width != s->width ||
+      d->params.height != s->height)
+    return th_set_error(TH_E_MISMATCH, "scene counts changed after setup");
+  if (d->bvh_node_count > s->node_capacity)
+    return th_set_error(TH_E_CAPACITY, "bvh node count exceeds capacity");
+  int32_t rc = validate_desc(d);
+  if (rc) return rc;
+  TH_CHECK(hipSetDevice(ctx->device));
+  bool geometry = false;
+  if (mask & TH_CHANGE_SPHERES) {
+    if ((rc = upload(ctx, s->d_spheres, d->spheres, d->sphere_count))) return rc;
+    geometry = true;
+  }
+  if (mask & TH_CHANGE_TRIANGLES) {
+    if ((rc = upload(ctx, s->d_tris, d->triangles, d->triangle_count)) ||
+        (rc = upload(ctx, s->d_attrs, d->triangle_attributes, d->triangle_count)))
+      return rc;
+    geometry = true;
+  }
+  if (mask & TH_CHANGE_MATERIALS)
+    if ((rc = upload(ctx, s->d_mats, d->materials, d->material_count))) return rc;
+  if (mask & TH_CHANGE_LIGHTS)
+    if ((rc = upload(ctx, s->d_lights, d->lights, d->light_count))) return rc;
+  if (mask & TH_CHANGE_BVH) {
+    if ((rc = upload(ctx, s->d_nodes, d->bvh_nodes, d->bvh_node_count)) ||
+        (rc = upload(ctx, s->d_prims, d->bvh_primitives, d->bvh_primitive_count)))
+      return rc;
+    geometry = true;
+  }
+  s->params = d->params;
+  if (geometry) {
+    if ((rc = prepare_bvh(s, d))) return rc;
+  }
+  TH_CHECK(hipStreamSynchronize(ctx->stream));
+  return TH_OK;
+}
+
+void th_scene_destroy(th_scene* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  free_scene_buffers(s);
+  delete s;
+}
+
+int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                  const th_render_options* opt) {
+  if (!ctx || !s || !cam) return th_set_error(TH_E_INVALID, "null argument");
+  if (s->ctx != ctx) return th_set_error(TH_E_INVALID, "scene belongs to another context");
+  int32_t rc = check_render_params(s, p);
+  if (rc) return rc;
+  th_render_options o;
+  memset(&o, 0, sizeof o);
+  if (opt) o = *opt;
+  if (o.rng_mode > 1u || o.traversal > 1u || o.kernel > 1u)
+    return th_set_error(TH_E_INVALID, "unknown render option");
+  if (o.kernel == TH_KERNEL_WAVEFRONT)
+    return th_set_error(TH_E_UNSUPPORTED, "wavefront kernel not built");
+  TH_CHECK(hipSetDevice(ctx->device));
+  DevScene S = dev_scene(s);
+  uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
+  const uint32_t block = 64;
+  size_t lds = (size_t)depth * block * sizeof(uint2);
+  if (lds > 64 * 1024) return th_set_error(TH_E_UNSUPPORTED, "bvh too deep for the traversal stack");
+  if (o.collect_stats) TH_CHECK(hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
+  unsigned long long* cnt = ctx->d_counters;
+  std::pair<hipEvent_t, hipEvent_t> evp;
+  if ((rc = begin_timed(ctx, &evp))) return rc;
+  if (o.rng_mode == TH_RNG_REFERENCE) {
+    uint32_t tiles = ((p->width + 7) / 8) * ((p->height + 7) / 8);
+    dim3 grid(tiles), blk(block);
+#define TH_LAUNCH(TM, C) \
+  hipLaunchKernelGGL((k_render_pixel<TM, C>), grid, blk, lds, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cnt)
+    if (o.traversal == 0 && !o.collect_stats) TH_LAUNCH(0, false);
+    else if (o.traversal == 0) TH_LAUNCH(0, true);
+    else if (!o.collect_stats) TH_LAUNCH(1, false);
+    else TH_LAUNCH(1, true);
+#undef TH_LAUNCH
+  } else {
+    // G lanes per pixel, G = min(samples, 8) rounded down to a power of two
+    int G = p->samples >= 8 ? 8 : (p->samples >= 4 ? 4 : (p->samples >= 2 ? 2 : 1));
+    uint32_t ppw = 64u / (uint32_t)G;
+    uint32_t th_ = 8u / (uint32_t)std::min(G, 8), tw = ppw / th_;
+    uint32_t waves = ((p->width + tw - 1) / tw) * ((p->height + th_ - 1) / th_);
+    dim3 grid(waves), blk(block);
+#define TH_LAUNCH(TM, C, GG) \
+  hipLaunchKernelGGL((k_render_sample<TM, C, GG>), grid, blk, lds, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cnt)
+#define TH_LAUNCH_G(TM, C)            \
+  switch (G) {                        \
+    case 8: TH_LAUNCH(TM, C, 8); break; \
+    case 4: TH_LAUNCH(TM, C, 4); break; \
+    case 2: TH_LAUNCH(TM, C, 2); break; \
+    default: TH_LAUNCH(TM, C, 1); break; \
+  }
+    if (o.traversal == 0 && !o.collect_stats) { TH_LAUNCH_G(0, false) }
+    else if (o.traversal == 0) { TH_LAUNCH_G(0, true) }
+    else if (!o.collect_stats) { TH_LAUNCH_G(1, false) }
+    else { TH_LAUNCH_G(1, true) }
+#undef TH_LAUNCH_G
+#undef TH_LAUNCH
+  }
+  hipError_t le = hipGetLastError();
+  if ((rc = end_timed(ctx, evp))) return rc;
+  if (le != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("render launch: ") + hipGetErrorString(le));
+  ctx->counters_pending = o.collect_stats != 0;
+  ctx->stats.paths = (uint64_t)p->width * p->height * p->samples;
+  return TH_OK;
+}
+
+int32_t th_synchronize(th_context* ctx) {
+  if (!ctx) return th_set_error(TH_E_INVALID, "null argument");
+  TH_CHECK(hipSetDevice(ctx->device));
+  TH_CHECK(hipStreamSynchronize(ctx->stream));
+  return TH_OK;
+}
+
+int32_t th_readback(th_context* ctx, th_scene* s, float* rgba, size_t bytes) {
+  if (!ctx || !s || !rgba) return th_set_error(TH_E_INVALID, "null argument");
+  size_t need = (size_t)s->width * s->height * 4 * sizeof(float);
+  if (bytes != need) return th_set_error(TH_E_INVALID, "readback buffer size mismatch");
+  TH_CHECK(hipSetDevice(ctx->device));
+  TH_CHECK(hipMemcpyAsync(rgba, s->d_accum, need, hipMemcpyDeviceToHost, ctx->stream));
+  TH_CHECK(hipStreamSynchronize(ctx->stream));
+  return TH_OK;
+}
+
+int32_t th_upload_accumulation(th_context* ctx, th_scene* s, const float* rgba, size_t bytes) {
+  if (!ctx || !s || !rgba) return th_set_error(TH_E_INVALID, "null argument");
+  size_t need = (size_t)s->width * s->height * 4 * sizeof(float);
+  if (bytes != need) return th_set_error(TH_E_INVALID, "accumulation buffer size mismatch");
+  TH_CHECK(hipSetDevice(ctx->device));
+  TH_CHECK(hipMemcpyAsync(s->d_accum, rgba, need, hipMemcpyHostToDevice, ctx->stream));
+  TH_CHECK(hipStreamSynchronize(ctx->stream));
+  return TH_OK;
+}
+
+int32_t th_get_stats(const th_context* cctx, th_stats* out) {
+  if (!cctx || !out) return th_set_error(TH_E_INVALID, "null argument");
+  th_context* ctx = const_cast<th_context*>(cctx);
+  TH_CHECK(hipSetDevice(ctx->device));
+  int32_t rc = fold_timing(ctx);
+  if (rc) return rc;
+  if (ctx->counters_pending) {
+    unsigned long long h[16];
+    TH_CHECK(hipMemcpy(h, ctx->d_counters, sizeof h, hipMemcpyDeviceToHost));
+    th_stats& s = ctx->stats;
+    s.paths = h[0];
+    s.closest_rays = h[1];
+    s.shadow_rays = h[2];
+    s.rays = h[1] + h[2];
+    s.nodes_visited = h[3];
+    s.primref_tests = h[4];
+    s.triangle_tests = h[5];
+    s.sphere_tests = h[6];
+    s.closest_hits = h[7];
+    s.triangle_hits = h[8];
+    s.lights_scanned = h[9];
+    s.env_lookups = h[10];
+    s.env_samples = h[11];
+    ctx->counters_pending = false;
+  }
+  *out = ctx->stats;
+  return TH_OK;
+}
+
+int32_t th_reset_stats(th_context* ctx) {
+  if (!ctx) return th_set_error(TH_E_INVALID, "null argument");
+  TH_CHECK(hipSetDevice(ctx->device));
+  int32_t rc = fold_timing(ctx);
+  if (rc) return rc;
+  memset(&ctx->stats, 0, sizeof ctx->stats);
+  return TH_OK;
+}
+
+int32_t th_primary_hits(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
+                        const TH_GpuCamera* cam, uint32_t traversal, uint32_t* slot_out,
+                        float* distance_out, size_t count) {
+  if (!ctx || !s || !cam || !slot_out || !distance_out)
+    return th_set_error(TH_E_INVALID, "null argument");
+  int32_t rc = check_render_params(s, p);
+  if (rc) return rc;
+  size_t npix = (size_t)p->width * p->height;
+  if (count != npix) return th_set_error(TH_E_INVALID, "output size must equal width*height");
+  if (traversal > 1u) return th_set_error(TH_E_INVALID, "unknown traversal");
+  TH_CHECK(hipSetDevice(ctx->device));
+  uint32_t* d_slot = nullptr;
+  float* d_t = nullptr;
+  TH_CHECK(hipMalloc((void**)&d_slot, npix * sizeof(uint32_t)));
+  hipError_t e = hipMalloc((void**)&d_t, npix * sizeof(float));
+  if (e != hipSuccess) {
+    (void)hipFree(d_slot);
+    return th_set_error(TH_E_DEVICE, "hipMalloc");
+  }
+  DevScene S = dev_scene(s);
+  uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
+  size_t lds = (size_t)depth * 64 * sizeof(uint2);
+  uint32_t tiles = ((p->width + 7) / 8) * ((p->height + 7) / 8);
+  if (traversal == 0)
+    hipLaunchKernelGGL(k_primary<0>, dim3(tiles), dim3(64), lds, ctx->stream, S, *p, *cam, d_slot, d_t);
+  else
+    hipLaunchKernelGGL(k_primary<1>, dim3(tiles), dim3(64), lds, ctx->stream, S, *p, *cam, d_slot, d_t);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(slot_out, d_slot, npix * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(distance_out, d_t, npix * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_slot);
+  (void)hipFree(d_t);
+  if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("primary hits: ") + hipGetErrorString(e));
+  return TH_OK;
+}
+
+int32_t th_display_rgba8_device(th_context* ctx, th_scene* s, float exposure_stops, uint8_t* out,
+                                size_t out_bytes) {
+  if (!ctx || !s || !out) return th_set_error(TH_E_INVALID, "null argument");
+  size_t npix = (size_t)s->width * s->height;
+  if (out_bytes != npix * 4) return th_set_error(TH_E_INVALID, "output size must be width*height*4");
+  TH_CHECK(hipSetDevice(ctx->device));
+  uchar4* d_out = nullptr;
+  TH_CHECK(hipMalloc((void**)&d_out, npix * 4));
+  float scale = exp2f(exposure_stops);
+  hipLaunchKernelGGL(k_display, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, s->d_accum,
+                     (uint32_t)npix, scale, ctx->d_srgb_lut + 256, d_out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, npix * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("display: ") + hipGetErrorString(e));
+  return TH_OK;
+}
+
+}  // extern "C"

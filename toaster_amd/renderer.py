@@ -1,0 +1,185 @@
+"""GPU renderer over the C ABI, shaped like toaster-gpu's renderer interface.
+
+Mirrors crates/toaster-gpu/src/path_tracer.rs: a context (device.rs:49-100), a scene
+uploaded once with fixed counts (buffers.rs:117-271, path_tracer.rs:639-653),
+per-frame dispatches that fold `samples` new samples into the progressive
+running average (wgsl:538-547, path_tracer.rs:655-674), linear-HDR readback
+(readback.rs:8-25) and the exposure/ACES/sRGB display stage (image_output.rs:59-92).
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi as A
+from .scene import PackedScene, Scene, pack
+
+
+@dataclass
+class RenderOptions:
+    rng_mode: int = A.TH_RNG_REFERENCE
+    kernel: int = A.TH_KERNEL_MEGA
+    traversal: int = A.TH_TRAVERSE_REFERENCE
+    collect_stats: bool = False
+
+    def struct(self):
+        o = A.RenderOptions()
+        o.rng_mode, o.kernel, o.traversal = self.rng_mode, self.kernel, self.traversal
+        o.collect_stats = 1 if self.collect_stats else 0
+        return o
+
+
+def device_count():
+    n = C.c_int32()
+    A.check(A.lib().th_device_count(C.byref(n)))
+    return n.value
+
+
+class GpuContext:
+    def __init__(self, device=0):
+        self._lib = A.lib()
+        self._h = C.c_void_p()
+        A.check(self._lib.th_context_create(int(device), C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        A.check(self._lib.th_synchronize(self._h))
+
+    def stats(self):
+        s = A.Stats()
+        A.check(self._lib.th_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in A.Stats._fields_}
+
+    def reset_stats(self):
+        A.check(self._lib.th_reset_stats(self._h))
+
+    def close(self):
+        if self._h:
+            self._lib.th_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuScene:
+    """An uploaded scene plus its accumulation buffer."""
+
+    def __init__(self, ctx: GpuContext, packed: PackedScene, node_capacity=0):
+        self.ctx = ctx
+        self.packed = packed
+        self._lib = ctx._lib
+        self._h = C.c_void_p()
+        d = packed.desc()
+        A.check(self._lib.th_scene_create(ctx.handle, C.byref(d), int(node_capacity), C.byref(self._h)))
+        self.width, self.height = packed.params.width, packed.params.height
+        self.accumulated_samples = 0
+        self.frame_index = 0
+
+    @classmethod
+    def from_scene(cls, ctx, scene: Scene, node_capacity=0):
+        return cls(ctx, pack(scene), node_capacity)
+
+    def update(self, packed: PackedScene, change_mask):
+        d = packed.desc()
+        A.check(self._lib.th_scene_update(self._h, int(change_mask), C.byref(d)))
+        self.packed = packed
+
+    def params(self, samples=None, frame_index=None, accumulated_samples=None, max_bounces=None):
+        p = A.GpuRenderParams.from_buffer_copy(self.packed.params)
+        if samples is not None:
+            p.samples = samples
+        if max_bounces is not None:
+            p.max_bounces = max_bounces
+        p.frame_index = self.frame_index if frame_index is None else frame_index
+        p.accumulated_samples = (self.accumulated_samples if accumulated_samples is None
+                                 else accumulated_samples)
+        return p
+
+    def render(self, samples=None, frame_index=None, accumulated_samples=None,
+               options: RenderOptions = None, camera=None, max_bounces=None):
+        """One asynchronous dispatch (dispatch_compute_2d without the wait)."""
+        p = self.params(samples, frame_index, accumulated_samples, max_bounces)
+        cam = self.packed.camera if camera is None else camera
+        o = (options or RenderOptions()).struct()
+        A.check(self._lib.th_render(self.ctx.handle, self._h, C.byref(p), C.byref(cam), C.byref(o)))
+        return p
+
+    def render_progressive(self, samples, options=None):
+        """Progressive batch: frame seed = batch counter, running average continues."""
+        p = self.render(samples=samples, frame_index=self.frame_index,
+                        accumulated_samples=self.accumulated_samples, options=options)
+        self.accumulated_samples += p.samples
+        self.frame_index += 1
+        return p
+
+    def readback(self):
+        out = np.empty((self.height, self.width, 4), np.float32)
+        A.check(self._lib.th_readback(self.ctx.handle, self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def upload_accumulation(self, rgba, accumulated_samples, frame_index):
+        """Restores a progressive checkpoint (accum buffer, sample count, next seed)."""
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        A.check(self._lib.th_upload_accumulation(self.ctx.handle, self._h, rgba.ctypes.data, rgba.nbytes))
+        self.accumulated_samples = int(accumulated_samples)
+        self.frame_index = int(frame_index)
+
+    def display_rgba8(self, exposure_stops=0.0):
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        A.check(self._lib.th_display_rgba8_device(self.ctx.handle, self._h, float(exposure_stops),
+                                                   out.ctypes.data, out.nbytes))
+        return out
+
+    def primary_hits(self, traversal=A.TH_TRAVERSE_REFERENCE, frame_index=0):
+        n = self.width * self.height
+        slots = np.empty(n, np.uint32)
+        dist = np.empty(n, np.float32)
+        p = self.params(frame_index=frame_index)
+        A.check(self._lib.th_primary_hits(self.ctx.handle, self._h, C.byref(p),
+                                          C.byref(self.packed.camera), int(traversal),
+                                          slots.ctypes.data, dist.ctypes.data, n))
+        return slots, dist
+
+    def close(self):
+        if self._h:
+            self._lib.th_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def display_rgba8(pixels, exposure_stops=0.0):
+    """Host display conversion of linear-HDR float RGBA (pixels_to_rgba_image)."""
+    pixels = np.ascontiguousarray(pixels, np.float32)
+    h, w = pixels.shape[0], pixels.shape[1]
+    out = np.empty((h, w, 4), np.uint8)
+    A.check(A.lib().th_display_rgba8(pixels.ctypes.data, w, h, float(exposure_stops),
+                                     out.ctypes.data, out.nbytes))
+    return out
+
+
+def render(scene: Scene, options: RenderOptions = None, device=0):
+    """Renders one frame of a scene (frame seed 0) and returns linear-HDR (H, W, 4)."""
+    with GpuContext(device) as ctx:
+        gs = GpuScene.from_scene(ctx, scene)
+        gs.render(options=options)
+        img = gs.readback()
+        gs.close()
+        return img
