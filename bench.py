@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--triangles", type=int, default=65536)
-    ap.add_argument("--traversal", choices=["reference", "ordered"], default="ordered")
+    ap.add_argument("--traversal", choices=["reference", "ordered", "wide"], default="wide")
     ap.add_argument("--rng", choices=["reference", "per_sample"], default="reference")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -91,13 +91,16 @@ def main():
 
     from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
     sc = scenes.bvh_stress(args.triangles)
-    trav = A.TH_TRAVERSE_ORDERED if args.traversal == "ordered" else A.TH_TRAVERSE_REFERENCE
+    trav = {"reference": A.TH_TRAVERSE_REFERENCE, "ordered": A.TH_TRAVERSE_ORDERED,
+            "wide": A.TH_TRAVERSE_WIDE}[args.traversal]
     rng = A.TH_RNG_REFERENCE if args.rng == "reference" else A.TH_RNG_PER_SAMPLE
     opts = RenderOptions(rng_mode=rng, traversal=trav)
     ctx = GpuContext(0)
     gs = GpuScene.from_scene(ctx, sc)
-    # one counting pass (outside the timed region) for rays per frame
-    gs.render(options=RenderOptions(rng_mode=rng, traversal=trav, collect_stats=True))
+    # one counting pass (outside the timed region) for rays per frame; rays do not
+    # depend on the traversal order, so the scalar ordered traversal counts them
+    gs.render(options=RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_ORDERED,
+                                    collect_stats=True))
     st = ctx.stats()
     rays_per_frame = st["rays"]
     for _ in range(args.warmup):
@@ -110,6 +113,24 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     st = ctx.stats()
+    # other kernel variants on the same frame (short runs, reported beside the headline)
+    variants = {}
+    for name, o in (("megakernel_reference_traversal", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_REFERENCE)),
+                    ("megakernel_ordered", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_ORDERED)),
+                    ("quad_wide", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE)),
+                    ("quad_wide_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                               traversal=A.TH_TRAVERSE_WIDE)),
+                    ("megakernel_ordered_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                                        traversal=A.TH_TRAVERSE_ORDERED))):
+        gs.render(options=o, frame_index=0, accumulated_samples=0)
+        ctx.synchronize()
+        ctx.reset_stats()
+        n = 10
+        for _ in range(n):
+            gs.render(options=o, frame_index=0, accumulated_samples=0)
+        vs = ctx.stats()
+        kms = vs["total_kernel_ms"] / max(1, vs["launches"])
+        variants[name] = {"kernel_ms": round(kms, 4), "mpaths_s": round(sc.paths_per_frame / kms / 1e3, 2)}
     gs.close()
     ctx.close()
     dt = t1 - t0
@@ -132,7 +153,8 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes_per_launch": bytes_launch,
-                    "kernel": "k_render_pixel", "kernel_ms": round(kernel_ms, 5)}
+                    "kernel": "k_render_quad" if args.traversal == "wide" else "k_render_pixel",
+                    "kernel_ms": round(kernel_ms, 5)}
     line = {
         "metric": "Mpaths/s (plain HIP tracer, BVH-stress workload)",
         "value": round(args.steps * paths / dt / 1e6, 3),
@@ -143,10 +165,12 @@ def main():
                                "1 emissive sphere, 320x180, 4 spp, 4 bounces, median-split BVH",
                    "triangles": args.triangles, "width": sc.width, "height": sc.height,
                    "spp": sc.samples, "max_bounces": sc.max_bounces, "paths_per_step": paths,
-                   "rng": args.rng, "traversal": args.traversal, "kernel": "megakernel"},
+                   "rng": args.rng, "traversal": args.traversal,
+                   "kernel": "quad" if args.traversal == "wide" else "megakernel"},
         "rays_per_s": round(args.steps * rays_per_frame / dt, 1),
         "kernel_ms_mean": round(kernel_ms, 5),
         "roofline": roofline,
+        "variants": variants,
         "cpu_baseline": cpu,
         "host": platform.node(),
     }

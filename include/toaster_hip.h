@@ -246,6 +246,8 @@ typedef struct th_scene th_scene;
 /* Traversal order for closest-hit queries. */
 #define TH_TRAVERSE_REFERENCE 0 /* visit order of wgsl:746-792 (right child first) */
 #define TH_TRAVERSE_ORDERED 1   /* nearer child first, reference tie-breaking      */
+#define TH_TRAVERSE_WIDE 2      /* 4 lanes per path over the 4-wide collapsed tree,
+                                   nearer children first, reference tie-breaking   */
 
 typedef struct th_render_options {
     uint32_t rng_mode;       /* TH_RNG_*                                         */

@@ -56,6 +56,8 @@ _SIGS = {
     "or_last_error": (C.c_char_p, []),
     "or_render": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), U, U,
                               C.c_int32, P, C.POINTER(Counters)]),
+    "or_render_ex": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), U, U,
+                                 C.c_int32, U, U, P, C.POINTER(Counters)]),
     "or_primary_hits": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), P, P]),
     "or_closest": (C.c_int32, [P, F3, F3, C.c_int32, C.POINTER(Hit)]),
     "or_intersect_sphere": (C.c_int32, [F3, F3, F3, F, F, F, C.POINTER(Hit)]),
@@ -122,15 +124,18 @@ class OracleScene:
     def camera(self):
         return self.packed.camera
 
-    def render(self, params=None, rows=None, threads=1, accum=None, counters=False):
-        """pcg-mode render of rows [r0, r1); returns (rgba (H, W, 4), counters dict|None)."""
+    def render(self, params=None, rows=None, threads=1, accum=None, counters=False, rng_mode=0,
+               group=1):
+        """pcg-mode render of rows [r0, r1); returns (rgba (H, W, 4), counters dict|None).
+        rng_mode 1: per-sample streams with `group` partial sums (the GPU's per-sample mode)."""
         p = params if params is not None else self.params
         h, w = p.height, p.width
         out = np.zeros((h, w, 4), np.float32) if accum is None else np.ascontiguousarray(accum, np.float32).copy()
         r0, r1 = (0, h) if rows is None else rows
         cnt = Counters()
-        _check(lib().or_render(self._h, C.byref(p), C.byref(self.camera), r0, r1, int(threads),
-                               out.ctypes.data, C.byref(cnt) if counters else None))
+        _check(lib().or_render_ex(self._h, C.byref(p), C.byref(self.camera), r0, r1, int(threads),
+                                  int(rng_mode), int(group), out.ctypes.data,
+                                  C.byref(cnt) if counters else None))
         return out, (cnt.as_dict() if counters else None)
 
     def primary_hits(self, params=None):

@@ -1014,6 +1014,13 @@ void or_free(or_scene* s) { delete s; }
 int32_t or_render(const or_scene* s, const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
                   uint32_t row_begin, uint32_t row_end, int32_t threads, float* rgba,
                   or_counters* counters) {
+  return or_render_ex(s, params, camera, row_begin, row_end, threads, 0u, 1u, rgba, counters);
+}
+
+int32_t or_render_ex(const or_scene* s, const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
+                     uint32_t row_begin, uint32_t row_end, int32_t threads, uint32_t rng_mode,
+                     uint32_t group, float* rgba, or_counters* counters) {
+  if (group == 0u || group > 64u || (group & (group - 1u)) != 0u) return fail("group must be a power of two <= 64");
   if (!s || !params || !camera || !rgba) return fail("null argument");
   const TH_GpuRenderParams P = *params;
   if (P.samples == 0) return fail("samples must be nonzero");
@@ -1036,16 +1043,48 @@ int32_t or_render(const or_scene* s, const TH_GpuRenderParams* params, const TH_
     Tracer tr{*s, P, c};
     uint32_t y = row_begin + (uint32_t)(k / P.width), x = (uint32_t)(k % P.width);
     uint32_t idx = y * P.width + x;
-    uint32_t st = th::pcg_hash(idx ^ (P.frame_index * 9781u));  // wgsl:519
+    uint32_t base = idx ^ (P.frame_index * 9781u);
+    uint32_t st = th::pcg_hash(base);  // wgsl:519
     float cr = 0, cg = 0, cb = 0, ca = 0;
-    for (uint32_t i = 0; i < P.samples; ++i) {
-      Ray r = camera_ray(cam, P, x, y, &st);
-      v3 L = tr.ray_color(r, &st);
-      cr = cr + L.x;
-      cg = cg + L.y;
-      cb = cb + L.z;
-      ca = ca + 1.0f;
-      if (c) c->paths++;
+    if (rng_mode == 0u) {
+      for (uint32_t i = 0; i < P.samples; ++i) {
+        Ray r = camera_ray(cam, P, x, y, &st);
+        v3 L = tr.ray_color(r, &st);
+        cr = cr + L.x;
+        cg = cg + L.y;
+        cb = cb + L.z;
+        ca = ca + 1.0f;
+        if (c) c->paths++;
+      }
+    } else {
+      // per-sample streams pcg_hash(base + s * 0x9E3779B9); group j sums samples
+      // j, j+G, ... and the G partial sums combine as a butterfly (xor 1, 2, 4, ...)
+      float part[64][4];
+      for (uint32_t j = 0; j < group; ++j) {
+        part[j][0] = part[j][1] = part[j][2] = part[j][3] = 0.0f;
+        for (uint32_t i = j; i < P.samples; i += group) {
+          uint32_t ss = th::pcg_hash(base + i * 0x9E3779B9u);
+          Ray r = camera_ray(cam, P, x, y, &ss);
+          v3 L = tr.ray_color(r, &ss);
+          part[j][0] = part[j][0] + L.x;
+          part[j][1] = part[j][1] + L.y;
+          part[j][2] = part[j][2] + L.z;
+          part[j][3] = part[j][3] + 1.0f;
+          if (c) c->paths++;
+        }
+      }
+      for (uint32_t off = 1; off < group; off <<= 1)
+        for (uint32_t j = 0; j < group; ++j)
+          if ((j & off) == 0u)
+            for (int k = 0; k < 4; ++k) {
+              float a = part[j][k] + part[j ^ off][k];
+              part[j][k] = a;
+              part[j ^ off][k] = a;
+            }
+      cr = part[0][0];
+      cg = part[0][1];
+      cb = part[0][2];
+      ca = part[0][3];
     }
     float ns = (float)P.samples;
     float b0 = cr / ns, b1 = cg / ns, b2 = cb / ns, b3 = ca / ns;

@@ -51,6 +51,12 @@ int32_t or_render(const or_scene* s, const TH_GpuRenderParams* params, const TH_
                   uint32_t row_begin, uint32_t row_end, int32_t threads, float* rgba,
                   or_counters* counters);
 
+/* As or_render; rng_mode 1 = per-sample streams pcg_hash((idx ^ frame*9781) + s*0x9E3779B9)
+ * with samples split over `group` partial sums combined as a butterfly. */
+int32_t or_render_ex(const or_scene* s, const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
+                     uint32_t row_begin, uint32_t row_end, int32_t threads, uint32_t rng_mode,
+                     uint32_t group, float* rgba, or_counters* counters);
+
 /* First camera ray of each pixel (draws 1-2 of its stream): closest hit slot + t. */
 int32_t or_primary_hits(const or_scene* s, const TH_GpuRenderParams* params,
                         const TH_GpuCamera* camera, uint32_t* slot_out, float* distance_out);

@@ -31,10 +31,15 @@ def _render(ctx, sc, opts=None, **kw):
     return img
 
 
-@pytest.mark.parametrize("traversal", [A.TH_TRAVERSE_REFERENCE, A.TH_TRAVERSE_ORDERED])
+TRAVERSALS = [A.TH_TRAVERSE_REFERENCE, A.TH_TRAVERSE_ORDERED, A.TH_TRAVERSE_WIDE]
+
+
+@pytest.mark.parametrize("traversal", TRAVERSALS)
 @pytest.mark.parametrize("name,make", [
     ("cornell", lambda: scenes.cornell()),
     ("stress2048", lambda: scenes.bvh_stress(2048)),
+    ("mixed_env", lambda: scenes.mixed_environment(width=160, height=90, samples=4, max_bounces=8,
+                                                   env_size=(256, 128))),
 ])
 def test_image_matches_oracle(gpu_ctx, name, make, traversal):
     sc = make()
@@ -47,12 +52,32 @@ def test_image_matches_oracle(gpu_ctx, name, make, traversal):
     assert frac >= TOL_FRAC
 
 
+@pytest.mark.parametrize("traversal", [A.TH_TRAVERSE_ORDERED, A.TH_TRAVERSE_WIDE])
+@pytest.mark.parametrize("samples", [1, 8, 16])
+def test_per_sample_rng_matches_oracle(gpu_ctx, traversal, samples):
+    sc = scenes.bvh_stress(2048, samples=samples)
+    print(f"case per-sample traversal={traversal} spp={samples}", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=traversal))
+    # lanes/quads per pixel used by the kernel: megakernel min(spp, 8), quad min(spp, 16)
+    cap = 16 if traversal == A.TH_TRAVERSE_WIDE else 8
+    group = 1
+    while group * 2 <= min(samples, cap):
+        group *= 2
+    ref, _ = oracle.OracleScene(sc).render(threads=8, rng_mode=1, group=group)
+    frac, exact = _compare(img, ref)
+    print(f"per-sample spp={samples} traversal={traversal}: within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert frac >= TOL_FRAC
+    if samples == 1:  # one sample: the per-sample stream is the reference stream
+        ref1, _ = oracle.OracleScene(sc).render(threads=8)
+        assert _compare(img, ref1)[0] >= TOL_FRAC
+
+
 def test_primary_hits_match(gpu_ctx):
     sc = scenes.bvh_stress(8192)
     gs = GpuScene.from_scene(gpu_ctx, sc)
     oc = oracle.OracleScene(sc)
     ref_slot, ref_t = oc.primary_hits()
-    for trav in (A.TH_TRAVERSE_REFERENCE, A.TH_TRAVERSE_ORDERED):
+    for trav in TRAVERSALS:
         slot, t = gs.primary_hits(traversal=trav)
         same = slot == ref_slot
         print(f"primary traversal={trav}: identical {same.mean():.6f}")

@@ -23,7 +23,7 @@ TH_MAT_DIFFUSE, TH_MAT_METAL, TH_MAT_DIELECTRIC, TH_MAT_EMISSIVE, TH_MAT_TEXTURE
 TH_BG_SKY, TH_BG_BLACK, TH_BG_ENVIRONMENT = 0, 1, 2
 TH_RNG_REFERENCE, TH_RNG_PER_SAMPLE = 0, 1
 TH_KERNEL_MEGA, TH_KERNEL_WAVEFRONT = 0, 1
-TH_TRAVERSE_REFERENCE, TH_TRAVERSE_ORDERED = 0, 1
+TH_TRAVERSE_REFERENCE, TH_TRAVERSE_ORDERED, TH_TRAVERSE_WIDE = 0, 1, 2
 
 TH_CHANGE_CAMERA = 1
 TH_CHANGE_SPHERES = 2

@@ -52,6 +52,8 @@ struct DevScene {
   uint32_t node_count;
   uint32_t root_entry;
   float root_min[3], root_max[3];
+  const float4* nodes4;  // 4-wide nodes (th_wide.h), 8 float4 per node
+  uint32_t root4;        // 4-wide traversal entry of the root
 };
 
 struct Counters {

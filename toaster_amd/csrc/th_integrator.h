@@ -7,7 +7,7 @@
 // (the scan of wgsl:394-400 stops at the same index), and the environment CDF search
 // is the same lower-bound search as wgsl:352-361.
 #pragma once
-#include "th_device.h"
+#include "th_wide.h"
 
 namespace thd {
 
@@ -60,8 +60,10 @@ struct Integrator {
       t = closest_linear<COUNT>(S, P.sphere_count, P.triangle_count, o, d, TH_FLT_MAX, rs, &slot);
     } else if (TMODE == 0) {
       traverse<0, COUNT>(S, o, d, TH_FLT_MAX, st, rs, &t, &slot);
-    } else {
+    } else if (TMODE == 1) {
       traverse<1, COUNT>(S, o, d, TH_FLT_MAX, st, rs, &t, &slot);
+    } else {
+      traverse_quad<1>(S, (const Child4*)S.nodes4, S.root4, o, d, TH_FLT_MAX, st, &t, &slot);
     }
     lc.add_ray(rs, false);
     if (slot == THD_NO_SLOT || t > TH_MAX_DISTANCE) return false;
@@ -87,8 +89,10 @@ struct Integrator {
       t = closest_linear<COUNT>(S, P.sphere_count, P.triangle_count, o, d, limit, rs, &slot);
     } else if (TMODE == 0) {
       traverse<0, COUNT>(S, o, d, TH_FLT_MAX, st, rs, &t, &slot);
-    } else {
+    } else if (TMODE == 1) {
       traverse<2, COUNT>(S, o, d, limit, st, rs, &t, &slot);
+    } else {
+      traverse_quad<2>(S, (const Child4*)S.nodes4, S.root4, o, d, limit, st, &t, &slot);
     }
     lc.add_ray(rs, true);
     return t < limit;

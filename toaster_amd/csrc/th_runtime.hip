@@ -207,6 +207,88 @@ __global__ void __launch_bounds__(64) k_primary(DevScene S, TH_GpuRenderParams P
   t_out[idx] = t;
 }
 
+// Quad kernels: four lanes own one path (th_wide.h).  G quads share a pixel in the
+// per-sample mode (sample j, j+G, ... per quad); in the reference RNG mode G = 1 and the
+// quad runs the pixel's samples in stream order.  A wave covers 16/G pixels.
+template <bool PER_SAMPLE, int G>
+__global__ void __launch_bounds__(64) k_render_quad(DevScene S, TH_GpuRenderParams P,
+                                                    TH_GpuCamera cam, float4* __restrict__ accum,
+                                                    const float* __restrict__ srgb_lut) {
+  extern __shared__ uint2 lds_stack[];
+  constexpr uint32_t PPW = 16u / G;
+  constexpr uint32_t TW = PPW >= 8u ? 4u : (PPW >= 2u ? 2u : 1u);
+  constexpr uint32_t TH = PPW / TW;
+  const uint32_t lane = threadIdx.x & 63u, qw = lane >> 2;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t pw = qw / G, j = qw % G;
+  const uint32_t tiles_x = (P.width + TW - 1u) / TW;
+  const uint32_t x = (wave % tiles_x) * TW + pw % TW;
+  const uint32_t y = (wave / tiles_x) * TH + pw / TW;
+  const bool active = x < P.width && y < P.height;
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  LaneCounters<false> lc;
+  Integrator<3, false> it{S, P, srgb_lut, st, lc};
+  const uint32_t idx = y * P.width + x;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
+  if (active) {
+    uint32_t base = idx ^ (P.frame_index * 9781u);
+    uint32_t s = th::pcg_hash(base);
+    for (uint32_t smp = PER_SAMPLE ? j : 0u; smp < P.samples; smp += PER_SAMPLE ? G : 1u) {
+      if (PER_SAMPLE) s = th::pcg_hash(base + smp * 0x9E3779B9u);
+      v3 o, d;
+      camera_ray(cam, P, x, y, &s, &o, &d);
+      v3 L = it.ray_color(o, d, &s);
+      cr = cr + L.x;
+      cg = cg + L.y;
+      cb = cb + L.z;
+      ca = ca + 1.0f;
+    }
+  }
+#pragma unroll
+  for (int off = 4; off < 4 * G; off <<= 1) {
+    cr = cr + __shfl_xor(cr, off, 64);
+    cg = cg + __shfl_xor(cg, off, 64);
+    cb = cb + __shfl_xor(cb, off, 64);
+    ca = ca + __shfl_xor(ca, off, 64);
+  }
+  if (active && (lane % (4u * G)) == 0u) {
+    float ns = (float)P.samples;
+    float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
+    if (P.accumulated_samples == 0u) {
+      accum[idx] = b;
+    } else {
+      float prev = (float)P.accumulated_samples;
+      float tot = prev + ns;
+      float4 o = accum[idx];
+      accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                               (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_primary_quad(DevScene S, TH_GpuRenderParams P,
+                                                     TH_GpuCamera cam, uint32_t* __restrict__ slot_out,
+                                                     float* __restrict__ t_out) {
+  extern __shared__ uint2 lds_stack[];
+  const uint32_t lane = threadIdx.x & 63u, qw = lane >> 2;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t tiles_x = (P.width + 3u) / 4u;
+  const uint32_t x = (wave % tiles_x) * 4u + (qw & 3u), y = (wave / tiles_x) * 4u + (qw >> 2);
+  if (x >= P.width || y >= P.height) return;
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  const uint32_t idx = y * P.width + x;
+  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  v3 o, d;
+  camera_ray(cam, P, x, y, &s, &o, &d);
+  float t;
+  uint32_t slot;
+  traverse_quad<1>(S, (const Child4*)S.nodes4, S.root4, o, d, TH_FLT_MAX, st, &t, &slot);
+  if ((lane & 3u) == 0u) {
+    slot_out[idx] = slot;
+    t_out[idx] = t;
+  }
+}
+
 // Display conversion on the device (color.rs:42-55, image_output.rs:59-92).
 __global__ void k_display(const float4* __restrict__ accum, uint32_t n, float scale,
                           const float* __restrict__ thr, uchar4* __restrict__ out) {
@@ -244,6 +326,8 @@ struct th_scene {
   float4* d_env = nullptr;
   uint32_t* d_tex = nullptr;
   CNode* d_cnodes = nullptr;
+  Child4* d_nodes4 = nullptr;
+  uint32_t root4 = 0, depth4 = 0;
   float4* d_slots = nullptr;
   float4* d_accum = nullptr;
   uint32_t node_count = 0, prim_count = 0, env_count = 0, tex_count = 0;
@@ -280,7 +364,8 @@ int32_t upload(th_context* ctx, T* dst, const T* src, size_t count) {
 
 void free_scene_buffers(th_scene* s) {
   void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
-                  s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum};
+                  s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum,
+                  s->d_nodes4};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -376,6 +461,66 @@ int32_t validate_desc(const th_scene_desc* d) {
   return validate_bvh(d);
 }
 
+// Collapses the reference binary tree into the 4-wide tree of th_wide.h (breadth-first
+// node order, so the upper levels are contiguous): the children of a 4-wide node are
+// the children of a binary interior node, with each interior child replaced by its own
+// two children.  Returns the root entry and the number of 4-wide levels.
+float host_bits_to_float(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out, uint32_t* root4,
+                uint32_t* depth4) {
+  out.clear();
+  *depth4 = 0;
+  if (n == 0) {
+    *root4 = 0;
+    return;
+  }
+  if (nodes[0].primitive_count > 0) {
+    *root4 = th_entry_of(nodes[0], 0u);
+    return;
+  }
+  std::vector<std::pair<uint32_t, uint32_t>> q;  // (binary node, level)
+  q.push_back({0u, 1u});
+  for (size_t qi = 0; qi < q.size(); ++qi) {
+    uint32_t b = q[qi].first, lvl = q[qi].second;
+    *depth4 = std::max(*depth4, lvl);
+    uint32_t kids[4], nk = 0;
+    uint32_t bc[2] = {b + 1u, nodes[b].first_or_right};
+    for (uint32_t c : bc) {
+      if (nodes[c].primitive_count == 0) {
+        kids[nk++] = c + 1u;
+        kids[nk++] = nodes[c].first_or_right;
+      } else {
+        kids[nk++] = c;
+      }
+    }
+    for (uint32_t k = 0; k < 4; ++k) {
+      Child4 ch;
+      if (k < nk) {
+        const TH_GpuBvhNode& g = nodes[kids[k]];
+        uint32_t entry;
+        if (g.primitive_count == 0) {
+          entry = (uint32_t)q.size();
+          q.push_back({kids[k], lvl + 1u});
+        } else {
+          entry = th_entry_of(g, kids[k]);
+        }
+        ch.a = make_float4(g.min[0], g.min[1], g.min[2], host_bits_to_float(entry));
+        ch.b = make_float4(g.max[0], g.max[1], g.max[2], 0.0f);
+      } else {
+        ch.a = make_float4(0, 0, 0, host_bits_to_float(THD_NO_CHILD));
+        ch.b = make_float4(0, 0, 0, 0);
+      }
+      out.push_back(ch);
+    }
+  }
+  *root4 = 0u;
+}
+
 int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
   th_context* ctx = s->ctx;
   s->node_count = d->bvh_node_count;
@@ -388,6 +533,11 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
       s->root_max[k] = r.max[k];
     }
     s->max_depth = bvh_depth(d->bvh_nodes, d->bvh_node_count);
+    std::vector<Child4> w;
+    build_wide(d->bvh_nodes, d->bvh_node_count, w, &s->root4, &s->depth4);
+    if (!w.empty()) {
+      TH_CHECK(hipMemcpy(s->d_nodes4, w.data(), w.size() * sizeof(Child4), hipMemcpyHostToDevice));
+    }
     hipLaunchKernelGGL(k_prepare_cnodes, dim3((s->node_count + 255) / 256), dim3(256), 0, ctx->stream,
                        s->d_nodes, s->node_count, s->d_cnodes);
     TH_CHECK(hipGetLastError());
@@ -414,6 +564,8 @@ DevScene dev_scene(const th_scene* s) {
   S.tex = s->d_tex;
   S.node_count = s->node_count;
   S.root_entry = s->root_entry;
+  S.nodes4 = (const float4*)s->d_nodes4;
+  S.root4 = s->root4;
   for (int k = 0; k < 3; ++k) {
     S.root_min[k] = s->root_min[k];
     S.root_max[k] = s->root_max[k];
@@ -563,7 +715,8 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_c
       (rc = dalloc(&s->d_attrs, d->triangle_count)) || (rc = dalloc(&s->d_mats, d->material_count)) ||
       (rc = dalloc(&s->d_lights, d->light_count)) || (rc = dalloc(&s->d_env, d->environment_pixel_count)) ||
       (rc = dalloc(&s->d_tex, d->texture_pixel_count)) || (rc = dalloc(&s->d_cnodes, cap)) ||
-      (rc = dalloc(&s->d_slots, 3 * (size_t)nprim)) || (rc = dalloc(&s->d_accum, npix))) {
+      (rc = dalloc(&s->d_slots, 3 * (size_t)nprim)) || (rc = dalloc(&s->d_accum, npix)) ||
+      (rc = dalloc(&s->d_nodes4, 4 * (size_t)cap))) {
     free_scene_buffers(s);
     delete s;
     return rc;
@@ -653,8 +806,10 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   th_render_options o;
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
-  if (o.rng_mode > 1u || o.traversal > 1u || o.kernel > 1u)
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u)
     return th_set_error(TH_E_INVALID, "unknown render option");
+  if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
+    return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
   if (o.kernel == TH_KERNEL_WAVEFRONT)
     return th_set_error(TH_E_UNSUPPORTED, "wavefront kernel not built");
   TH_CHECK(hipSetDevice(ctx->device));
@@ -667,7 +822,30 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   unsigned long long* cnt = ctx->d_counters;
   std::pair<hipEvent_t, hipEvent_t> evp;
   if ((rc = begin_timed(ctx, &evp))) return rc;
-  if (o.rng_mode == TH_RNG_REFERENCE) {
+  if (o.traversal == TH_TRAVERSE_WIDE && s->node_count > 0) {
+    uint32_t depth = 3u * s->depth4 + 2u;
+    size_t lds4 = (size_t)depth * (block / 4) * sizeof(uint2);
+    int G = o.rng_mode == TH_RNG_REFERENCE ? 1
+            : (p->samples >= 16 ? 16 : p->samples >= 8 ? 8 : p->samples >= 4 ? 4 : p->samples >= 2 ? 2 : 1);
+    uint32_t ppw = 16u / (uint32_t)G;
+    uint32_t tw = ppw >= 8 ? 4 : (ppw >= 2 ? 2 : 1), tht = ppw / tw;
+    uint32_t waves = ((p->width + tw - 1) / tw) * ((p->height + tht - 1) / tht);
+    dim3 grid(waves), blk(block);
+#define TH_LAUNCHQ(PS, GG) \
+  hipLaunchKernelGGL((k_render_quad<PS, GG>), grid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut)
+    if (o.rng_mode == TH_RNG_REFERENCE) {
+      TH_LAUNCHQ(false, 1);
+    } else {
+      switch (G) {
+        case 16: TH_LAUNCHQ(true, 16); break;
+        case 8: TH_LAUNCHQ(true, 8); break;
+        case 4: TH_LAUNCHQ(true, 4); break;
+        case 2: TH_LAUNCHQ(true, 2); break;
+        default: TH_LAUNCHQ(true, 1); break;
+      }
+    }
+#undef TH_LAUNCHQ
+  } else if (o.rng_mode == TH_RNG_REFERENCE) {
     uint32_t tiles = ((p->width + 7) / 8) * ((p->height + 7) / 8);
     dim3 grid(tiles), blk(block);
 #define TH_LAUNCH(TM, C) \
@@ -782,7 +960,7 @@ int32_t th_primary_hits(th_context* ctx, th_scene* s, const TH_GpuRenderParams* 
   if (rc) return rc;
   size_t npix = (size_t)p->width * p->height;
   if (count != npix) return th_set_error(TH_E_INVALID, "output size must equal width*height");
-  if (traversal > 1u) return th_set_error(TH_E_INVALID, "unknown traversal");
+  if (traversal > 2u) return th_set_error(TH_E_INVALID, "unknown traversal");
   TH_CHECK(hipSetDevice(ctx->device));
   uint32_t* d_slot = nullptr;
   float* d_t = nullptr;
@@ -796,7 +974,11 @@ int32_t th_primary_hits(th_context* ctx, th_scene* s, const TH_GpuRenderParams* 
   uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
   size_t lds = (size_t)depth * 64 * sizeof(uint2);
   uint32_t tiles = ((p->width + 7) / 8) * ((p->height + 7) / 8);
-  if (traversal == 0)
+  if (traversal == 2 && s->node_count > 0) {
+    size_t lds4 = (size_t)(3u * s->depth4 + 2u) * 16 * sizeof(uint2);
+    uint32_t waves = ((p->width + 3) / 4) * ((p->height + 3) / 4);
+    hipLaunchKernelGGL(k_primary_quad, dim3(waves), dim3(64), lds4, ctx->stream, S, *p, *cam, d_slot, d_t);
+  } else if (traversal == 0)
     hipLaunchKernelGGL(k_primary<0>, dim3(tiles), dim3(64), lds, ctx->stream, S, *p, *cam, d_slot, d_t);
   else
     hipLaunchKernelGGL(k_primary<1>, dim3(tiles), dim3(64), lds, ctx->stream, S, *p, *cam, d_slot, d_t);

@@ -1,0 +1,193 @@
+// th_wide.h -- cooperative 4-lane traversal over a 4-wide BVH (gfx950).
+//
+// A path is owned by a quad of lanes (lanes 4k..4k+3 of a wave).  The quad walks a
+// 4-wide hierarchy obtained by collapsing every interior node of the reference's
+// median-split binary tree with its interior children (same boxes, same leaves, same
+// leaf primitive order): at an interior node lane j tests child j's box, at a leaf lane
+// j tests primitive j, and the quad combines results with DPP quad permutes (no LDS
+// round trip).  Children are visited nearest first; the farther hit children go on a
+// per-quad LDS stack with their entry distance, re-checked against the current best
+// when popped.  The closest hit is resolved exactly as the reference resolves it:
+// smallest distance, ties to the reference's first-found primitive (leaves in
+// descending first-slot order, ascending slot within a leaf).
+//
+// Record layout (node4 array, 128 B per node): 4 children x {float4(min.xyz, entry),
+// float4(max.xyz, 0)}; an absent child has entry 0xffffffff.  Entries use the encoding
+// of th_device.h, interior entries index node4 records.
+#pragma once
+#include "th_device.h"
+
+namespace thd {
+
+#define THD_NO_CHILD 0xffffffffu
+
+struct Child4 {
+  float4 a;  // min.xyz, entry bits
+  float4 b;  // max.xyz, unused
+};
+
+// ------------------------------------------------------------------ quad exchange
+template <int K>
+__device__ __forceinline__ uint32_t qbcast_u(uint32_t v) {  // value of lane K of this quad
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, K * 0x55, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ float qbcast_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+
+// ------------------------------------------------------------------ ray + box
+struct RayPre {
+  v3 o, d, inv;
+  bool px, py, pz;  // |d_axis| < 1e-8: parallel axis (wgsl:712-717)
+};
+
+__device__ __forceinline__ RayPre make_ray(const v3& o, const v3& d) {
+  RayPre r;
+  r.o = o;
+  r.d = d;
+  r.inv = th::mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  r.px = th::fabs_(d.x) < TH_EPSILON;
+  r.py = th::fabs_(d.y) < TH_EPSILON;
+  r.pz = th::fabs_(d.z) < TH_EPSILON;
+  return r;
+}
+
+// Branch-free form of intersects_aabb (wgsl:700-730): the per-axis early exits of the
+// reference only ever shrink [t_min, t_max], so testing t_min <= t_max once at the end
+// accepts exactly the same boxes; a parallel axis contributes (-inf, inf) when the
+// origin lies in the slab and (inf, -inf) otherwise.
+__device__ __forceinline__ bool box_bf(float mnx, float mny, float mnz, float mxx, float mxy,
+                                       float mxz, const RayPre& r, float tmax, float* te) {
+  const float INF = __builtin_inff();
+  float nx = (mnx - r.o.x) * r.inv.x, fx = (mxx - r.o.x) * r.inv.x;
+  float ny = (mny - r.o.y) * r.inv.y, fy = (mxy - r.o.y) * r.inv.y;
+  float nz = (mnz - r.o.z) * r.inv.z, fz = (mxz - r.o.z) * r.inv.z;
+  float ax = __builtin_fminf(nx, fx), bx = __builtin_fmaxf(nx, fx);
+  float ay = __builtin_fminf(ny, fy), by = __builtin_fmaxf(ny, fy);
+  float az = __builtin_fminf(nz, fz), bz = __builtin_fmaxf(nz, fz);
+  if (r.px) {
+    bool in = !(r.o.x < mnx || r.o.x > mxx);
+    ax = in ? -INF : INF;
+    bx = in ? INF : -INF;
+  }
+  if (r.py) {
+    bool in = !(r.o.y < mny || r.o.y > mxy);
+    ay = in ? -INF : INF;
+    by = in ? INF : -INF;
+  }
+  if (r.pz) {
+    bool in = !(r.o.z < mnz || r.o.z > mxz);
+    az = in ? -INF : INF;
+    bz = in ? INF : -INF;
+  }
+  float lo = __builtin_fmaxf(__builtin_fmaxf(TH_MIN_DISTANCE, ax), __builtin_fmaxf(ay, az));
+  float hi = __builtin_fminf(__builtin_fminf(tmax, bx), __builtin_fminf(by, bz));
+  *te = lo;
+  return !(hi < lo);
+}
+
+// ------------------------------------------------------------------ traversal
+// MODE 1: closest hit (reference tie-breaking).  MODE 2: any hit with t < limit.
+// Every value that steers control flow is identical in the four lanes of the quad.
+template <int MODE>
+__device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* __restrict__ nodes4,
+                                              uint32_t root4, const v3& o, const v3& d, float limit,
+                                              Stack& st, float* best_t, uint32_t* best_slot) {
+  const uint32_t lq = threadIdx.x & 3u;
+  const float INF = __builtin_inff();
+  float best = (MODE == 2) ? limit : TH_FLT_MAX;
+  uint32_t bslot = THD_NO_SLOT, bleaf = 0;
+  RayPre r = make_ray(o, d);
+  float te;
+  *best_t = TH_FLT_MAX;
+  *best_slot = THD_NO_SLOT;
+  if (!box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1],
+              S.root_max[2], r, best, &te))
+    return false;
+  st.sp = 0;
+  uint32_t e = root4;
+  for (;;) {
+    if (e & THD_LEAF_BIT) {
+      uint32_t first, count;
+      leaf_range(S, e, &first, &count);
+      for (uint32_t base = 0; base < count; base += 4u) {
+        uint32_t k = base + lq;
+        float t = INF;
+        if (k < count) {
+          uint32_t tag;
+          t = slot_t(S, first + k, o, d, &tag);
+        }
+        if (MODE == 2) {
+          bool h = t < limit;
+          uint32_t any = qbcast_u<0>(h) | qbcast_u<1>(h) | qbcast_u<2>(h) | qbcast_u<3>(h);
+          if (any) {
+            *best_t = 0.0f;
+            *best_slot = first;
+            return true;
+          }
+        } else {
+          // quad argmin over (t, lane): the smallest slot wins among equal t in a leaf
+          float t0 = qbcast_f<0>(t), t1 = qbcast_f<1>(t), t2 = qbcast_f<2>(t), t3 = qbcast_f<3>(t);
+          uint32_t w = 0;
+          float tw = t0;
+          if (t1 < tw) { tw = t1; w = 1; }
+          if (t2 < tw) { tw = t2; w = 2; }
+          if (t3 < tw) { tw = t3; w = 3; }
+          if (tw < TH_FLT_MAX) {
+            uint32_t slot = first + base + w;
+            if (tw < best || (tw == best && (first > bleaf || (first == bleaf && slot < bslot)))) {
+              best = tw;
+              bslot = slot;
+              bleaf = first;
+            }
+          }
+        }
+      }
+    } else {
+      Child4 c = nodes4[4u * e + lq];
+      uint32_t ce = __float_as_uint(c.a.w);
+      float tE = INF;
+      if (ce != THD_NO_CHILD) {
+        float t;
+        if (box_bf(c.a.x, c.a.y, c.a.z, c.b.x, c.b.y, c.b.z, r, best, &t)) tE = t;
+      }
+      float t0 = qbcast_f<0>(tE), t1 = qbcast_f<1>(tE), t2 = qbcast_f<2>(tE), t3 = qbcast_f<3>(tE);
+      uint32_t e0 = qbcast_u<0>(ce), e1 = qbcast_u<1>(ce), e2 = qbcast_u<2>(ce), e3 = qbcast_u<3>(ce);
+      uint32_t nh = (t0 != INF) + (t1 != INF) + (t2 != INF) + (t3 != INF);
+      if (nh > 0u) {
+        // rank of this lane's child among hit children by (tE, lane)
+        uint32_t rank = (t0 < tE || (t0 == tE && 0u < lq)) + (t1 < tE || (t1 == tE && 1u < lq)) +
+                        (t2 < tE || (t2 == tE && 2u < lq)) + (t3 < tE || (t3 == tE && 3u < lq));
+        // nearest child (rank 0)
+        uint32_t w = 0;
+        float tw = t0;
+        if (t1 < tw) { tw = t1; w = 1; }
+        if (t2 < tw) { tw = t2; w = 2; }
+        if (t3 < tw) { tw = t3; w = 3; }
+        uint32_t ew = w == 0 ? e0 : (w == 1 ? e1 : (w == 2 ? e2 : e3));
+        if (tE != INF && rank > 0u) st.base[(st.sp + (int)(nh - 1u - rank)) * st.stride] =
+            make_uint2(ce, __float_as_uint(tE));
+        st.sp += (int)nh - 1;
+        e = ew;
+        continue;
+      }
+    }
+    bool found = false;
+    while (st.sp > 0) {
+      uint2 x = st.pop();
+      if (__uint_as_float(x.y) <= best) {
+        e = x.x;
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  if (MODE == 2) return false;
+  *best_t = best;
+  *best_slot = bslot;
+  return bslot != THD_NO_SLOT;
+}
+
+}  // namespace thd
