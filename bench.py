@@ -121,7 +121,14 @@ def main():
                     ("quad_wide_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
                                                                traversal=A.TH_TRAVERSE_WIDE)),
                     ("megakernel_ordered_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
-                                                                        traversal=A.TH_TRAVERSE_ORDERED))):
+                                                                        traversal=A.TH_TRAVERSE_ORDERED)),
+                    ("wavefront_wide", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE,
+                                                     kernel=A.TH_KERNEL_WAVEFRONT)),
+                    ("wavefront_wide_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                                    traversal=A.TH_TRAVERSE_WIDE,
+                                                                    kernel=A.TH_KERNEL_WAVEFRONT)),
+                    ("wavefront_ordered", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_ORDERED,
+                                                        kernel=A.TH_KERNEL_WAVEFRONT))):
         gs.render(options=o, frame_index=0, accumulated_samples=0)
         ctx.synchronize()
         ctx.reset_stats()

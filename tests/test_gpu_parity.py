@@ -72,6 +72,28 @@ def test_per_sample_rng_matches_oracle(gpu_ctx, traversal, samples):
         assert _compare(img, ref1)[0] >= TOL_FRAC
 
 
+@pytest.mark.parametrize("traversal", TRAVERSALS)
+@pytest.mark.parametrize("rng,samples", [(A.TH_RNG_REFERENCE, 4), (A.TH_RNG_PER_SAMPLE, 8)])
+@pytest.mark.parametrize("name,make", [
+    ("cornell", lambda spp: scenes.cornell(samples=spp)),
+    ("mixed_env", lambda spp: scenes.mixed_environment(width=160, height=90, samples=spp, max_bounces=8,
+                                                       env_size=(256, 128))),
+])
+def test_wavefront_matches_oracle(gpu_ctx, name, make, rng, samples, traversal):
+    sc = make(samples)
+    print(f"case wavefront {name} rng={rng} spp={samples} traversal={traversal}", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(rng_mode=rng, traversal=traversal,
+                                             kernel=A.TH_KERNEL_WAVEFRONT))
+    group = 1
+    if rng == A.TH_RNG_PER_SAMPLE:
+        while group * 2 <= min(samples, 16):
+            group *= 2
+    ref, _ = oracle.OracleScene(sc).render(threads=8, rng_mode=rng, group=group)
+    frac, exact = _compare(img, ref)
+    print(f"wavefront {name} rng={rng} traversal={traversal}: within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert frac >= TOL_FRAC
+
+
 def test_primary_hits_match(gpu_ctx):
     sc = scenes.bvh_stress(8192)
     gs = GpuScene.from_scene(gpu_ctx, sc)

@@ -1,0 +1,70 @@
+// th_internal.h -- runtime objects shared by the translation units of libtoaster_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "th_common.h"
+#include "th_integrator.h"
+
+using namespace thd;
+
+#define TH_CHECK(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return th_set_error(TH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct th_wavefront;  // wavefront workspace (th_wavefront.hip)
+
+struct th_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // unused spare pair
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool, ev_pending;
+  unsigned long long* d_counters = nullptr;
+  bool counters_pending = false;
+  th_stats stats;
+  float* d_srgb_lut = nullptr;
+};
+
+struct th_scene {
+  th_context* ctx = nullptr;
+  TH_GpuRenderParams params;  // counts fixed at create
+  uint32_t node_capacity = 0;
+  uint32_t width = 0, height = 0;
+  TH_GpuBvhNode* d_nodes = nullptr;
+  TH_GpuPrimitiveRef* d_prims = nullptr;
+  TH_GpuSphere* d_spheres = nullptr;
+  TH_GpuTriangle* d_tris = nullptr;
+  TH_GpuTriangleAttributes* d_attrs = nullptr;
+  TH_GpuMaterial* d_mats = nullptr;
+  TH_GpuLight* d_lights = nullptr;
+  float4* d_env = nullptr;
+  uint32_t* d_tex = nullptr;
+  CNode* d_cnodes = nullptr;
+  Child4* d_nodes4 = nullptr;
+  uint32_t root4 = 0, depth4 = 0;
+  float4* d_slots = nullptr;
+  float4* d_accum = nullptr;
+  uint32_t node_count = 0, prim_count = 0, env_count = 0, tex_count = 0;
+  uint32_t root_entry = 0;
+  float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
+  uint32_t max_depth = 0;
+  th_wavefront* wf = nullptr;  // allocated by the first wavefront render
+};
+
+
+DevScene dev_scene(const th_scene* s);
+int32_t begin_timed(th_context* ctx, std::pair<hipEvent_t, hipEvent_t>* pr);
+int32_t end_timed(th_context* ctx, std::pair<hipEvent_t, hipEvent_t> pr);
+int32_t fold_timing(th_context* ctx);
+
+// Wavefront renderer (th_wavefront.hip).
+int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
+                            const TH_GpuCamera* cam, const th_render_options* o);
+void th_wavefront_free(th_wavefront* wf);

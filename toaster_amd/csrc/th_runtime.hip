@@ -11,8 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "th_common.h"
-#include "th_integrator.h"
+#include "th_internal.h"
 
 using namespace thd;
 
@@ -300,50 +299,8 @@ __global__ void k_display(const float4* __restrict__ accum, uint32_t n, float sc
 }
 
 // ====================================================================== host side
-struct th_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // unused spare pair
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool, ev_pending;
-  unsigned long long* d_counters = nullptr;
-  bool counters_pending = false;
-  th_stats stats;
-  float* d_srgb_lut = nullptr;
-};
-
-struct th_scene {
-  th_context* ctx = nullptr;
-  TH_GpuRenderParams params;  // counts fixed at create
-  uint32_t node_capacity = 0;
-  uint32_t width = 0, height = 0;
-  TH_GpuBvhNode* d_nodes = nullptr;
-  TH_GpuPrimitiveRef* d_prims = nullptr;
-  TH_GpuSphere* d_spheres = nullptr;
-  TH_GpuTriangle* d_tris = nullptr;
-  TH_GpuTriangleAttributes* d_attrs = nullptr;
-  TH_GpuMaterial* d_mats = nullptr;
-  TH_GpuLight* d_lights = nullptr;
-  float4* d_env = nullptr;
-  uint32_t* d_tex = nullptr;
-  CNode* d_cnodes = nullptr;
-  Child4* d_nodes4 = nullptr;
-  uint32_t root4 = 0, depth4 = 0;
-  float4* d_slots = nullptr;
-  float4* d_accum = nullptr;
-  uint32_t node_count = 0, prim_count = 0, env_count = 0, tex_count = 0;
-  uint32_t root_entry = 0;
-  float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
-  uint32_t max_depth = 0;
-};
-
 namespace {
 
-#define TH_CHECK(expr)                                                          \
-  do {                                                                          \
-    hipError_t e_ = (expr);                                                     \
-    if (e_ != hipSuccess)                                                       \
-      return th_set_error(TH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 template <class T>
 int32_t dalloc(T** p, size_t count) {
@@ -550,6 +507,24 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
   return TH_OK;
 }
 
+int32_t check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
+  if (!p) return th_set_error(TH_E_INVALID, "null params");
+  if (p->width != s->width || p->height != s->height)
+    return th_set_error(TH_E_MISMATCH, "render dimensions differ from the scene's output buffer");
+  if (p->samples == 0) return th_set_error(TH_E_INVALID, "samples must be nonzero");
+  if (p->sphere_count != s->params.sphere_count || p->triangle_count != s->params.triangle_count ||
+      p->material_count != s->params.material_count || p->bvh_node_count != s->node_count ||
+      p->bvh_primitive_count != s->prim_count || p->light_count != s->params.light_count)
+    return th_set_error(TH_E_MISMATCH, "params counts differ from the uploaded scene");
+  if (p->background_kind > 2u) return th_set_error(TH_E_INVALID, "unknown background kind");
+  if (p->background_kind == 2u &&
+      ((uint64_t)p->environment_width * p->environment_height != s->env_count || s->env_count == 0))
+    return th_set_error(TH_E_MISMATCH, "environment dimensions differ from the uploaded map");
+  return TH_OK;
+}
+
+}  // namespace
+
 DevScene dev_scene(const th_scene* s) {
   DevScene S;
   S.cnodes = s->d_cnodes;
@@ -614,23 +589,6 @@ int32_t fold_timing(th_context* ctx) {
   return TH_OK;
 }
 
-int32_t check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
-  if (!p) return th_set_error(TH_E_INVALID, "null params");
-  if (p->width != s->width || p->height != s->height)
-    return th_set_error(TH_E_MISMATCH, "render dimensions differ from the scene's output buffer");
-  if (p->samples == 0) return th_set_error(TH_E_INVALID, "samples must be nonzero");
-  if (p->sphere_count != s->params.sphere_count || p->triangle_count != s->params.triangle_count ||
-      p->material_count != s->params.material_count || p->bvh_node_count != s->node_count ||
-      p->bvh_primitive_count != s->prim_count || p->light_count != s->params.light_count)
-    return th_set_error(TH_E_MISMATCH, "params counts differ from the uploaded scene");
-  if (p->background_kind > 2u) return th_set_error(TH_E_INVALID, "unknown background kind");
-  if (p->background_kind == 2u &&
-      ((uint64_t)p->environment_width * p->environment_height != s->env_count || s->env_count == 0))
-    return th_set_error(TH_E_MISMATCH, "environment dimensions differ from the uploaded map");
-  return TH_OK;
-}
-
-}  // namespace
 
 // ====================================================================== C ABI
 extern "C" {
@@ -794,6 +752,7 @@ void th_scene_destroy(th_scene* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   free_scene_buffers(s);
+  th_wavefront_free(s->wf);
   delete s;
 }
 
@@ -810,8 +769,8 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
     return th_set_error(TH_E_INVALID, "unknown render option");
   if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
     return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
-  if (o.kernel == TH_KERNEL_WAVEFRONT)
-    return th_set_error(TH_E_UNSUPPORTED, "wavefront kernel not built");
+  if (o.kernel == TH_KERNEL_WAVEFRONT && o.collect_stats)
+    return th_set_error(TH_E_UNSUPPORTED, "statistics are counted by the megakernel");
   TH_CHECK(hipSetDevice(ctx->device));
   DevScene S = dev_scene(s);
   uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
@@ -822,7 +781,12 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   unsigned long long* cnt = ctx->d_counters;
   std::pair<hipEvent_t, hipEvent_t> evp;
   if ((rc = begin_timed(ctx, &evp))) return rc;
-  if (o.traversal == TH_TRAVERSE_WIDE && s->node_count > 0) {
+  if (o.kernel == TH_KERNEL_WAVEFRONT && p->max_bounces > 0) {
+    if ((rc = th_wavefront_render(ctx, s, p, cam, &o))) {
+      (void)end_timed(ctx, evp);
+      return rc;
+    }
+  } else if (o.traversal == TH_TRAVERSE_WIDE && s->node_count > 0) {
     uint32_t depth = 3u * s->depth4 + 2u;
     size_t lds4 = (size_t)depth * (block / 4) * sizeof(uint2);
     int G = o.rng_mode == TH_RNG_REFERENCE ? 1
