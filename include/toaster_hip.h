@@ -317,6 +317,35 @@ int32_t th_primary_hits(th_context* ctx, th_scene* scene, const TH_GpuRenderPara
                         const TH_GpuCamera* camera, uint32_t traversal, uint32_t* slot_out,
                         float* distance_out, size_t count);
 
+/* ---------------------------------------------------------- batching */
+/* One entry of a batched launch: a scene (or one animation frame uploaded as its own
+ * scene), its dispatch params/camera and its launch shape.  No counterpart in the
+ * reference (one scene per dispatch there). */
+typedef struct th_batch_item {
+    th_scene* scene;
+    TH_GpuRenderParams params;
+    TH_GpuCamera camera;
+    uint32_t quads_per_pixel; /* per-sample RNG: 1, 2, 4, 8, 16 quads share a pixel; 0 = default */
+    uint32_t cached_nodes;    /* 4-wide nodes copied to LDS (<= 85); UINT32_MAX = default   */
+} th_batch_item;
+
+typedef struct th_tuning {
+    uint32_t quads_per_pixel;
+    uint32_t cached_nodes;
+    float best_ms;
+    uint32_t reserved;
+} th_tuning;
+
+/* Renders every item in one persistent launch (work items handed out by an atomic
+ * counter, most expensive scenes first), replayed from a captured hipGraph when the
+ * batch shape repeats.  Asynchronous; rng_mode of options applies to all items. */
+int32_t th_render_batch(th_context* ctx, const th_batch_item* items, uint32_t count,
+                        const th_render_options* options);
+/* Times the launch-shape candidates for one scene on the device and returns the
+ * fastest (also remembered by the scene). */
+int32_t th_autotune(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
+                    const TH_GpuCamera* camera, const th_render_options* options, th_tuning* out);
+
 /* Linear HDR float4 -> RGBA8 with exposure, ACES fitted and sRGB OETF
  * (image_output.rs:59-92, color.rs:23-75).  Host implementation. */
 int32_t th_display_rgba8(const float* rgba, uint32_t width, uint32_t height, float exposure_stops,

@@ -121,6 +121,15 @@ class SceneDesc(C.Structure):
                 ("texture_pixel_count", u32), ("environment_pixel_count", u32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("quads_per_pixel", u32), ("cached_nodes", u32), ("best_ms", f32), ("reserved", u32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", GpuRenderParams), ("camera", GpuCamera),
+                ("quads_per_pixel", u32), ("cached_nodes", u32)]
+
+
 class RenderOptions(C.Structure):
     _fields_ = [("rng_mode", u32), ("kernel", u32), ("traversal", u32), ("block_size", u32),
                 ("collect_stats", u32), ("reserved", u32 * 3)]
@@ -158,6 +167,9 @@ SIGNATURES = {
     "th_reset_stats": (C.c_int32, [P]),
     "th_primary_hits": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera), u32,
                                     P, P, C.c_size_t]),
+    "th_render_batch": (C.c_int32, [P, C.POINTER(BatchItem), u32, C.POINTER(RenderOptions)]),
+    "th_autotune": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
+                                C.POINTER(RenderOptions), C.POINTER(Tuning)]),
     "th_display_rgba8": (C.c_int32, [P, u32, u32, f32, P, C.c_size_t]),
     "th_display_rgba8_device": (C.c_int32, [P, P, f32, P, C.c_size_t]),
 }

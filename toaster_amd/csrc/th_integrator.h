@@ -49,6 +49,8 @@ struct Integrator {
   const float* srgb_lut;
   Stack& st;
   LaneCounters<COUNT>& lc;
+  const Child4* lds_nodes = nullptr;  // LDS copy of the top 4-wide nodes (quad traversal)
+  uint32_t cached = 0;
 
   // closest hit; false = no surface within MAX_DISTANCE (wgsl:817-819)
   __device__ bool trace(const v3& o, const v3& d, Hit* h) {
@@ -63,7 +65,8 @@ struct Integrator {
     } else if (TMODE == 1) {
       traverse<1, COUNT>(S, o, d, TH_FLT_MAX, st, rs, &t, &slot);
     } else {
-      traverse_quad<1>(S, (const Child4*)S.nodes4, S.root4, o, d, TH_FLT_MAX, st, &t, &slot);
+      traverse_quad<1>(S, (const Child4*)S.nodes4, S.root4, o, d, TH_FLT_MAX, st, &t, &slot,
+                       lds_nodes, cached);
     }
     lc.add_ray(rs, false);
     if (slot == THD_NO_SLOT || t > TH_MAX_DISTANCE) return false;
@@ -92,7 +95,8 @@ struct Integrator {
     } else if (TMODE == 1) {
       traverse<2, COUNT>(S, o, d, limit, st, rs, &t, &slot);
     } else {
-      traverse_quad<2>(S, (const Child4*)S.nodes4, S.root4, o, d, limit, st, &t, &slot);
+      traverse_quad<2>(S, (const Child4*)S.nodes4, S.root4, o, d, limit, st, &t, &slot, lds_nodes,
+                       cached);
     }
     lc.add_ray(rs, true);
     return t < limit;

@@ -19,7 +19,8 @@ using namespace thd;
       return th_set_error(TH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-struct th_wavefront;  // wavefront workspace (th_wavefront.hip)
+struct th_wavefront;    // wavefront workspace (th_wavefront.hip)
+struct th_batch_state;  // batch tables and captured graph (th_batch.hip)
 
 struct th_context {
   int device = 0;
@@ -30,6 +31,7 @@ struct th_context {
   bool counters_pending = false;
   th_stats stats;
   float* d_srgb_lut = nullptr;
+  th_batch_state* batch = nullptr;
 };
 
 struct th_scene {
@@ -48,7 +50,7 @@ struct th_scene {
   uint32_t* d_tex = nullptr;
   CNode* d_cnodes = nullptr;
   Child4* d_nodes4 = nullptr;
-  uint32_t root4 = 0, depth4 = 0;
+  uint32_t root4 = 0, depth4 = 0, node4_count = 0;
   float4* d_slots = nullptr;
   float4* d_accum = nullptr;
   uint32_t node_count = 0, prim_count = 0, env_count = 0, tex_count = 0;
@@ -56,6 +58,7 @@ struct th_scene {
   float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
   uint32_t max_depth = 0;
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
+  th_tuning tuning = {0, 0, 0.0f, 0};
 };
 
 
@@ -68,3 +71,5 @@ int32_t fold_timing(th_context* ctx);
 int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
                             const TH_GpuCamera* cam, const th_render_options* o);
 void th_wavefront_free(th_wavefront* wf);
+void th_batch_free(th_batch_state* b);
+int32_t th_check_render_params(const th_scene* s, const TH_GpuRenderParams* p);

@@ -209,11 +209,18 @@ __global__ void __launch_bounds__(64) k_primary(DevScene S, TH_GpuRenderParams P
 // Quad kernels: four lanes own one path (th_wide.h).  G quads share a pixel in the
 // per-sample mode (sample j, j+G, ... per quad); in the reference RNG mode G = 1 and the
 // quad runs the pixel's samples in stream order.  A wave covers 16/G pixels.
+// LDS layout of the quad kernels: `cached` 4-wide nodes, then the per-quad stacks.
 template <bool PER_SAMPLE, int G>
-__global__ void __launch_bounds__(64) k_render_quad(DevScene S, TH_GpuRenderParams P,
-                                                    TH_GpuCamera cam, float4* __restrict__ accum,
-                                                    const float* __restrict__ srgb_lut) {
-  extern __shared__ uint2 lds_stack[];
+__global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderParams P,
+                                                     TH_GpuCamera cam, float4* __restrict__ accum,
+                                                     const float* __restrict__ srgb_lut,
+                                                     uint32_t cached) {
+  extern __shared__ uint2 lds_raw[];
+  Child4* lds_nodes = (Child4*)lds_raw;
+  for (uint32_t i = threadIdx.x; i < 4u * cached; i += blockDim.x)
+    lds_nodes[i] = ((const Child4*)S.nodes4)[i];
+  __syncthreads();
+  uint2* lds_stack = lds_raw + 4u * cached * (sizeof(Child4) / sizeof(uint2));
   constexpr uint32_t PPW = 16u / G;
   constexpr uint32_t TW = PPW >= 8u ? 4u : (PPW >= 2u ? 2u : 1u);
   constexpr uint32_t TH = PPW / TW;
@@ -226,7 +233,7 @@ __global__ void __launch_bounds__(64) k_render_quad(DevScene S, TH_GpuRenderPara
   const bool active = x < P.width && y < P.height;
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
   LaneCounters<false> lc;
-  Integrator<3, false> it{S, P, srgb_lut, st, lc};
+  Integrator<3, false> it{S, P, srgb_lut, st, lc, lds_nodes, cached};
   const uint32_t idx = y * P.width + x;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   if (active) {
@@ -492,6 +499,7 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
     s->max_depth = bvh_depth(d->bvh_nodes, d->bvh_node_count);
     std::vector<Child4> w;
     build_wide(d->bvh_nodes, d->bvh_node_count, w, &s->root4, &s->depth4);
+    s->node4_count = (uint32_t)(w.size() / 4);
     if (!w.empty()) {
       TH_CHECK(hipMemcpy(s->d_nodes4, w.data(), w.size() * sizeof(Child4), hipMemcpyHostToDevice));
     }
@@ -507,7 +515,7 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
   return TH_OK;
 }
 
-int32_t check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
+int32_t check_render_params_impl(const th_scene* s, const TH_GpuRenderParams* p) {
   if (!p) return th_set_error(TH_E_INVALID, "null params");
   if (p->width != s->width || p->height != s->height)
     return th_set_error(TH_E_MISMATCH, "render dimensions differ from the scene's output buffer");
@@ -590,6 +598,13 @@ int32_t fold_timing(th_context* ctx) {
 }
 
 
+int32_t th_check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
+  return check_render_params_impl(s, p);
+}
+static int32_t check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
+  return check_render_params_impl(s, p);
+}
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -638,6 +653,7 @@ void th_context_destroy(th_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_counters) (void)hipFree(c->d_counters);
   if (c->d_srgb_lut) (void)hipFree(c->d_srgb_lut);
+  th_batch_free(c->batch);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (auto* v : {&c->ev_pool, &c->ev_pending})
@@ -787,16 +803,18 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
       return rc;
     }
   } else if (o.traversal == TH_TRAVERSE_WIDE && s->node_count > 0) {
+    const uint32_t qblock = 256;
     uint32_t depth = 3u * s->depth4 + 2u;
-    size_t lds4 = (size_t)depth * (block / 4) * sizeof(uint2);
+    uint32_t cached = std::min<uint32_t>(s->node4_count, 85u);  // top four 4-wide levels
+    size_t lds4 = (size_t)depth * (qblock / 4) * sizeof(uint2) + (size_t)cached * 4 * sizeof(Child4);
     int G = o.rng_mode == TH_RNG_REFERENCE ? 1
             : (p->samples >= 16 ? 16 : p->samples >= 8 ? 8 : p->samples >= 4 ? 4 : p->samples >= 2 ? 2 : 1);
     uint32_t ppw = 16u / (uint32_t)G;
     uint32_t tw = ppw >= 8 ? 4 : (ppw >= 2 ? 2 : 1), tht = ppw / tw;
     uint32_t waves = ((p->width + tw - 1) / tw) * ((p->height + tht - 1) / tht);
-    dim3 grid(waves), blk(block);
+    dim3 grid((waves + 3) / 4), blk(qblock);
 #define TH_LAUNCHQ(PS, GG) \
-  hipLaunchKernelGGL((k_render_quad<PS, GG>), grid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut)
+  hipLaunchKernelGGL((k_render_quad<PS, GG>), grid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cached)
     if (o.rng_mode == TH_RNG_REFERENCE) {
       TH_LAUNCHQ(false, 1);
     } else {

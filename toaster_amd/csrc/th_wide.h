@@ -40,6 +40,7 @@ __device__ __forceinline__ float qbcast_f(float v) {
 struct RayPre {
   v3 o, d, inv;
   bool px, py, pz;  // |d_axis| < 1e-8: parallel axis (wgsl:712-717)
+  bool anypar;
 };
 
 __device__ __forceinline__ RayPre make_ray(const v3& o, const v3& d) {
@@ -50,6 +51,7 @@ __device__ __forceinline__ RayPre make_ray(const v3& o, const v3& d) {
   r.px = th::fabs_(d.x) < TH_EPSILON;
   r.py = th::fabs_(d.y) < TH_EPSILON;
   r.pz = th::fabs_(d.z) < TH_EPSILON;
+  r.anypar = r.px || r.py || r.pz;
   return r;
 }
 
@@ -66,6 +68,12 @@ __device__ __forceinline__ bool box_bf(float mnx, float mny, float mnz, float mx
   float ax = __builtin_fminf(nx, fx), bx = __builtin_fmaxf(nx, fx);
   float ay = __builtin_fminf(ny, fy), by = __builtin_fmaxf(ny, fy);
   float az = __builtin_fminf(nz, fz), bz = __builtin_fmaxf(nz, fz);
+  if (__builtin_expect(!r.anypar, 1)) {  // (almost) every ray: no parallel axis
+    float lo = __builtin_fmaxf(__builtin_fmaxf(TH_MIN_DISTANCE, ax), __builtin_fmaxf(ay, az));
+    float hi = __builtin_fminf(__builtin_fminf(tmax, bx), __builtin_fminf(by, bz));
+    *te = lo;
+    return !(hi < lo);
+  }
   if (r.px) {
     bool in = !(r.o.x < mnx || r.o.x > mxx);
     ax = in ? -INF : INF;
@@ -90,10 +98,14 @@ __device__ __forceinline__ bool box_bf(float mnx, float mny, float mnz, float mx
 // ------------------------------------------------------------------ traversal
 // MODE 1: closest hit (reference tie-breaking).  MODE 2: any hit with t < limit.
 // Every value that steers control flow is identical in the four lanes of the quad.
+// The first `cached` 4-wide nodes (breadth-first order: the top levels every ray
+// visits) may be served from a copy in LDS.
 template <int MODE>
 __device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* __restrict__ nodes4,
                                               uint32_t root4, const v3& o, const v3& d, float limit,
-                                              Stack& st, float* best_t, uint32_t* best_slot) {
+                                              Stack& st, float* best_t, uint32_t* best_slot,
+                                              const Child4* lds_nodes = nullptr,
+                                              uint32_t cached = 0) {
   const uint32_t lq = threadIdx.x & 3u;
   const float INF = __builtin_inff();
   float best = (MODE == 2) ? limit : TH_FLT_MAX;
@@ -145,7 +157,7 @@ __device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* _
         }
       }
     } else {
-      Child4 c = nodes4[4u * e + lq];
+      Child4 c = e < cached ? lds_nodes[4u * e + lq] : nodes4[4u * e + lq];
       uint32_t ce = __float_as_uint(c.a.w);
       float tE = INF;
       if (ce != THD_NO_CHILD) {

@@ -214,3 +214,98 @@ def mixed_environment(width=1280, height=720, samples=64, max_bounces=8, n_trian
 
 def no_attributes(n):
     return np.zeros(n, ATTR_IN_DTYPE)
+
+
+def random_rotation(rng, max_degrees):
+    """Unit quaternion (w, x, y, z): axis uniform on the sphere, angle U[0, max_degrees]."""
+    u = rng.uniform(3).astype(np.float64)
+    z = 2.0 * u[0] - 1.0
+    phi = 2.0 * np.pi * u[1]
+    r = np.sqrt(max(0.0, 1.0 - z * z))
+    axis = np.array([r * np.cos(phi), r * np.sin(phi), z])
+    ang = np.radians(max_degrees * u[2])
+    return np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * axis])
+
+
+def quat_mul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+def quat_matrix(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def rigid_pose(scene, rotation, translation, pivot=(0.0, 0.0, 0.0)):
+    """Absolute rigid pose of the whole primitive set (transform.rs:17-82 semantics:
+    v' = translation + R (v - pivot); sphere centres move the same way)."""
+    R = quat_matrix(rotation)
+    pv = np.asarray(pivot, np.float64)
+    t = np.asarray(translation, np.float64)
+    out = Scene(**{k: getattr(scene, k) for k in scene.__dataclass_fields__})
+    if len(scene.triangles):
+        tri = scene.triangles.copy()
+        v = tri["v"].astype(np.float64) - pv
+        tri["v"] = (v @ R.T + pv + t).astype(np.float32)
+        out.triangles = tri
+    if len(scene.spheres):
+        sph = scene.spheres.copy()
+        c = sph["center"].astype(np.float64) - pv
+        sph["center"] = (c @ R.T + pv + t).astype(np.float32)
+        out.spheres = sph
+    return out
+
+
+def batched_workload(n_scenes=16, n_frames=4, samples=16, max_bounces=6, seed=4,
+                     tri_range=(128, 262144), resolutions=((320, 180), (640, 360), (1280, 720))):
+    """Workload 4: scenes of log-uniform triangle count (random-triangle generator of
+    workload 2), 0-256 extra spheres (diffuse/metal/dielectric), resolution drawn from
+    `resolutions`, each with `n_frames` animation frames of a rigid pose composed per
+    frame (rotation <= 10 degrees, translation within a 0.2 ball).  Returns a list of
+    scenes, each a list of frames."""
+    rng = Pcg32(workload_seed(seed))
+    out = []
+    for s in range(n_scenes):
+        nt = int(round(np.exp(rng.uniform(1, np.log(tri_range[0]), np.log(tri_range[1]))[0])))
+        ns = int(rng.u32(1)[0] % 257)
+        res = resolutions[int(rng.u32(1)[0] % len(resolutions))]
+        base = bvh_stress(nt, width=res[0], height=res[1], samples=samples, max_bounces=max_bounces)
+        if ns:
+            extra = np.zeros(ns, SPHERE_IN_DTYPE)
+            extra["center"] = rng.uniform(3 * ns, -2.0, 2.0).reshape(ns, 3)
+            extra["radius"] = np.exp(rng.uniform(ns, np.log(0.05), np.log(0.3)))
+            pick = rng.uniform(ns)
+            mats = np.zeros(ns, MATERIAL_IN_DTYPE)
+            alb = rng.uniform(3 * ns, 0.2, 0.9).reshape(ns, 3)
+            rough = rng.uniform(ns, 0.0, 0.5)
+            for i in range(ns):
+                if pick[i] < 0.5:
+                    mats[i] = diffuse(alb[i])
+                elif pick[i] < 0.8:
+                    mats[i] = metal(alb[i], float(rough[i]))
+                else:
+                    mats[i] = dielectric(1.5)
+            extra["material_index"] = len(base.materials) + np.arange(ns)
+            base.materials = np.concatenate([base.materials, mats])
+            base.spheres = np.concatenate([base.spheres, extra])
+        base.name = f"batch{s}_{nt}t_{ns}s_{res[0]}x{res[1]}"
+        frames = [base]
+        q = np.array([1.0, 0.0, 0.0, 0.0])
+        t = np.zeros(3)
+        for f in range(1, n_frames):
+            q = quat_mul(random_rotation(rng, 10.0), q)
+            while True:
+                d = rng.uniform(3, -0.2, 0.2).astype(np.float64)
+                if d @ d <= 0.04:
+                    break
+            t = t + d
+            fr = rigid_pose(base, q, t)
+            fr.name = f"{base.name}_f{f}"
+            frames.append(fr)
+        out.append(frames)
+    return out
