@@ -346,6 +346,30 @@ int32_t th_render_batch(th_context* ctx, const th_batch_item* items, uint32_t co
 int32_t th_autotune(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
                     const TH_GpuCamera* camera, const th_render_options* options, th_tuning* out);
 
+/* ---------------------------------------------------------- path guiding */
+/* Online-trained guiding distribution (no counterpart in the reference): an MLP
+ * 48 -> 64 -> 64 -> 64 -> 40 (ReLU, BF16 weights, FP32 accumulation on MFMA) mapping the
+ * encoded (position, normal, outgoing direction) of a diffuse vertex to an 8-lobe vMF
+ * mixture over incident directions, combined with cosine sampling by one-sample MIS. */
+#define TH_GUIDE_FEATURES 48
+#define TH_GUIDE_OUTPUTS 40
+#define TH_GUIDE_PARAMS 14056
+typedef struct th_guide th_guide;
+int32_t th_guide_create(th_context* ctx, uint32_t seed, th_guide** out);
+void th_guide_destroy(th_guide* guide);
+int32_t th_guide_get_params(th_guide* guide, float* out, uint32_t count);
+int32_t th_guide_set_params(th_guide* guide, const float* params, uint32_t count);
+int32_t th_guide_set_hyper(th_guide* guide, float lr, float beta1, float beta2, float eps);
+/* Network outputs (count x 40) for host-supplied features (count x 48). */
+int32_t th_guide_forward(th_guide* guide, const float* features, uint32_t count, float* out);
+/* One Adam step (update != 0) or the gradient only on host-supplied samples: loss
+ * -sum_b w_b log q_b(dir_b) / sum_b w_b; loss and gradient (14056 floats) optional. */
+int32_t th_guide_train_batch(th_guide* guide, const float* features, const float* dirs,
+                             const float* weights, uint32_t count, int32_t update, float* loss,
+                             float* grad);
+/* One v_mfma_f32_16x16x32_bf16 with this library's fragment mapping: C = A(16x32) B(32x16). */
+int32_t th_guide_mfma_selftest(th_context* ctx, const float* a, const float* b, float* c);
+
 /* Linear HDR float4 -> RGBA8 with exposure, ACES fitted and sRGB OETF
  * (image_output.rs:59-92, color.rs:23-75).  Host implementation. */
 int32_t th_display_rgba8(const float* rgba, uint32_t width, uint32_t height, float exposure_stops,

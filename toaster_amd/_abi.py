@@ -25,6 +25,8 @@ TH_RNG_REFERENCE, TH_RNG_PER_SAMPLE = 0, 1
 TH_KERNEL_MEGA, TH_KERNEL_WAVEFRONT = 0, 1
 TH_TRAVERSE_REFERENCE, TH_TRAVERSE_ORDERED, TH_TRAVERSE_WIDE = 0, 1, 2
 
+TH_GUIDE_FEATURES, TH_GUIDE_OUTPUTS, TH_GUIDE_PARAMS = 48, 40, 14056
+
 TH_CHANGE_CAMERA = 1
 TH_CHANGE_SPHERES = 2
 TH_CHANGE_TRIANGLES = 4
@@ -172,6 +174,14 @@ SIGNATURES = {
                                 C.POINTER(RenderOptions), C.POINTER(Tuning)]),
     "th_display_rgba8": (C.c_int32, [P, u32, u32, f32, P, C.c_size_t]),
     "th_display_rgba8_device": (C.c_int32, [P, P, f32, P, C.c_size_t]),
+    "th_guide_create": (C.c_int32, [P, u32, C.POINTER(P)]),
+    "th_guide_destroy": (None, [P]),
+    "th_guide_get_params": (C.c_int32, [P, P, u32]),
+    "th_guide_set_params": (C.c_int32, [P, P, u32]),
+    "th_guide_set_hyper": (C.c_int32, [P, f32, f32, f32, f32]),
+    "th_guide_forward": (C.c_int32, [P, P, u32, P]),
+    "th_guide_train_batch": (C.c_int32, [P, P, P, P, u32, C.c_int32, P, P]),
+    "th_guide_mfma_selftest": (C.c_int32, [P, P, P, P]),
 }
 
 _lib = None

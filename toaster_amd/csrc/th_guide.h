@@ -1,0 +1,165 @@
+// th_guide.h -- the learned path-guiding proposal: feature encoding, the 8-lobe
+// von Mises-Fisher mixture decoded from the network output, its density and sampler,
+// and the MLP parameter layout.  No counterpart in the reference (its diffuse vertices
+// sample the cosine lobe only, wgsl:849-857).
+//
+// Network: 48 -> 64 -> 64 -> 64 -> 40, ReLU on hidden layers, BF16 weights and
+// activations, FP32 accumulation (MFMA v_mfma_f32_16x16x32_bf16), FP32 master weights
+// for Adam.  Output o[0..39]: lobe k has direction r_k = o[3k..3k+2] (mean = r/|r|),
+// concentration kappa_k = softplus(o[24+k]) + 0.01 and mixture logit o[32+k].
+//
+// vMF density with the stable normaliser f(w) = kappa / (2 pi (1 - e^{-2 kappa}))
+// * exp(kappa (mu.w - 1)); sampling by inversion of the cosine marginal
+// w = 1 + log(xi + (1 - xi) e^{-2 kappa}) / kappa (Wenzel Jakob, "Numerically stable
+// sampling of the von Mises-Fisher distribution on S^2", 2012).
+#pragma once
+#include <stdint.h>
+
+#include "th_math.h"
+
+#define TG_IN 48
+#define TG_INP 64  // input padded to two MFMA K steps
+#define TG_H 64
+#define TG_OUT 40
+#define TG_OUTP 64  // output padded to the 64-wide layer shape
+#define TG_LOBES 8
+// master parameter count: W1 64x48, b1 64, W2 64x64, b2, W3 64x64, b3, W4 40x64, b4 40
+#define TG_NPARAMS (64 * 48 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 40 * 64 + 40)
+
+namespace tg {
+using th::v3;
+
+// ------------------------------------------------------------------ bf16
+TH_HD uint16_t f2bf(float f) {  // round to nearest even (finite inputs)
+  union { float f; uint32_t u; } v;
+  v.f = f;
+  uint32_t u = v.u;
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+TH_HD float bf2f(uint16_t b) {
+  union { float f; uint32_t u; } v;
+  v.u = (uint32_t)b << 16;
+  return v.f;
+}
+
+// ------------------------------------------------------------------ encoding
+// Scene-bounds-normalised position p01 in [0,1]^3 -> sin/cos at 4 octaves (24), normal
+// and outgoing direction -> one-blob with 4 Gaussian bins per component (12 + 12).
+TH_HD void encode(v3 p01, v3 n, v3 wo, float* out /* TG_IN */) {
+  const float TWO_PI = 6.2831855f;
+  float pc[3] = {p01.x, p01.y, p01.z};
+  int o = 0;
+  for (int a = 0; a < 3; ++a) {
+    float f = 1.0f;
+    for (int k = 0; k < 4; ++k) {
+      float s, c;
+      th::sincos_(TWO_PI * f * pc[a], &s, &c);
+      out[o++] = s;
+      out[o++] = c;
+      f *= 2.0f;
+    }
+  }
+  float dc[6] = {n.x, n.y, n.z, wo.x, wo.y, wo.z};
+  for (int a = 0; a < 6; ++a) {
+    float u = 0.5f * (dc[a] + 1.0f);  // [-1,1] -> [0,1]
+    for (int k = 0; k < 4; ++k) {
+      float c = ((float)k + 0.5f) * 0.25f;
+      float d = (u - c) * 4.0f;  // sigma = 1/4
+#if defined(__HIP_DEVICE_COMPILE__)
+      out[o++] = __expf(-0.5f * d * d);
+#else
+      out[o++] = expf(-0.5f * d * d);
+#endif
+    }
+  }
+}
+
+// ------------------------------------------------------------------ mixture
+struct Mixture {
+  v3 mu[TG_LOBES];
+  float kappa[TG_LOBES];
+  float w[TG_LOBES];
+};
+
+TH_HD float softplus(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return x > 15.0f ? x : __logf(1.0f + __expf(x));
+#else
+  return x > 15.0f ? x : logf(1.0f + expf(x));
+#endif
+}
+TH_HD float fexp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __expf(x);
+#else
+  return expf(x);
+#endif
+}
+TH_HD float flog(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __logf(x);
+#else
+  return logf(x);
+#endif
+}
+
+TH_HD void decode(const float* o, Mixture* m) {
+  float mx = -3.0e38f;
+  for (int k = 0; k < TG_LOBES; ++k) mx = th::fmax_(mx, o[32 + k]);
+  float sum = 0.0f;
+  for (int k = 0; k < TG_LOBES; ++k) {
+    v3 r = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
+    float len = th::fsqrt(th::dot(r, r) + 1e-8f);
+    m->mu[k] = th::divs(r, len);
+    m->kappa[k] = th::fmin_(softplus(o[24 + k]) + 0.01f, 1000.0f);
+    m->w[k] = fexp(o[32 + k] - mx);
+    sum += m->w[k];
+  }
+  for (int k = 0; k < TG_LOBES; ++k) m->w[k] = m->w[k] / sum;
+}
+
+TH_HD float vmf_pdf(v3 mu, float kappa, v3 w) {
+  const float INV_2PI = 0.15915494f;
+  float norm = kappa * INV_2PI / (1.0f - fexp(-2.0f * kappa));
+  return norm * fexp(kappa * (th::dot(mu, w) - 1.0f));
+}
+
+TH_HD float mixture_pdf(const Mixture& m, v3 w) {
+  float p = 0.0f;
+  for (int k = 0; k < TG_LOBES; ++k) p += m.w[k] * vmf_pdf(m.mu[k], m.kappa[k], w);
+  return p;
+}
+
+// Orthonormal basis around a unit vector (Duff et al. 2017, branchless).
+TH_HD void onb(v3 n, v3* t, v3* b) {
+  float s = n.z >= 0.0f ? 1.0f : -1.0f;
+  float a = -1.0f / (s + n.z);
+  float bb = n.x * n.y * a;
+  *t = th::mk(1.0f + s * n.x * n.x * a, s * bb, -s * n.x);
+  *b = th::mk(bb, s + n.y * n.y * a, -n.y);
+}
+
+TH_HD v3 vmf_sample(v3 mu, float kappa, float u1, float u2) {
+  float w = 1.0f + flog(u1 + (1.0f - u1) * fexp(-2.0f * kappa)) / kappa;
+  w = th::clampf(w, -1.0f, 1.0f);
+  float r = th::fsqrt(th::fmax_(0.0f, 1.0f - w * w));
+  float s, c;
+  th::sincos_(6.2831855f * u2, &s, &c);
+  v3 t, b;
+  onb(mu, &t, &b);
+  return th::add(th::smul(w, mu), th::add(th::smul(r * c, t), th::smul(r * s, b)));
+}
+
+TH_HD v3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+  int k = 0;
+  float c = m.w[0];
+  while (k < TG_LOBES - 1 && u0 >= c) {
+    ++k;
+    c += m.w[k];
+  }
+  return vmf_sample(m.mu[k], m.kappa[k], u1, u2);
+}
+
+}  // namespace tg

@@ -1,0 +1,549 @@
+// th_guide.hip -- the guiding network on BF16 MFMA: forward (inference and training),
+// vMF-mixture loss gradient, backward (activation and weight gradients) and Adam.
+//
+// Every dense layer is Y[B x N] = X[B x K] * W^T with X and W in BF16 and FP32
+// accumulation in v_mfma_f32_16x16x32_bf16 (gfx950).  Fragment layout of that
+// instruction on a 64-lane wave (checked on the device by th_guide_mfma_selftest
+// against a host product): lane l = 16 g + r holds A[r][8g..8g+7], B[8g..8g+7][r] and
+// C[4g..4g+3][r].  A wave owns 16 rows of a batch; weights are kept in two padded
+// BF16 layouts, out-major ([N][K], forward) and in-major ([K][N], backward), so every
+// B fragment is one 16-byte load.  Weight gradients dW[N x K] = dZ^T X reduce over the
+// batch inside the MFMA K dimension (32 rows per instruction) with the 32-row chunk
+// staged through LDS; each block writes one partial, and the Adam kernel sums the
+// partials, updates the FP32 master weights and refreshes both BF16 copies.
+#include <algorithm>
+#include <vector>
+
+#include "th_guide.h"
+#include "th_internal.h"
+
+typedef __bf16 tg_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float tg_f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// Every layer is stored padded to 64 x 64 (zero rows/columns beyond the true shape).
+// Master parameter geometry: true in/out widths and offsets of W (out-major) and b.
+struct LayerDims {
+  int kin, nout, woff, boff;
+};
+__host__ __device__ constexpr LayerDims layer_dims(int l) {
+  return l == 0 ? LayerDims{48, 64, 0, 64 * 48}
+       : l == 1 ? LayerDims{64, 64, 64 * 48 + 64, 64 * 48 + 64 + 64 * 64}
+       : l == 2 ? LayerDims{64, 64, 64 * 48 + 64 + 64 * 64 + 64, 64 * 48 + 64 + 2 * 64 * 64 + 64}
+                : LayerDims{64, 40, 64 * 48 + 64 * 3 + 2 * 64 * 64,
+                            64 * 48 + 64 * 3 + 2 * 64 * 64 + 40 * 64};
+}
+static_assert(layer_dims(3).boff + 40 == TG_NPARAMS, "parameter layout");
+#define TG_PART (64 * 64 + 64)  // one block's partial gradient of one layer
+
+__device__ __forceinline__ tg_bf16x8 ld8(const uint16_t* p) {
+  uint4 v = *(const uint4*)p;  // 16-byte aligned
+  return __builtin_bit_cast(tg_bf16x8, v);
+}
+
+__device__ __forceinline__ tg_f32x4 mfma(tg_bf16x8 a, tg_bf16x8 b, tg_f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Y = epilogue(X * M + bias) with M given as MT[N][K]; one wave per 16 rows.
+// MODE 0: ReLU, BF16 out.  MODE 1: linear, FP32 out.  MODE 2: gated by gate > 0
+// (ReLU derivative of the layer input), BF16 out, no bias.
+template <int K, int N, int MODE>
+__global__ void __launch_bounds__(256) k_gemm_rows(const uint16_t* __restrict__ X,
+                                                   const uint16_t* __restrict__ MT,
+                                                   const float* __restrict__ bias,
+                                                   const uint32_t* __restrict__ count_ptr,
+                                                   uint32_t count_static, void* __restrict__ Y,
+                                                   const uint16_t* __restrict__ gate) {
+  const uint32_t rows = count_ptr ? *count_ptr : count_static;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t row0 = wave * 16u;
+  if (row0 >= rows) return;
+  const uint32_t l = threadIdx.x & 63u, r = l & 15u, g = l >> 4;
+  tg_bf16x8 a[K / 32];
+#pragma unroll
+  for (int kb = 0; kb < K / 32; ++kb) a[kb] = ld8(X + (size_t)(row0 + r) * K + kb * 32 + g * 8);
+#pragma unroll
+  for (int nb = 0; nb < N / 16; ++nb) {
+    tg_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < K / 32; ++kb)
+      acc = mfma(a[kb], ld8(MT + (size_t)(nb * 16 + r) * K + kb * 32 + g * 8), acc);
+    const uint32_t col = nb * 16 + r;
+    const float bv = (MODE != 2 && bias) ? bias[col] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t row = row0 + g * 4 + i;
+      float v = acc[i] + bv;
+      if (MODE == 0) {
+        ((uint16_t*)Y)[(size_t)row * N + col] = tg::f2bf(v > 0.0f ? v : 0.0f);
+      } else if (MODE == 1) {
+        ((float*)Y)[(size_t)row * N + col] = v;
+      } else {
+        uint16_t gb = gate[(size_t)row * N + col];
+        ((uint16_t*)Y)[(size_t)row * N + col] = tg::f2bf((gb & 0x7fffu) ? v : 0.0f);
+      }
+    }
+  }
+}
+
+// Partial weight/bias gradient of one block: dW[n][k] = sum_b dZ[b][n] X[b][k],
+// db[n] = sum_b dZ[b][n], over rows [blk * rows_per, (blk+1) * rows_per).
+template <int K, int N>
+__global__ void __launch_bounds__(64) k_gemm_wgrad(const uint16_t* __restrict__ X,
+                                                   const uint16_t* __restrict__ dZ,
+                                                   uint32_t rows_per, uint32_t rows_total,
+                                                   float* __restrict__ partial) {
+  __shared__ __align__(16) uint16_t xs[32 * K];
+  __shared__ __align__(16) uint16_t zs[32 * N];
+  const uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
+  tg_f32x4 acc[N / 16][K / 16];
+#pragma unroll
+  for (int a = 0; a < N / 16; ++a)
+#pragma unroll
+    for (int b = 0; b < K / 16; ++b) acc[a][b] = tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float db = 0.0f;
+  const uint32_t r0 = blockIdx.x * rows_per;
+  const uint32_t r1 = min(r0 + rows_per, rows_total);
+  for (uint32_t c0 = r0; c0 < r1; c0 += 32u) {
+    // stage 32 rows of X and dZ (16-byte pieces), rows past the end read as zero
+    for (uint32_t i = l; i < 32u * K / 8u; i += 64u) {
+      uint32_t row = c0 + (i * 8u) / K;
+      uint4 v = row < r1 ? *(const uint4*)(X + (size_t)c0 * K + i * 8u) : make_uint4(0, 0, 0, 0);
+      *(uint4*)(xs + i * 8u) = v;
+    }
+    for (uint32_t i = l; i < 32u * N / 8u; i += 64u) {
+      uint32_t row = c0 + (i * 8u) / N;
+      uint4 v = row < r1 ? *(const uint4*)(dZ + (size_t)c0 * N + i * 8u) : make_uint4(0, 0, 0, 0);
+      *(uint4*)(zs + i * 8u) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int nb = 0; nb < N / 16; ++nb) {
+      uint16_t av[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) av[j] = zs[(g * 8 + j) * N + nb * 16 + r];
+      tg_bf16x8 A = __builtin_bit_cast(tg_bf16x8, *(uint4*)av);
+#pragma unroll
+      for (int kb = 0; kb < K / 16; ++kb) {
+        uint16_t bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = xs[(g * 8 + j) * K + kb * 16 + r];
+        tg_bf16x8 B = __builtin_bit_cast(tg_bf16x8, *(uint4*)bv);
+        acc[nb][kb] = mfma(A, B, acc[nb][kb]);
+      }
+    }
+    if (l < (uint32_t)N)
+      for (int j = 0; j < 32; ++j) db += tg::bf2f(zs[j * N + l]);
+    __syncthreads();
+  }
+  float* out = partial + (size_t)blockIdx.x * (N * K + N);
+#pragma unroll
+  for (int nb = 0; nb < N / 16; ++nb)
+#pragma unroll
+    for (int kb = 0; kb < K / 16; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[(nb * 16 + g * 4 + i) * K + kb * 16 + r] = acc[nb][kb][i];
+  if (l < (uint32_t)N) out[N * K + l] = db;
+}
+
+// Loss gradient of one batch: L = -sum_b w_b log q_b(dir_b) / sum_b w_b.  The weight sum
+// is one block with a fixed reduction order, so gradients are reproducible call to call.
+__global__ void __launch_bounds__(1024) k_weight_sum(const float* __restrict__ w, uint32_t n,
+                                                     float* __restrict__ sums) {
+  __shared__ float red[1024];
+  float s = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 1024u) s += w[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[0] = red[0];
+}
+
+// dL/dO per sample (BF16, 64 wide, zero beyond the 40 outputs) and the loss itself.
+// With g_k = w_k f_k / q: d log q / d r_k = g_k kappa_k (w - mu_k (mu_k.w)) / |r_k|,
+// d log q / d kappa_k = g_k (1/kappa_k - 2 e^{-2 kappa_k} / (1 - e^{-2 kappa_k}) + mu_k.w - 1)
+// (times the softplus slope sigmoid(o), zero where the clamp is active) and
+// d log q / d logit_k = g_k - w_k.
+__global__ void k_loss_grad(const float* __restrict__ O, const float* __restrict__ dirs,
+                            const float* __restrict__ w, uint32_t count, uint32_t rows,
+                            const float* __restrict__ sums, uint16_t* __restrict__ dZ,
+                            float* __restrict__ loss) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = b < rows && b < count && sums[0] > 0.0f;
+  float dzv[TG_OUT];
+  float lterm = 0.0f;
+  if (live) {
+    const float* o = O + (size_t)b * TG_OUTP;
+    tg::Mixture m;
+    tg::decode(o, &m);
+    const v3 om = th::mk(dirs[3 * b], dirs[3 * b + 1], dirs[3 * b + 2]);
+    float f[TG_LOBES], q = 0.0f;
+    for (int k = 0; k < TG_LOBES; ++k) {
+      f[k] = tg::vmf_pdf(m.mu[k], m.kappa[k], om);
+      q += m.w[k] * f[k];
+    }
+    q = th::fmax_(q, 1e-30f);
+    const float scale = -w[b] / sums[0];
+    lterm = scale * tg::flog(q);
+    for (int k = 0; k < TG_LOBES; ++k) {
+      const float gk = m.w[k] * f[k] / q;
+      const v3 r = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
+      const float len = th::fsqrt(th::dot(r, r) + 1e-8f);
+      const float c = th::dot(m.mu[k], om);
+      const float kap = m.kappa[k];
+      const v3 dr = th::muls(th::sub(om, th::muls(m.mu[k], c)), gk * kap / len);
+      const float e2 = tg::fexp(-2.0f * kap);
+      const float dk = gk * (1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c - 1.0f));
+      const float sig = kap >= 1000.0f ? 0.0f : 1.0f / (1.0f + tg::fexp(-o[24 + k]));
+      dzv[3 * k] = scale * dr.x;
+      dzv[3 * k + 1] = scale * dr.y;
+      dzv[3 * k + 2] = scale * dr.z;
+      dzv[24 + k] = scale * dk * sig;
+      dzv[32 + k] = scale * (gk - m.w[k]);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) lterm += __shfl_xor(lterm, off, 64);
+  if ((threadIdx.x & 63u) == 0u && lterm != 0.0f) atomicAdd(loss, lterm);
+  if (b >= rows) return;
+  uint16_t* dz = dZ + (size_t)b * TG_OUTP;
+  for (int j = 0; j < TG_OUTP; ++j) dz[j] = (live && j < TG_OUT) ? tg::f2bf(dzv[j]) : (uint16_t)0;
+}
+
+__global__ void k_clear(float* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0f;
+}
+
+// Adam on the FP32 master parameters (sum of the per-block partials as gradient, or
+// only the gradient when lr == 0), then both padded BF16 copies and biases.
+__global__ void k_adam(float* __restrict__ master, float* __restrict__ m1, float* __restrict__ m2,
+                       const float* __restrict__ partials, uint32_t nblocks, float lr, float b1,
+                       float b2, float eps, float bc1, float bc2, float* __restrict__ grad_out,
+                       uint16_t* __restrict__ woi, uint16_t* __restrict__ wio, float* __restrict__ bias) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= TG_NPARAMS) return;
+  int l = 0;
+  while (l < 3 && (int)i >= layer_dims(l + 1).woff) ++l;
+  const LayerDims D = layer_dims(l);
+  bool is_bias = (int)i >= D.boff;
+  int n, k = 0;
+  if (is_bias) {
+    n = (int)i - D.boff;
+  } else {
+    n = ((int)i - D.woff) / D.kin;
+    k = ((int)i - D.woff) % D.kin;
+  }
+  // partials: layer l at l * 256 * TG_PART, block stride TG_PART
+  const size_t lbase = (size_t)l * 256u * TG_PART;
+  const size_t off = is_bias ? (size_t)64 * 64 + n : (size_t)n * 64 + k;
+  float gsum = 0.0f;
+  for (uint32_t b = 0; b < nblocks; ++b) gsum += partials[lbase + (size_t)b * TG_PART + off];
+  if (grad_out) grad_out[i] = gsum;
+  float p = master[i];
+  if (lr > 0.0f) {
+    float a = b1 * m1[i] + (1.0f - b1) * gsum;
+    float v = b2 * m2[i] + (1.0f - b2) * gsum * gsum;
+    m1[i] = a;
+    m2[i] = v;
+    p = p - lr * (a / bc1) / (sqrtf(v / bc2) + eps);
+    master[i] = p;
+  }
+  // padded copies: woi[l] is [out][in], wio[l] is [in][out], 64 x 64 per layer
+  if (is_bias) {
+    bias[l * 64 + n] = p;
+  } else {
+    uint16_t h = tg::f2bf(p);
+    woi[l * 4096 + n * 64 + k] = h;
+    wio[l * 4096 + k * 64 + n] = h;
+  }
+}
+
+__global__ void k_encode_host(const float* __restrict__ feats, uint32_t count, uint32_t rows,
+                              uint16_t* __restrict__ X) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= rows) return;
+  for (int j = 0; j < TG_INP; ++j)
+    X[(size_t)b * TG_INP + j] = (b < count && j < TG_IN) ? tg::f2bf(feats[(size_t)b * TG_IN + j]) : 0;
+}
+
+__global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
+  // A 16x32 and B 32x16 row-major FP32 (rounded to BF16 here); C 16x16
+  uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
+  uint16_t av[8], bv[8];
+  for (int j = 0; j < 8; ++j) {
+    av[j] = tg::f2bf(A[r * 32 + g * 8 + j]);
+    bv[j] = tg::f2bf(B[(g * 8 + j) * 16 + r]);
+  }
+  tg_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  acc = mfma(__builtin_bit_cast(tg_bf16x8, *(uint4*)av), __builtin_bit_cast(tg_bf16x8, *(uint4*)bv), acc);
+  for (int i = 0; i < 4; ++i) C[(g * 4 + i) * 16 + r] = acc[i];
+}
+
+uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+// ====================================================================== guide object
+struct th_guide {
+  th_context* ctx = nullptr;
+  float *d_master = nullptr, *d_m1 = nullptr, *d_m2 = nullptr, *d_bias = nullptr, *d_grad = nullptr;
+  uint16_t *d_woi = nullptr, *d_wio = nullptr;
+  uint32_t step = 0;
+  float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
+  // batch workspace
+  uint32_t cap = 0;
+  uint16_t *X0 = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr;
+  uint16_t *dZ4 = nullptr, *dZ3 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
+  float *O = nullptr, *feat = nullptr, *dirs = nullptr, *w = nullptr, *partials = nullptr, *scal = nullptr;
+  uint32_t nblocks = 0;
+  std::vector<void*> ws;
+};
+
+static const uint32_t kWgradBlocks = 256;  // partial slots per layer (k_adam assumes 256)
+
+static int32_t guide_reserve(th_guide* g, uint32_t rows) {
+  if (g->cap >= rows) return TH_OK;
+  for (void* p : g->ws) (void)hipFree(p);
+  g->ws.clear();
+  g->cap = 0;
+  uint32_t cap = round_up(std::max(rows, 1024u), 1024u);
+  auto al = [&](void** p, size_t bytes) -> int32_t {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return th_set_error(TH_E_DEVICE, "guide workspace allocation failed");
+    e = hipMemset(*p, 0, bytes);
+    g->ws.push_back(*p);
+    return e == hipSuccess ? TH_OK : th_set_error(TH_E_DEVICE, "guide workspace clear failed");
+  };
+  size_t h = (size_t)cap * 64 * 2;
+  int32_t rc;
+  if ((rc = al((void**)&g->X0, h)) || (rc = al((void**)&g->H1, h)) || (rc = al((void**)&g->H2, h)) ||
+      (rc = al((void**)&g->H3, h)) || (rc = al((void**)&g->dZ4, (size_t)cap * TG_OUTP * 2)) ||
+      (rc = al((void**)&g->dZ3, h)) || (rc = al((void**)&g->dZ2, h)) || (rc = al((void**)&g->dZ1, h)) ||
+      (rc = al((void**)&g->O, (size_t)cap * TG_OUTP * 4)) ||
+      (rc = al((void**)&g->feat, (size_t)cap * TG_IN * 4)) || (rc = al((void**)&g->dirs, (size_t)cap * 12)) ||
+      (rc = al((void**)&g->w, (size_t)cap * 4)) ||
+      (rc = al((void**)&g->partials, (size_t)kWgradBlocks * 4 * TG_PART * 4)) ||
+      (rc = al((void**)&g->scal, 64)))
+    return rc;
+  g->cap = cap;
+  return TH_OK;
+}
+
+// Forward over `rows` rows of g->X0 (BF16, 64 wide) into H1..H3 and O.
+int32_t th_guide_forward_rows(th_guide* g, const uint32_t* d_count, uint32_t rows, hipStream_t st) {
+  uint32_t grid = (round_up(rows, 64) / 64);  // 4 waves x 16 rows per block
+  if (grid == 0) return TH_OK;
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->X0, g->d_woi + 0 * 4096,
+                     g->d_bias + 0, d_count, rows, (void*)g->H1, nullptr);
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->H1, g->d_woi + 1 * 4096,
+                     g->d_bias + 64, d_count, rows, (void*)g->H2, nullptr);
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->H2, g->d_woi + 2 * 4096,
+                     g->d_bias + 128, d_count, rows, (void*)g->H3, nullptr);
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 1>), dim3(grid), dim3(256), 0, st, g->H3, g->d_woi + 3 * 4096,
+                     g->d_bias + 192, d_count, rows, (void*)g->O, nullptr);
+  TH_CHECK(hipGetLastError());
+  return TH_OK;
+}
+
+// One training step on rows [0, count) of feat/dirs/w already on the device
+// (features already encoded into X0).  lr = 0 computes the gradient only.
+static int32_t guide_step(th_guide* g, uint32_t count, float lr) {
+  hipStream_t st = g->ctx->stream;
+  uint32_t rows = round_up(count, 32);
+  int32_t rc = th_guide_forward_rows(g, nullptr, rows, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, g->scal, 16u);
+  hipLaunchKernelGGL(k_weight_sum, dim3(1), dim3(1024), 0, st, g->w, count, g->scal);
+  hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, g->O, g->dirs, g->w, count,
+                     rows, g->scal, g->dZ4, g->scal + 1);
+  uint32_t rows_per = round_up((rows + kWgradBlocks - 1) / kWgradBlocks, 32);
+  uint32_t nb = (rows + rows_per - 1) / rows_per;
+  uint32_t grid = round_up(rows, 64) / 64;
+  float* P = g->partials;
+  const size_t s3 = (size_t)kWgradBlocks * TG_PART;
+  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H3, g->dZ4, rows_per, rows,
+                     P + 3 * s3);
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ4, g->d_wio + 3 * 4096,
+                     nullptr, nullptr, rows, (void*)g->dZ3, g->H3);
+  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H2, g->dZ3, rows_per, rows, P + 2 * s3);
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ3, g->d_wio + 2 * 4096,
+                     nullptr, nullptr, rows, (void*)g->dZ2, g->H2);
+  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H1, g->dZ2, rows_per, rows, P + 1 * s3);
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ2, g->d_wio + 1 * 4096,
+                     nullptr, nullptr, rows, (void*)g->dZ1, g->H1);
+  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->X0, g->dZ1, rows_per, rows, P + 0 * s3);
+  TH_CHECK(hipGetLastError());
+  float bc1 = 1.0f, bc2 = 1.0f;
+  if (lr > 0.0f) {
+    g->step += 1;
+    bc1 = 1.0f - powf(g->beta1, (float)g->step);
+    bc2 = 1.0f - powf(g->beta2, (float)g->step);
+  }
+  // partials of layer l: kWgradBlocks slots of TG_PART floats; the first nb are summed
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 255) / 256), dim3(256), 0, st, g->d_master,
+                     g->d_m1, g->d_m2, P, nb, lr, g->beta1, g->beta2, g->eps, bc1, bc2, g->d_grad,
+                     g->d_woi, g->d_wio, g->d_bias);
+  TH_CHECK(hipGetLastError());
+  return TH_OK;
+}
+
+// ====================================================================== C ABI
+static void host_init_params(uint32_t seed, std::vector<float>& p) {
+  p.assign(TG_NPARAMS, 0.0f);
+  uint32_t s = seed * 2654435761u + 12345u;
+  for (int l = 0; l < 4; ++l) {
+    LayerDims D = layer_dims(l);
+    float a = sqrtf(6.0f / (float)D.kin) * (l == 3 ? 0.1f : 1.0f);
+    for (int i = 0; i < D.nout * D.kin; ++i) {
+      s = th::pcg_hash(s);
+      p[D.woff + i] = ((float)(s >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * a;
+    }
+  }
+}
+
+static int32_t guide_refresh(th_guide* g) {  // bf16 copies from the master weights
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 255) / 256), dim3(256), 0, g->ctx->stream, g->d_master,
+                     g->d_m1, g->d_m2, g->partials, 0u, 0.0f, g->beta1, g->beta2, g->eps, 1.0f, 1.0f,
+                     (float*)nullptr, g->d_woi, g->d_wio, g->d_bias);
+  TH_CHECK(hipGetLastError());
+  return TH_OK;
+}
+
+extern "C" {
+
+int32_t th_guide_create(th_context* ctx, uint32_t seed, th_guide** out) {
+  if (!ctx || !out) return th_set_error(TH_E_INVALID, "null argument");
+  *out = nullptr;
+  TH_CHECK(hipSetDevice(ctx->device));
+  th_guide* g = new th_guide();
+  g->ctx = ctx;
+  auto al = [&](void** p, size_t bytes) -> bool {
+    return hipMalloc(p, bytes) == hipSuccess && hipMemset(*p, 0, bytes) == hipSuccess;
+  };
+  if (!al((void**)&g->d_master, TG_NPARAMS * 4) || !al((void**)&g->d_m1, TG_NPARAMS * 4) ||
+      !al((void**)&g->d_m2, TG_NPARAMS * 4) || !al((void**)&g->d_grad, TG_NPARAMS * 4) ||
+      !al((void**)&g->d_bias, 4 * 64 * 4) || !al((void**)&g->d_woi, 4 * 4096 * 2) ||
+      !al((void**)&g->d_wio, 4 * 4096 * 2)) {
+    th_guide_destroy(g);
+    return th_set_error(TH_E_DEVICE, "guide allocation failed");
+  }
+  std::vector<float> p;
+  host_init_params(seed, p);
+  int32_t rc = guide_reserve(g, 1024);
+  if (rc) {
+    th_guide_destroy(g);
+    return rc;
+  }
+  if (hipMemcpy(g->d_master, p.data(), TG_NPARAMS * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      (rc = guide_refresh(g)) || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    th_guide_destroy(g);
+    return rc ? rc : th_set_error(TH_E_DEVICE, "guide init failed");
+  }
+  *out = g;
+  return TH_OK;
+}
+
+void th_guide_destroy(th_guide* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  for (void* p : g->ws) (void)hipFree(p);
+  void* ps[] = {g->d_master, g->d_m1, g->d_m2, g->d_grad, g->d_bias, g->d_woi, g->d_wio};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete g;
+}
+
+int32_t th_guide_get_params(th_guide* g, float* out, uint32_t n) {
+  if (!g || !out || n != TG_NPARAMS) return th_set_error(TH_E_INVALID, "expected TG_NPARAMS floats");
+  TH_CHECK(hipStreamSynchronize(g->ctx->stream));
+  TH_CHECK(hipMemcpy(out, g->d_master, TG_NPARAMS * 4, hipMemcpyDeviceToHost));
+  return TH_OK;
+}
+
+int32_t th_guide_set_params(th_guide* g, const float* in, uint32_t n) {
+  if (!g || !in || n != TG_NPARAMS) return th_set_error(TH_E_INVALID, "expected TG_NPARAMS floats");
+  TH_CHECK(hipStreamSynchronize(g->ctx->stream));
+  TH_CHECK(hipMemcpy(g->d_master, in, TG_NPARAMS * 4, hipMemcpyHostToDevice));
+  TH_CHECK(hipMemset(g->d_m1, 0, TG_NPARAMS * 4));
+  TH_CHECK(hipMemset(g->d_m2, 0, TG_NPARAMS * 4));
+  g->step = 0;
+  int32_t rc = guide_refresh(g);
+  if (rc) return rc;
+  TH_CHECK(hipStreamSynchronize(g->ctx->stream));
+  return TH_OK;
+}
+
+int32_t th_guide_set_hyper(th_guide* g, float lr, float beta1, float beta2, float eps) {
+  if (!g || !(lr >= 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f) ||
+      !(eps > 0.0f))
+    return th_set_error(TH_E_INVALID, "invalid Adam hyper-parameters");
+  g->lr = lr;
+  g->beta1 = beta1;
+  g->beta2 = beta2;
+  g->eps = eps;
+  return TH_OK;
+}
+
+int32_t th_guide_forward(th_guide* g, const float* feats, uint32_t count, float* out) {
+  if (!g || !feats || !out || count == 0) return th_set_error(TH_E_INVALID, "null argument");
+  int32_t rc = guide_reserve(g, count);
+  if (rc) return rc;
+  hipStream_t st = g->ctx->stream;
+  uint32_t rows = round_up(count, 32);
+  TH_CHECK(hipMemcpyAsync(g->feat, feats, (size_t)count * TG_IN * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_encode_host, dim3((rows + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
+  if ((rc = th_guide_forward_rows(g, nullptr, rows, st))) return rc;
+  std::vector<float> o((size_t)rows * TG_OUTP);
+  TH_CHECK(hipMemcpyAsync(o.data(), g->O, o.size() * 4, hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  for (uint32_t b = 0; b < count; ++b)
+    for (int j = 0; j < TG_OUT; ++j) out[(size_t)b * TG_OUT + j] = o[(size_t)b * TG_OUTP + j];
+  return TH_OK;
+}
+
+int32_t th_guide_train_batch(th_guide* g, const float* feats, const float* dirs, const float* weights,
+                             uint32_t count, int32_t update, float* loss, float* grad) {
+  if (!g || !feats || !dirs || !weights || count == 0) return th_set_error(TH_E_INVALID, "null argument");
+  int32_t rc = guide_reserve(g, count);
+  if (rc) return rc;
+  hipStream_t st = g->ctx->stream;
+  uint32_t rows = round_up(count, 32);
+  TH_CHECK(hipMemcpyAsync(g->feat, feats, (size_t)count * TG_IN * 4, hipMemcpyHostToDevice, st));
+  TH_CHECK(hipMemcpyAsync(g->dirs, dirs, (size_t)count * 12, hipMemcpyHostToDevice, st));
+  TH_CHECK(hipMemcpyAsync(g->w, weights, (size_t)count * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_encode_host, dim3((rows + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
+  if ((rc = guide_step(g, count, update ? g->lr : 0.0f))) return rc;
+  if (loss) TH_CHECK(hipMemcpyAsync(loss, g->scal + 1, 4, hipMemcpyDeviceToHost, st));
+  if (grad) TH_CHECK(hipMemcpyAsync(grad, g->d_grad, TG_NPARAMS * 4, hipMemcpyDeviceToHost, st));
+  TH_CHECK(hipStreamSynchronize(st));
+  return TH_OK;
+}
+
+int32_t th_guide_mfma_selftest(th_context* ctx, const float* A, const float* B, float* C) {
+  if (!ctx || !A || !B || !C) return th_set_error(TH_E_INVALID, "null argument");
+  TH_CHECK(hipSetDevice(ctx->device));
+  float *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  TH_CHECK(hipMalloc((void**)&dA, 16 * 32 * 4));
+  TH_CHECK(hipMalloc((void**)&dB, 32 * 16 * 4));
+  TH_CHECK(hipMalloc((void**)&dC, 16 * 16 * 4));
+  hipError_t e = hipMemcpy(dA, A, 16 * 32 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dB, B, 32 * 16 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(C, dC, 16 * 16 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("mfma selftest: ") + hipGetErrorString(e));
+  return TH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+"""The learned guiding distribution: an MLP (48 -> 64 -> 64 -> 64 -> 40, BF16 on MFMA)
+giving an 8-lobe von Mises-Fisher mixture over incident directions at diffuse vertices.
+It has no counterpart in the reference, whose diffuse vertices sample the cosine lobe
+only (shaders/pathtrace_spheres.wgsl:849-857)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from .renderer import GpuContext
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Guide:
+    """Owns one network (FP32 master weights, Adam moments, BF16 copies) on a context."""
+
+    def __init__(self, ctx: GpuContext, seed=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        A.check(A.lib().th_guide_create(ctx.handle, seed, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self):
+        out = np.empty(A.TH_GUIDE_PARAMS, np.float32)
+        A.check(A.lib().th_guide_get_params(self._h, _ptr(out), out.size))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        A.check(A.lib().th_guide_set_params(self._h, _ptr(p), p.size))
+
+    def set_hyper(self, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8):
+        A.check(A.lib().th_guide_set_hyper(self._h, lr, beta1, beta2, eps))
+
+    def forward(self, features):
+        f = np.ascontiguousarray(features, np.float32)
+        assert f.ndim == 2 and f.shape[1] == A.TH_GUIDE_FEATURES
+        out = np.empty((f.shape[0], A.TH_GUIDE_OUTPUTS), np.float32)
+        A.check(A.lib().th_guide_forward(self._h, _ptr(f), f.shape[0], _ptr(out)))
+        return out
+
+    def train_batch(self, features, dirs, weights, update=True):
+        """One Adam step (or the gradient only); returns (loss, gradient)."""
+        f = np.ascontiguousarray(features, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        w = np.ascontiguousarray(weights, np.float32)
+        n = f.shape[0]
+        assert f.shape == (n, A.TH_GUIDE_FEATURES) and d.shape == (n, 3) and w.shape == (n,)
+        loss = np.zeros(1, np.float32)
+        grad = np.empty(A.TH_GUIDE_PARAMS, np.float32)
+        A.check(A.lib().th_guide_train_batch(self._h, _ptr(f), _ptr(d), _ptr(w), n,
+                                             1 if update else 0, _ptr(loss), _ptr(grad)))
+        return float(loss[0]), grad
+
+    def close(self):
+        if self._h:
+            A.lib().th_guide_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mfma_selftest(ctx: GpuContext, a, b):
+    """C = A (16x32) B (32x16) through one v_mfma_f32_16x16x32_bf16 with this library's
+    fragment mapping (inputs rounded to BF16)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == (16, 32) and b.shape == (32, 16)
+    c = np.empty((16, 16), np.float32)
+    A.check(A.lib().th_guide_mfma_selftest(ctx.handle, _ptr(a), _ptr(b), _ptr(c)))
+    return c
