@@ -368,6 +368,36 @@ int32_t th_guide_train_batch(th_guide* guide, const float* features, const float
                              const float* weights, uint32_t count, int32_t update, float* loss,
                              float* grad);
 /* One v_mfma_f32_16x16x32_bf16 with this library's fragment mapping: C = A(16x32) B(32x16). */
+
+/* Guided rendering: `params->samples` passes of one sample per pixel (per-sample random
+ * streams, TH_RNG_PER_SAMPLE), folded into the scene's accumulation buffer like
+ * th_render.  At diffuse vertices the continuation direction comes from the guide with
+ * probability alpha and from the reference's cosine lobe otherwise; the estimator
+ * weights by the one-sample MIS density alpha q + (1 - alpha) cos/pi and uses the same
+ * density in the environment MIS, so it stays unbiased for any network.  With train != 0
+ * every pass records diffuse vertices (features, sampled direction, and the weight
+ * f cos L_i / pdf once the path completes) and runs Adam steps on them; alpha is treated
+ * as 0 until the guide has taken its first step.  With alpha = 0 and train = 0 the image
+ * equals th_render's TH_RNG_PER_SAMPLE image with one sample group. */
+typedef struct th_guided_options {
+    float alpha;              /* guide probability at diffuse vertices, [0, 0.95]      */
+    uint32_t train;           /* 1 = record samples and train after every pass        */
+    uint32_t steps_per_pass;  /* Adam steps per pass (0 = 1)                           */
+    uint32_t batch;           /* samples per step (0 = 2^18)                           */
+    uint32_t reserved[4];
+} th_guided_options;
+
+typedef struct th_guided_stats {
+    uint64_t diffuse_vertices; /* deferred diffuse vertices over the call               */
+    uint64_t records;          /* training samples recorded                            */
+    uint32_t train_steps;      /* Adam steps taken by this call                        */
+    float last_loss;           /* loss of the last step                                */
+    uint64_t reserved[2];
+} th_guided_stats;
+
+int32_t th_render_guided(th_context* ctx, th_scene* scene, th_guide* guide,
+                         const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
+                         const th_guided_options* options, th_guided_stats* stats);
 int32_t th_guide_mfma_selftest(th_context* ctx, const float* a, const float* b, float* c);
 
 /* Linear HDR float4 -> RGBA8 with exposure, ACES fitted and sRGB OETF

@@ -147,6 +147,16 @@ class Stats(C.Structure):
 
 # name -> (restype, argtypes) for every entry point declared in include/toaster_hip.h
 P = C.c_void_p
+class GuidedOptions(C.Structure):
+    _fields_ = [("alpha", f32), ("train", u32), ("steps_per_pass", u32), ("batch", u32),
+                ("reserved", u32 * 4)]
+
+
+class GuidedStats(C.Structure):
+    _fields_ = [("diffuse_vertices", C.c_uint64), ("records", C.c_uint64), ("train_steps", u32),
+                ("last_loss", f32), ("reserved", C.c_uint64 * 2)]
+
+
 SIGNATURES = {
     "th_pack_scene": (C.c_int32, [P, C.c_int32, C.POINTER(P)]),
     "th_packed_view": (C.c_int32, [P, C.POINTER(SceneDesc)]),
@@ -182,6 +192,8 @@ SIGNATURES = {
     "th_guide_forward": (C.c_int32, [P, P, u32, P]),
     "th_guide_train_batch": (C.c_int32, [P, P, P, P, u32, C.c_int32, P, P]),
     "th_guide_mfma_selftest": (C.c_int32, [P, P, P, P]),
+    "th_render_guided": (C.c_int32, [P, P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
+                                     C.POINTER(GuidedOptions), C.POINTER(GuidedStats)]),
 }
 
 _lib = None

@@ -24,7 +24,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
               "-Wno-unused-function"]
 
-HIP_SOURCES = ["th_runtime.hip", "th_wavefront.hip", "th_batch.hip", "th_guide.hip"]
+HIP_SOURCES = ["th_runtime.hip", "th_wavefront.hip", "th_batch.hip", "th_guide.hip", "th_guided.hip"]
 HOST_SOURCES = ["th_pack.cpp"]
 
 

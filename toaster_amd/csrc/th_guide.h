@@ -45,35 +45,32 @@ TH_HD float bf2f(uint16_t b) {
 }
 
 // ------------------------------------------------------------------ encoding
-// Scene-bounds-normalised position p01 in [0,1]^3 -> sin/cos at 4 octaves (24), normal
-// and outgoing direction -> one-blob with 4 Gaussian bins per component (12 + 12).
-TH_HD void encode(v3 p01, v3 n, v3 wo, float* out /* TG_IN */) {
-  const float TWO_PI = 6.2831855f;
-  float pc[3] = {p01.x, p01.y, p01.z};
-  int o = 0;
-  for (int a = 0; a < 3; ++a) {
-    float f = 1.0f;
-    for (int k = 0; k < 4; ++k) {
-      float s, c;
-      th::sincos_(TWO_PI * f * pc[a], &s, &c);
-      out[o++] = s;
-      out[o++] = c;
-      f *= 2.0f;
-    }
-  }
-  float dc[6] = {n.x, n.y, n.z, wo.x, wo.y, wo.z};
-  for (int a = 0; a < 6; ++a) {
-    float u = 0.5f * (dc[a] + 1.0f);  // [-1,1] -> [0,1]
-    for (int k = 0; k < 4; ++k) {
-      float c = ((float)k + 0.5f) * 0.25f;
-      float d = (u - c) * 4.0f;  // sigma = 1/4
+TH_HD float fexp(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-      out[o++] = __expf(-0.5f * d * d);
+  return __expf(x);
 #else
-      out[o++] = expf(-0.5f * d * d);
+  return expf(x);
 #endif
-    }
+}
+
+// Feature j (0..47) of the encoding of a vertex: scene-bounds-normalised position p01
+// in [0,1]^3 -> sin/cos(2 pi 2^k x) for k = 0..3 per axis (features 0..23); the normal
+// and outgoing direction components c -> u = (c + 1)/2 spread over 4 Gaussian bins at
+// (b + 1/2)/4 with sigma 1/4 (features 24..47).  Each feature is computed on its own,
+// so lanes can share the work of one row.
+TH_HD float encode_feature(int j, v3 p01, v3 n, v3 wo) {
+  if (j < 24) {
+    int a = j >> 3, k = (j & 7) >> 1;
+    float x = a == 0 ? p01.x : (a == 1 ? p01.y : p01.z);
+    float s, c;
+    th::sincos_(6.2831855f * (float)(1 << k) * x, &s, &c);
+    return (j & 1) ? c : s;
   }
+  int c = (j - 24) >> 2, b = (j - 24) & 3;
+  float v = c == 0 ? n.x : c == 1 ? n.y : c == 2 ? n.z : c == 3 ? wo.x : c == 4 ? wo.y : wo.z;
+  float u = 0.5f * (v + 1.0f);
+  float d = (u - ((float)b + 0.5f) * 0.25f) * 4.0f;
+  return fexp(-0.5f * d * d);
 }
 
 // ------------------------------------------------------------------ mixture
@@ -88,13 +85,6 @@ TH_HD float softplus(float x) {
   return x > 15.0f ? x : __logf(1.0f + __expf(x));
 #else
   return x > 15.0f ? x : logf(1.0f + expf(x));
-#endif
-}
-TH_HD float fexp(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __expf(x);
-#else
-  return expf(x);
 #endif
 }
 TH_HD float flog(float x) {

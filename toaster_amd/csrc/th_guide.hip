@@ -283,6 +283,86 @@ __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
   for (int i = 0; i < 4; ++i) C[(g * 4 + i) * 16 + r] = acc[i];
 }
 
+// Fused inference over a queue of deferred diffuse vertices, one wave per 16-row tile
+// and persistent over the queue.  Lane (r, g) encodes features 12g..12g+11 of row r into
+// an LDS tile; the four layers then run on MFMA with every weight fragment held in
+// registers (loaded once per wave) and activations passed through the same tile (BF16,
+// row stride 72 elements = 144 bytes, 16-byte aligned rows); each row's 40 outputs are
+// written in FP32.  Queue entry i names the path whose vertex is (gp.xyz, gn.xyz) with
+// outgoing direction -gd.xyz.
+__global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__ woi,
+                                                    const float* __restrict__ bias,
+                                                    const uint32_t* __restrict__ count_ptr,
+                                                    const uint32_t* __restrict__ queue,
+                                                    const float4* __restrict__ gp,
+                                                    const float4* __restrict__ gn,
+                                                    const float4* __restrict__ gd, float4 bmin,
+                                                    float4 bscale, float* __restrict__ O) {
+  __shared__ __align__(16) uint16_t T[16 * 72];
+  const uint32_t count = *count_ptr;
+  if (blockIdx.x * 16u >= count) return;  // one wave per block: uniform exit
+  const uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
+  tg_bf16x8 W[4][4][2];
+  float bv[4][4];
+#pragma unroll
+  for (int L = 0; L < 4; ++L)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      bv[L][nb] = bias[L * 64 + nb * 16 + r];
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) W[L][nb][kb] = ld8(woi + L * 4096 + (nb * 16 + r) * 64 + kb * 32 + g * 8);
+    }
+  for (uint32_t base = blockIdx.x * 16u; base < count; base += gridDim.x * 16u) {
+    const uint32_t row = base + r;
+    v3 p01 = th::mk(0.5f, 0.5f, 0.5f), n = th::mk(0.0f, 1.0f, 0.0f), wo = n;
+    if (row < count) {
+      const uint32_t k = queue[row];
+      const float4 a = gp[k], b = gn[k], c = gd[k];
+      p01 = th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z);
+      n = th::mk(b.x, b.y, b.z);
+      wo = th::mk(-c.x, -c.y, -c.z);
+    }
+#pragma unroll
+    for (int j = 0; j < 12; ++j) T[r * 72 + 12 * g + j] = tg::f2bf(tg::encode_feature(12 * (int)g + j, p01, n, wo));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) T[r * 72 + 48 + 4 * g + j] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int L = 0; L < 4; ++L) {
+      const tg_bf16x8 a0 = __builtin_bit_cast(tg_bf16x8, *(const uint4*)(T + r * 72 + g * 8));
+      const tg_bf16x8 a1 = __builtin_bit_cast(tg_bf16x8, *(const uint4*)(T + r * 72 + 32 + g * 8));
+      const int nbs = L < 3 ? 4 : 3;  // the last layer has 40 outputs: three 16-wide tiles
+      tg_f32x4 acc[4];
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        if (nb < nbs) {
+          acc[nb] = mfma(a0, W[L][nb][0], tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+          acc[nb] = mfma(a1, W[L][nb][1], acc[nb]);
+        }
+      }
+      __syncthreads();  // every lane has read the tile before it is overwritten
+      if (L < 3) {
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            float v = acc[nb][i] + bv[L][nb];
+            T[(4 * g + i) * 72 + nb * 16 + r] = tg::f2bf(v > 0.0f ? v : 0.0f);
+          }
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int nb = 0; nb < 3; ++nb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint32_t col = nb * 16 + r, rr = base + 4 * g + i;
+            if (col < (uint32_t)TG_OUT && rr < count) O[(size_t)rr * TG_OUT + col] = acc[nb][i] + bv[3][nb];
+          }
+      }
+    }
+  }
+}
+
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
 }  // namespace
@@ -333,11 +413,12 @@ static int32_t guide_reserve(th_guide* g, uint32_t rows) {
   return TH_OK;
 }
 
-// Forward over `rows` rows of g->X0 (BF16, 64 wide) into H1..H3 and O.
-int32_t th_guide_forward_rows(th_guide* g, const uint32_t* d_count, uint32_t rows, hipStream_t st) {
+// Forward over `rows` rows of X (BF16, 64 wide) into the workspace H1..H3 and O.
+static int32_t th_guide_forward_rows(th_guide* g, const uint16_t* X, const uint32_t* d_count,
+                                     uint32_t rows, hipStream_t st) {
   uint32_t grid = (round_up(rows, 64) / 64);  // 4 waves x 16 rows per block
   if (grid == 0) return TH_OK;
-  hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->X0, g->d_woi + 0 * 4096,
+  hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, X, g->d_woi + 0 * 4096,
                      g->d_bias + 0, d_count, rows, (void*)g->H1, nullptr);
   hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->H1, g->d_woi + 1 * 4096,
                      g->d_bias + 64, d_count, rows, (void*)g->H2, nullptr);
@@ -349,16 +430,18 @@ int32_t th_guide_forward_rows(th_guide* g, const uint32_t* d_count, uint32_t row
   return TH_OK;
 }
 
-// One training step on rows [0, count) of feat/dirs/w already on the device
-// (features already encoded into X0).  lr = 0 computes the gradient only.
-static int32_t guide_step(th_guide* g, uint32_t count, float lr) {
+// One training step on `count` samples already on the device: BF16 features X (64 wide,
+// zero beyond 48; rows up to count rounded to 32 must be finite), directions and
+// weights.  lr = 0 computes the gradient only.  Workspace must hold count rows.
+static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, const float* w,
+                          uint32_t count, float lr) {
   hipStream_t st = g->ctx->stream;
   uint32_t rows = round_up(count, 32);
-  int32_t rc = th_guide_forward_rows(g, nullptr, rows, st);
+  int32_t rc = th_guide_forward_rows(g, X, nullptr, rows, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, g->scal, 16u);
-  hipLaunchKernelGGL(k_weight_sum, dim3(1), dim3(1024), 0, st, g->w, count, g->scal);
-  hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, g->O, g->dirs, g->w, count,
+  hipLaunchKernelGGL(k_weight_sum, dim3(1), dim3(1024), 0, st, w, count, g->scal);
+  hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, g->O, dirs, w, count,
                      rows, g->scal, g->dZ4, g->scal + 1);
   uint32_t rows_per = round_up((rows + kWgradBlocks - 1) / kWgradBlocks, 32);
   uint32_t nb = (rows + rows_per - 1) / rows_per;
@@ -375,7 +458,7 @@ static int32_t guide_step(th_guide* g, uint32_t count, float lr) {
   hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H1, g->dZ2, rows_per, rows, P + 1 * s3);
   hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ2, g->d_wio + 1 * 4096,
                      nullptr, nullptr, rows, (void*)g->dZ1, g->H1);
-  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->X0, g->dZ1, rows_per, rows, P + 0 * s3);
+  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, X, g->dZ1, rows_per, rows, P + 0 * s3);
   TH_CHECK(hipGetLastError());
   float bc1 = 1.0f, bc2 = 1.0f;
   if (lr > 0.0f) {
@@ -390,6 +473,35 @@ static int32_t guide_step(th_guide* g, uint32_t count, float lr) {
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
+
+// ====================================================================== renderer interface
+int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const uint32_t* queue,
+                    const float4* gp, const float4* gn, const float4* gd, const float* bmin,
+                    const float* bmax, float* O, hipStream_t st) {
+  uint32_t grid = std::min<uint32_t>((max_rows + 15u) / 16u, 8192u);
+  if (grid == 0) return TH_OK;
+  float4 lo = make_float4(bmin[0], bmin[1], bmin[2], 0.0f), sc;
+  float e[3];
+  for (int a = 0; a < 3; ++a) e[a] = bmax[a] - bmin[a];
+  sc = make_float4(e[0] > 0.0f ? 1.0f / e[0] : 0.0f, e[1] > 0.0f ? 1.0f / e[1] : 0.0f,
+                   e[2] > 0.0f ? 1.0f / e[2] : 0.0f, 0.0f);
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, st, g->d_woi, g->d_bias, d_count, queue, gp,
+                     gn, gd, lo, sc, O);
+  TH_CHECK(hipGetLastError());
+  return TH_OK;
+}
+
+int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count,
+                    float* loss_out) {
+  int32_t rc = guide_reserve(g, count);
+  if (rc) return rc;
+  if ((rc = guide_step(g, X, dirs, w, count, g->lr))) return rc;
+  if (loss_out) TH_CHECK(hipMemcpyAsync(loss_out, g->scal + 1, 4, hipMemcpyDeviceToHost, g->ctx->stream));
+  return TH_OK;
+}
+
+uint32_t guide_steps(const th_guide* g) { return g->step; }
+th_context* guide_context(const th_guide* g) { return g->ctx; }
 
 // ====================================================================== C ABI
 static void host_init_params(uint32_t seed, std::vector<float>& p) {
@@ -497,7 +609,7 @@ int32_t th_guide_forward(th_guide* g, const float* feats, uint32_t count, float*
   uint32_t rows = round_up(count, 32);
   TH_CHECK(hipMemcpyAsync(g->feat, feats, (size_t)count * TG_IN * 4, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_encode_host, dim3((rows + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
-  if ((rc = th_guide_forward_rows(g, nullptr, rows, st))) return rc;
+  if ((rc = th_guide_forward_rows(g, g->X0, nullptr, rows, st))) return rc;
   std::vector<float> o((size_t)rows * TG_OUTP);
   TH_CHECK(hipMemcpyAsync(o.data(), g->O, o.size() * 4, hipMemcpyDeviceToHost, st));
   TH_CHECK(hipStreamSynchronize(st));
@@ -517,7 +629,7 @@ int32_t th_guide_train_batch(th_guide* g, const float* feats, const float* dirs,
   TH_CHECK(hipMemcpyAsync(g->dirs, dirs, (size_t)count * 12, hipMemcpyHostToDevice, st));
   TH_CHECK(hipMemcpyAsync(g->w, weights, (size_t)count * 4, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_encode_host, dim3((rows + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
-  if ((rc = guide_step(g, count, update ? g->lr : 0.0f))) return rc;
+  if ((rc = guide_step(g, g->X0, g->dirs, g->w, count, update ? g->lr : 0.0f))) return rc;
   if (loss) TH_CHECK(hipMemcpyAsync(loss, g->scal + 1, 4, hipMemcpyDeviceToHost, st));
   if (grad) TH_CHECK(hipMemcpyAsync(grad, g->d_grad, TG_NPARAMS * 4, hipMemcpyDeviceToHost, st));
   TH_CHECK(hipStreamSynchronize(st));

@@ -1,0 +1,676 @@
+// th_guided.hip -- path tracing with the learned guiding distribution.
+//
+// A wavefront over one sample of every pixel per pass: path k of pass s is pixel k's
+// sample s, seeded as th_render's per-sample mode.  Per bounce:
+//   extend   closest hit, 4 lanes per ray over the 4-wide BVH;
+//   shade A  the reference loop body (shaders/pathtrace_spheres.wgsl:817-911) for
+//            misses, emitters, metals and dielectrics; a diffuse vertex is parked
+//            (position, normal, albedo) and appended to the guide queue;
+//   infer    the fused guide MLP on MFMA over the queue (th_guide.hip);
+//   shade B  diffuse vertices: direct light and environment terms with deferred
+//            visibility, then the continuation by one-sample MIS between the guide
+//            (probability alpha) and the reference's cosine lobe;
+//   shadow   any hit, 4 lanes per ray;
+//   finish   visible direct terms, completed samples, compaction of live paths.
+// After each pass the training records of the pass receive their weights and the guide
+// takes its Adam steps on them.
+//
+// Estimator.  The continuation density is p(w) = alpha q(w) + (1 - alpha) cos/pi, the
+// throughput takes albedo (cos/pi) / p(w), and both the environment term's MIS weight
+// and an escaping ray's MIS weight use p: unbiased for any network, and with alpha = 0
+// every operation and random draw is the reference's (the image equals th_render's
+// per-sample image with one sample group).
+//
+// Training target.  A record at vertex v stores the features, the sampled direction w,
+// the throughput beta_v before the bounce and (set by finish) the path radiance L_v
+// after v's direct terms.  When the path completes with radiance L, the record's weight
+// is lum((L - L_v) / beta_v) = f cos L_i(w) / p(w): the Monte Carlo weight of the product
+// target f cos L_i under the sampling density, which the loss -sum w log q / sum w fits.
+#include <algorithm>
+#include <vector>
+
+#include "th_guide.h"
+#include "th_internal.h"
+
+namespace {
+
+#define GD_INCLUDE_EMISSIVE 1u
+#define GD_PREV_DIFFUSE 2u
+#define GD_DONE 4u
+#define GD_PEND_LIGHT 8u
+#define GD_PEND_ENV 16u
+#define GD_NO_RECORD 0xffffffffu
+
+struct GdBufs {
+  float4 *o, *d, *thr, *rad, *acc;  // o.w previous continuation pdf, d.w flags, thr.w record
+  uint32_t* rng;
+  uint2* hit;
+  float4 *gp, *gn;  // parked diffuse vertex: (p, albedo.x), (n, albedo.y)
+  float* gz;        // albedo.z
+  float4 *sho, *sh0, *c0, *sh1, *c1;
+  uint32_t* vis;
+  uint32_t *q0, *q1, *qs, *qg;
+  uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [4] records, [5] diffuse total
+  float* O;       // guide outputs, TG_OUT per guide-queue row
+  uint16_t* rX;   // training records: BF16 features, 64 wide (zero beyond 48)
+  float *rdir, *rw;
+  float4 *rbeta, *rL;
+  uint32_t* rpath;
+  uint32_t rcap;
+};
+
+struct GdArgs {
+  uint32_t n;           // paths per pass (pixels)
+  uint32_t smp;         // sample index of the pass
+  uint32_t bounce;      // bounce of this iteration, 0-based
+  float alpha;          // guide probability (0: cosine lobe only)
+  uint32_t defer;       // diffuse vertices go through the guide queue
+  uint32_t record;      // record training samples
+  uint32_t rec_thresh;  // record an eligible vertex when its hash < rec_thresh
+  uint32_t rec_all;     // record every eligible vertex
+  float bmin[3], bscale[3];
+};
+
+// Appends `live` lanes' values to a queue: one atomic per wave, order kept within a wave.
+// Returns the slot of this lane (valid when live).
+__device__ __forceinline__ uint32_t gd_push(bool live, uint32_t value, uint32_t* queue, uint32_t* count,
+                                            uint32_t cap = 0xffffffffu) {
+  uint64_t m = __ballot(live);
+  if (m == 0) return 0;
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)first, 64);
+  uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (live && queue && idx < cap) queue[idx] = value;
+  return idx;
+}
+
+__global__ void gd_reset(GdBufs B, uint32_t all) {
+  if (threadIdx.x < 5u || (all && threadIdx.x < 8u)) B.cnt[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------- generate
+__global__ void __launch_bounds__(256) gd_generate(GdBufs B, GdArgs A, TH_GpuRenderParams P,
+                                                   TH_GpuCamera cam) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = false;
+  if (k < A.n) {
+    float4 a = A.smp == 0u ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : B.acc[k];
+    if (P.max_bounces == 0u) {  // ray_color returns zero (wgsl:536)
+      a.w = a.w + 1.0f;
+    } else {
+      uint32_t rng = th::pcg_hash((k ^ (P.frame_index * 9781u)) + A.smp * 0x9E3779B9u);
+      v3 o, d;
+      camera_ray(cam, P, k % P.width, k / P.width, &rng, &o, &d);
+      B.o[k] = make_float4(o.x, o.y, o.z, 0.0f);
+      B.d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(GD_INCLUDE_EMISSIVE));
+      B.thr[k] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(GD_NO_RECORD));
+      B.rad[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      B.rng[k] = rng;
+      live = true;
+    }
+    B.acc[k] = a;
+  }
+  gd_push(live, k, B.q0, &B.cnt[0]);
+}
+
+// ---------------------------------------------------------------- extend / shadow
+__global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRenderParams P, uint32_t li) {
+  extern __shared__ uint2 lds_stack[];
+  const uint32_t* q = li ? B.q1 : B.q0;
+  const uint32_t n = B.cnt[li];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // queues filled later in this iteration
+    B.cnt[2] = 0;
+    B.cnt[3] = 0;
+  }
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  const uint32_t stride = gridDim.x * blockDim.x / 4u;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / 4u;; i += stride) {
+    bool act = i < n;  // uniform per quad
+    if (!__any(act)) break;
+    if (act) {
+      uint32_t k = q[i];
+      float4 o4 = B.o[k], d4 = B.d[k];
+      v3 o = th::mk(o4.x, o4.y, o4.z), d = th::mk(d4.x, d4.y, d4.z);
+      float t;
+      uint32_t slot;
+      if (S.node_count == 0u) {
+        RayStats<false> rs;
+        t = closest_linear<false>(S, P.sphere_count, P.triangle_count, o, d, TH_FLT_MAX, rs, &slot);
+      } else {
+        traverse_quad<1>(S, (const Child4*)S.nodes4, S.root4, o, d, TH_FLT_MAX, st, &t, &slot);
+      }
+      if ((threadIdx.x & 3u) == 0u) B.hit[k] = make_uint2(__float_as_uint(t), slot);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) gd_shadow(GdBufs B, DevScene S, TH_GpuRenderParams P) {
+  extern __shared__ uint2 lds_stack[];
+  const uint32_t n = B.cnt[2];
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  const uint32_t stride = gridDim.x * blockDim.x / 4u;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / 4u;; i += stride) {
+    bool act = i < n;
+    if (!__any(act)) break;
+    if (act) {
+      uint32_t e = B.qs[i], k = e >> 1;
+      float4 so = B.sho[k], sd = (e & 1u) ? B.sh1[k] : B.sh0[k];
+      v3 o = th::mk(so.x, so.y, so.z), d = th::mk(sd.x, sd.y, sd.z);
+      float limit = sd.w, t;
+      uint32_t slot;
+      if (S.node_count == 0u) {
+        RayStats<false> rs;
+        t = closest_linear<false>(S, P.sphere_count, P.triangle_count, o, d, limit, rs, &slot);
+      } else {
+        traverse_quad<2>(S, (const Child4*)S.nodes4, S.root4, o, d, limit, st, &t, &slot);
+      }
+      if ((threadIdx.x & 3u) == 0u) B.vis[e] = (t < limit) ? 0u : 1u;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- diffuse vertex
+// Direct light and environment terms of a diffuse vertex (wgsl:846-847) with their
+// visibility deferred, then the continuation.  Without a mixture (or alpha = 0) the
+// continuation is the reference's cosine lobe (wgsl:849-857), bit for bit; otherwise
+// one-sample MIS between the guide and the cosine lobe.  Returns false when the
+// continuation has zero contribution (a guide direction below the surface).
+__device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, false>& it,
+                                              const TH_GpuRenderParams& P, uint32_t k, const v3& p,
+                                              const v3& n, const v3& albedo, const tg::Mixture* m,
+                                              float alpha, v3& throughput, uint32_t& rng, v3& nd,
+                                              float& pdf, bool& sh_light, bool& sh_env) {
+  const bool guided = m != nullptr && alpha > 0.0f;
+  {
+    v3 so, dir, term;
+    float limit;
+    if (it.light_candidate(p, n, albedo, &rng, &so, &dir, &limit, &term)) {
+      v3 c = th::mul(throughput, term);
+      B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
+      B.sh0[k] = make_float4(dir.x, dir.y, dir.z, limit);
+      B.c0[k] = make_float4(c.x, c.y, c.z, 0.0f);
+      sh_light = true;
+    }
+  }
+  if (P.background_kind == 2u) {
+    EnvSample es;
+    float sc;
+    if (it.env_candidate(n, &rng, &es, &sc)) {
+      float bp = sc / TH_PI;
+      if (guided) bp = alpha * tg::mixture_pdf(*m, es.dir) + (1.0f - alpha) * bp;
+      v3 c = th::mul(throughput, it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, bp)));
+      v3 so = th::add(p, th::muls(n, TH_MIN_DISTANCE));
+      B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
+      B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);
+      B.c1[k] = make_float4(c.x, c.y, c.z, 0.0f);
+      sh_env = true;
+    }
+  }
+  if (!guided) {
+    v3 dir = th::add(n, it.random_unit_vector(&rng));
+    if (th::dot(dir, dir) < TH_EPSILON) dir = n;
+    nd = th::normalize(dir);
+    throughput = th::mul(throughput, albedo);
+    pdf = th::fmax_(0.0f, th::dot(n, nd)) / TH_PI;
+    return true;
+  }
+  if (th::rand01(&rng) < alpha) {
+    float u0 = th::rand01(&rng);
+    float u1 = th::rand01(&rng);
+    float u2 = th::rand01(&rng);
+    nd = tg::mixture_sample(*m, u0, u1, u2);
+  } else {
+    v3 dir = th::add(n, it.random_unit_vector(&rng));
+    if (th::dot(dir, dir) < TH_EPSILON) dir = n;
+    nd = th::normalize(dir);
+  }
+  float c = th::dot(n, nd);
+  if (!(c > 0.0f)) {
+    pdf = 0.0f;
+    return false;
+  }
+  float cp = c / TH_PI;
+  pdf = alpha * tg::mixture_pdf(*m, nd) + (1.0f - alpha) * cp;
+  throughput = th::mul(throughput, th::muls(albedo, cp / pdf));
+  return true;
+}
+
+// ---------------------------------------------------------------- shade A
+__global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
+                                                 const float* __restrict__ srgb_lut, uint32_t li) {
+  const uint32_t* q = li ? B.q1 : B.q0;
+  const uint32_t n = B.cnt[li];
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[li ^ 1u] = 0;  // next live queue
+  Stack st{nullptr, 0, 0};
+  LaneCounters<false> lc;
+  Integrator<1, false> it{S, P, srgb_lut, st, lc};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
+    bool act = i < n;
+    if (!__any(act)) break;
+    bool sh_light = false, sh_env = false, park = false;
+    uint32_t k = 0;
+    if (act) {
+      k = q[i];
+      float4 o4 = B.o[k], d4 = B.d[k], t4 = B.thr[k], r4 = B.rad[k];
+      uint32_t rng = B.rng[k];
+      uint2 h = B.hit[k];
+      v3 d = th::mk(d4.x, d4.y, d4.z);
+      v3 throughput = th::mk(t4.x, t4.y, t4.z), radiance = th::mk(r4.x, r4.y, r4.z);
+      uint32_t flags = __float_as_uint(d4.w) & (GD_INCLUDE_EMISSIVE | GD_PREV_DIFFUSE);
+      float prev_pdf = o4.w;
+      float t = __uint_as_float(h.x);
+      uint32_t slot = h.y;
+      bool done = false;
+      v3 nd = d;
+      float4 o_next = o4;
+      if (slot == THD_NO_SLOT || t > TH_MAX_DISTANCE) {  // miss (wgsl:819-834)
+        if (P.background_kind == 0u) {
+          float lt = (th::normalize(d).y + 1.0f) * 0.5f;
+          radiance = th::add(radiance, th::mul(throughput, th::mix(th::mk(1.0f, 1.0f, 1.0f),
+                                                                    th::mk(0.35f, 0.65f, 1.0f), lt)));
+        } else if (P.background_kind == 2u) {
+          float mis = 1.0f;
+          if (flags & GD_PREV_DIFFUSE) mis = it.power_heuristic(prev_pdf, it.env_pdf(d));
+          radiance = th::add(radiance, th::muls(th::mul(throughput, it.env_lookup(d)), mis));
+        }
+        done = true;
+      } else {
+        v3 o = th::mk(o4.x, o4.y, o4.z);
+        Hit rec;
+        if (S.node_count == 0u)
+          finalize_linear(S, slot, o, d, t, &rec);
+        else
+          finalize_slot(S, slot, o, d, t, &rec);
+        const TH_GpuMaterial& m = S.materials[rec.mat];
+        v3 att = th::mk(0, 0, 0);
+        bool brk = false;
+        if (m.kind == 0u) {
+          v3 albedo = it.base_color(m, rec);
+          if (A.defer) {
+            B.gp[k] = make_float4(rec.p.x, rec.p.y, rec.p.z, albedo.x);
+            B.gn[k] = make_float4(rec.n.x, rec.n.y, rec.n.z, albedo.y);
+            B.gz[k] = albedo.z;
+            park = true;
+          } else {
+            float pdf;
+            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, throughput, rng, nd, pdf,
+                          sh_light, sh_env);
+            flags = GD_PREV_DIFFUSE;
+            o_next = make_float4(rec.p.x, rec.p.y, rec.p.z, pdf);
+            if (A.bounce + 1u >= P.max_bounces) done = true;
+          }
+        } else {
+          if (m.kind == 1u) {
+            v3 refl = it.reflect_vec(d, rec.n);
+            float rough = th::clampf(m.params[0], 0.0f, 1.0f);
+            v3 dir = th::add(refl, th::smul(rough, it.random_unit_vector(&rng)));
+            if (th::dot(dir, rec.n) <= 0.0f) brk = true;
+            nd = th::normalize(dir);
+            att = th::mk(m.albedo[0], m.albedo[1], m.albedo[2]);
+            flags = GD_INCLUDE_EMISSIVE;
+          } else if (m.kind == 2u) {
+            float ratio = rec.front ? 1.0f / m.params[1] : m.params[1];
+            v3 ud = th::normalize(d);
+            float ct = th::fmin_(1.0f, th::dot(th::neg(ud), rec.n));
+            float stt = th::fsqrt(1.0f - ct * ct);
+            bool cannot = (ratio * stt) > 1.0f;
+            v3 dir;
+            if (cannot || it.reflectance(ct, ratio) > th::rand01(&rng))
+              dir = it.reflect_vec(ud, rec.n);
+            else
+              dir = it.refract_vec(ud, rec.n, ratio);
+            nd = th::normalize(dir);
+            att = th::mk(1, 1, 1);
+            flags = GD_INCLUDE_EMISSIVE;
+          } else {
+            if (m.kind == 3u && (flags & GD_INCLUDE_EMISSIVE))
+              radiance = th::add(radiance, th::muls(th::mul(throughput, th::mk(m.albedo[0], m.albedo[1],
+                                                                                m.albedo[2])),
+                                                    m.params[2]));
+            brk = true;
+          }
+          if (brk) {
+            done = true;
+          } else {
+            throughput = th::mul(throughput, att);
+            o_next = make_float4(rec.p.x, rec.p.y, rec.p.z, prev_pdf);
+            if (A.bounce + 1u >= P.max_bounces) done = true;
+          }
+        }
+      }
+      if (!park) {
+        uint32_t f = flags | (done ? GD_DONE : 0u) | (sh_light ? GD_PEND_LIGHT : 0u) |
+                     (sh_env ? GD_PEND_ENV : 0u);
+        B.o[k] = o_next;
+        B.d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(f));
+        B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, t4.w);
+        B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        B.rng[k] = rng;
+      }
+    }
+    gd_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
+    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
+    gd_push(park, k, B.qg, &B.cnt[3]);
+  }
+}
+
+// ---------------------------------------------------------------- shade B
+__global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
+                                                 const float* __restrict__ srgb_lut) {
+  const uint32_t n = B.cnt[3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.cnt[5], n);
+  Stack st{nullptr, 0, 0};
+  LaneCounters<false> lc;
+  Integrator<1, false> it{S, P, srgb_lut, st, lc};
+  const bool use_guide = A.alpha > 0.0f;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
+    bool act = i < n;
+    if (!__any(act)) break;
+    bool sh_light = false, sh_env = false, rec_ok = false;
+    uint32_t k = 0;
+    v3 p, nrm, wo, nd, beta;
+    if (act) {
+      k = B.qg[i];
+      float4 gp = B.gp[k], gn = B.gn[k], d4 = B.d[k], t4 = B.thr[k];
+      p = th::mk(gp.x, gp.y, gp.z);
+      nrm = th::mk(gn.x, gn.y, gn.z);
+      wo = th::mk(-d4.x, -d4.y, -d4.z);
+      v3 albedo = th::mk(gp.w, gn.w, B.gz[k]);
+      v3 throughput = th::mk(t4.x, t4.y, t4.z);
+      beta = throughput;
+      uint32_t rng = B.rng[k];
+      tg::Mixture mix;
+      if (use_guide) tg::decode(B.O + (size_t)i * TG_OUT, &mix);
+      float pdf;
+      bool cont = shade_diffuse(B, it, P, k, p, nrm, albedo, use_guide ? &mix : nullptr, A.alpha, throughput,
+                                rng, nd, pdf, sh_light, sh_env);
+      bool done = !cont || A.bounce + 1u >= P.max_bounces;
+      if (A.record && !done) {
+        uint32_t h = th::pcg_hash((k * 0x9E3779B9u) ^ (A.bounce * 0x85EBCA6Bu) ^
+                                  (A.smp * 0xC2B2AE35u + P.frame_index));
+        rec_ok = A.rec_all || h < A.rec_thresh;
+      }
+      uint32_t f = GD_PREV_DIFFUSE | (done ? GD_DONE : 0u) | (sh_light ? GD_PEND_LIGHT : 0u) |
+                   (sh_env ? GD_PEND_ENV : 0u);
+      B.o[k] = make_float4(p.x, p.y, p.z, pdf);
+      B.d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(f));
+      B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, __uint_as_float(GD_NO_RECORD));
+      B.rng[k] = rng;
+    }
+    gd_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
+    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
+    uint32_t r = gd_push(rec_ok, k, B.rpath, &B.cnt[4], B.rcap);
+    if (rec_ok && r < B.rcap) {
+      v3 p01 = th::mk((p.x - A.bmin[0]) * A.bscale[0], (p.y - A.bmin[1]) * A.bscale[1],
+                      (p.z - A.bmin[2]) * A.bscale[2]);
+      uint16_t* x = B.rX + (size_t)r * 64u;
+      for (int j = 0; j < TG_IN; ++j) x[j] = tg::f2bf(tg::encode_feature(j, p01, nrm, wo));
+      for (int j = TG_IN; j < 64; ++j) x[j] = 0;
+      B.rdir[3u * r] = nd.x;
+      B.rdir[3u * r + 1u] = nd.y;
+      B.rdir[3u * r + 2u] = nd.z;
+      B.rbeta[r] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+      B.thr[k].w = __uint_as_float(r);  // finish stores the radiance after this vertex
+    }
+  }
+}
+
+// ---------------------------------------------------------------- finish
+__global__ void __launch_bounds__(64) gd_finish(GdBufs B, TH_GpuRenderParams P, uint32_t li) {
+  const uint32_t* q = li ? B.q1 : B.q0;
+  uint32_t* qn = li ? B.q0 : B.q1;
+  const uint32_t n = B.cnt[li];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
+    bool act = i < n;
+    if (!__any(act)) break;
+    bool live = false;
+    uint32_t k = 0;
+    if (act) {
+      k = q[i];
+      float4 d4 = B.d[k];
+      uint32_t f = __float_as_uint(d4.w);
+      float4 r4 = B.rad[k];
+      if (f & (GD_PEND_LIGHT | GD_PEND_ENV)) {
+        // radiance += throughput * direct term, light first then environment (wgsl:846-847)
+        if ((f & GD_PEND_LIGHT) && B.vis[2u * k]) {
+          float4 c = B.c0[k];
+          r4 = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, 0.0f);
+        }
+        if ((f & GD_PEND_ENV) && B.vis[2u * k + 1u]) {
+          float4 c = B.c1[k];
+          r4 = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, 0.0f);
+        }
+        B.rad[k] = r4;
+      }
+      uint32_t rec = __float_as_uint(B.thr[k].w);
+      if (rec != GD_NO_RECORD) {
+        B.rL[rec] = r4;
+        B.thr[k].w = __uint_as_float(GD_NO_RECORD);
+      }
+      if (f & GD_DONE) {
+        // color += vec4(ray_color, 1) (wgsl:536)
+        float4 a = B.acc[k];
+        B.acc[k] = make_float4(a.x + r4.x, a.y + r4.y, a.z + r4.z, a.w + 1.0f);
+      } else {
+        B.d[k] = make_float4(d4.x, d4.y, d4.z, __uint_as_float(f & (GD_INCLUDE_EMISSIVE | GD_PREV_DIFFUSE)));
+        live = true;
+      }
+    }
+    gd_push(live, k, qn, &B.cnt[li ^ 1u]);
+  }
+}
+
+// ---------------------------------------------------------------- records, resolve
+// Weight of each record of the pass: lum((L - L_v) / beta_v), zero where not finite.
+__global__ void __launch_bounds__(256) gd_weights(GdBufs B) {
+  const uint32_t n = min(B.cnt[4], B.rcap);
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    float4 L = B.rad[B.rpath[r]], Lv = B.rL[r], b = B.rbeta[r];
+    float v0 = b.x > 0.0f ? (L.x - Lv.x) / b.x : 0.0f;
+    float v1 = b.y > 0.0f ? (L.y - Lv.y) / b.y : 0.0f;
+    float v2 = b.z > 0.0f ? (L.z - Lv.z) / b.z : 0.0f;
+    float w = 0.2126f * v0 + 0.7152f * v1 + 0.0722f * v2;
+    B.rw[r] = (w >= 0.0f && w < 3.0e38f) ? w : 0.0f;
+  }
+}
+
+// Sample average (wgsl:537) folded into the progressive running average (wgsl:538-547).
+__global__ void __launch_bounds__(256) gd_resolve(GdBufs B, GdArgs A, TH_GpuRenderParams P,
+                                                  float4* __restrict__ accum) {
+  uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= A.n) return;
+  float4 s = B.acc[px];
+  float ns = (float)P.samples;
+  float4 b = make_float4(s.x / ns, s.y / ns, s.z / ns, s.w / ns);
+  if (P.accumulated_samples == 0u) {
+    accum[px] = b;
+  } else {
+    float prev = (float)P.accumulated_samples;
+    float tot = prev + ns;
+    float4 o = accum[px];
+    accum[px] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                            (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+  }
+}
+
+}  // namespace
+
+struct th_guided_ws {
+  uint32_t capacity = 0, rcap = 0;
+  uint64_t diffuse_per_pass = 0;  // last pass's diffuse vertices (record-rate estimate)
+  GdBufs b;
+  std::vector<void*> allocs;
+};
+
+void th_guided_free(th_guided_ws* w) {
+  if (!w) return;
+  for (void* p : w->allocs) (void)hipFree(p);
+  delete w;
+}
+
+static int32_t gd_alloc(th_guided_ws* w, void** p, size_t bytes) {
+  TH_CHECK(hipMalloc(p, bytes));
+  w->allocs.push_back(*p);
+  TH_CHECK(hipMemset(*p, 0, bytes));
+  return TH_OK;
+}
+
+static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
+  if (s->gw && s->gw->capacity >= n && s->gw->rcap >= rcap) return TH_OK;
+  TH_CHECK(hipStreamSynchronize(s->ctx->stream));
+  th_guided_free(s->gw);
+  s->gw = nullptr;
+  th_guided_ws* w = new th_guided_ws();
+  w->capacity = n;
+  w->rcap = rcap;
+  GdBufs& b = w->b;
+  const size_t f4 = (size_t)n * sizeof(float4), u = (size_t)n * sizeof(uint32_t);
+  const size_t rows = (size_t)((n + 63u) / 64u) * 64u;
+  int32_t rc;
+  if ((rc = gd_alloc(w, (void**)&b.o, f4)) || (rc = gd_alloc(w, (void**)&b.d, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.thr, f4)) || (rc = gd_alloc(w, (void**)&b.rad, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.acc, f4)) || (rc = gd_alloc(w, (void**)&b.rng, u)) ||
+      (rc = gd_alloc(w, (void**)&b.hit, (size_t)n * sizeof(uint2))) ||
+      (rc = gd_alloc(w, (void**)&b.gp, f4)) || (rc = gd_alloc(w, (void**)&b.gn, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.gz, u)) || (rc = gd_alloc(w, (void**)&b.sho, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.sh0, f4)) || (rc = gd_alloc(w, (void**)&b.c0, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.sh1, f4)) || (rc = gd_alloc(w, (void**)&b.c1, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, u)) ||
+      (rc = gd_alloc(w, (void**)&b.q1, u)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * u)) ||
+      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.cnt, 16 * sizeof(uint32_t))) ||
+      (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(float))) ||
+      (rc = gd_alloc(w, (void**)&b.rX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
+      (rc = gd_alloc(w, (void**)&b.rdir, (size_t)rcap * 3 * sizeof(float))) ||
+      (rc = gd_alloc(w, (void**)&b.rw, (size_t)rcap * sizeof(float))) ||
+      (rc = gd_alloc(w, (void**)&b.rbeta, (size_t)rcap * sizeof(float4))) ||
+      (rc = gd_alloc(w, (void**)&b.rL, (size_t)rcap * sizeof(float4))) ||
+      (rc = gd_alloc(w, (void**)&b.rpath, (size_t)rcap * sizeof(uint32_t)))) {
+    th_guided_free(w);
+    return rc;
+  }
+  b.rcap = rcap;
+  s->gw = w;
+  return TH_OK;
+}
+
+extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, const TH_GpuRenderParams* p,
+                                    const TH_GpuCamera* cam, const th_guided_options* opt,
+                                    th_guided_stats* stats) {
+  if (!ctx || !s || !g || !p || !cam || !opt) return th_set_error(TH_E_INVALID, "null argument");
+  if (s->ctx != ctx || guide_context(g) != ctx)
+    return th_set_error(TH_E_INVALID, "scene or guide belongs to another context");
+  int32_t rc = th_check_render_params(s, p);
+  if (rc) return rc;
+  if (!(opt->alpha >= 0.0f && opt->alpha <= 0.95f))
+    return th_set_error(TH_E_INVALID, "alpha must lie in [0, 0.95]");
+  const uint32_t batch = opt->batch ? opt->batch : (1u << 18);
+  const uint32_t steps = opt->steps_per_pass ? opt->steps_per_pass : 1u;
+  if (batch % 32u != 0u || batch > (1u << 22) || steps > 64u)
+    return th_set_error(TH_E_INVALID, "batch must be a multiple of 32 up to 2^22, steps at most 64");
+  const uint64_t npix64 = (uint64_t)p->width * p->height;
+  if (npix64 > (1u << 26)) return th_set_error(TH_E_UNSUPPORTED, "image too large for the guided wavefront");
+  const uint32_t npix = (uint32_t)npix64;
+  const bool train = opt->train != 0u;
+  const uint64_t target = (uint64_t)batch * steps;
+  const uint32_t rcap = train ? (uint32_t)((2u * target + 31u) / 32u * 32u) : 32u;
+  TH_CHECK(hipSetDevice(ctx->device));
+  if ((rc = gd_reserve(s, npix, rcap))) return rc;
+  th_guided_ws* w = s->gw;
+  GdBufs B = w->b;
+  DevScene S = dev_scene(s);
+  const uint32_t block = 64;
+  const uint32_t depth = 3u * s->depth4 + 2u;
+  const size_t lds = (size_t)depth * (block / 4) * sizeof(uint2);
+  const uint32_t grid_trace = std::min<uint32_t>((npix * 4u + block - 1) / block, 8192u);
+  const uint32_t grid_scalar = std::min<uint32_t>((npix + block - 1) / block, 4096u);
+  const uint32_t grid_shadow = std::min<uint32_t>((2u * npix * 4u + block - 1) / block, 8192u);
+  GdArgs A;
+  memset(&A, 0, sizeof A);
+  A.n = npix;
+  for (int a = 0; a < 3; ++a) {
+    A.bmin[a] = s->root_min[a];
+    float e = s->root_max[a] - s->root_min[a];
+    A.bscale[a] = e > 0.0f ? 1.0f / e : 0.0f;
+  }
+  th_guided_stats st;
+  memset(&st, 0, sizeof st);
+  uint64_t diffuse_before = 0;
+  if (w->diffuse_per_pass == 0) w->diffuse_per_pass = (uint64_t)npix * std::max<uint32_t>(p->max_bounces, 2u) / 2u;
+  std::pair<hipEvent_t, hipEvent_t> evp;
+  if ((rc = begin_timed(ctx, &evp))) return rc;
+  hipStream_t sm = ctx->stream;
+  hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 1u);
+  uint32_t cnt_host[8];
+  for (uint32_t smp = 0; smp < p->samples; ++smp) {
+    const bool trained = guide_steps(g) > 0u;
+    A.smp = smp;
+    A.alpha = trained ? opt->alpha : 0.0f;
+    A.record = train ? 1u : 0u;
+    A.defer = (A.alpha > 0.0f || train) ? 1u : 0u;
+    if (train) {
+      double pr = (double)target / (double)std::max<uint64_t>(w->diffuse_per_pass, 1);
+      A.rec_all = pr >= 1.0 ? 1u : 0u;
+      A.rec_thresh = (uint32_t)std::min(pr * 4294967296.0, 4294967295.0);
+    }
+    hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 0u);
+    hipLaunchKernelGGL(gd_generate, dim3((npix + 255) / 256), dim3(256), 0, sm, B, A, *p, *cam);
+    for (uint32_t b = 0; b < p->max_bounces; ++b) {
+      const uint32_t li = b & 1u;
+      A.bounce = b;
+      hipLaunchKernelGGL(gd_extend, dim3(grid_trace), dim3(block), lds, sm, B, S, *p, li);
+      hipLaunchKernelGGL(gd_shade_a, dim3(grid_scalar), dim3(block), 0, sm, B, A, S, *p,
+                         (const float*)ctx->d_srgb_lut, li);
+      if (A.defer) {
+        if (A.alpha > 0.0f &&
+            (rc = guide_infer(g, B.cnt + 3, npix, B.qg, B.gp, B.gn, B.d, s->root_min, s->root_max, B.O, sm))) {
+          (void)end_timed(ctx, evp);
+          return rc;
+        }
+        hipLaunchKernelGGL(gd_shade_b, dim3(grid_scalar), dim3(block), 0, sm, B, A, S, *p,
+                           (const float*)ctx->d_srgb_lut);
+      }
+      hipLaunchKernelGGL(gd_shadow, dim3(grid_shadow), dim3(block), lds, sm, B, S, *p);
+      hipLaunchKernelGGL(gd_finish, dim3(grid_scalar), dim3(block), 0, sm, B, *p, li);
+    }
+    if (train) {
+      hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((rcap + 255) / 256, 2048u)), dim3(256), 0, sm, B);
+      TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
+      TH_CHECK(hipStreamSynchronize(sm));
+      const uint32_t recs = std::min(cnt_host[4], rcap);
+      const uint64_t diffuse_total = cnt_host[5];
+      w->diffuse_per_pass = std::max<uint64_t>(diffuse_total - diffuse_before, 1);
+      diffuse_before = diffuse_total;
+      st.records += recs;
+      uint32_t nsteps = recs >= batch ? std::min(steps, recs / batch) : (recs >= 1024u ? 1u : 0u);
+      for (uint32_t j = 0; j < nsteps; ++j) {
+        const uint32_t off = j * batch, cnt = std::min(batch, recs - off);
+        if ((rc = guide_train(g, B.rX + (size_t)off * 64u, B.rdir + (size_t)off * 3u, B.rw + off, cnt,
+                              &st.last_loss))) {
+          (void)end_timed(ctx, evp);
+          return rc;
+        }
+        st.train_steps++;
+      }
+    }
+  }
+  hipLaunchKernelGGL(gd_resolve, dim3((npix + 255) / 256), dim3(256), 0, sm, B, A, *p, s->d_accum);
+  hipError_t le = hipGetLastError();
+  if ((rc = end_timed(ctx, evp))) return rc;
+  if (le != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("guided launch: ") + hipGetErrorString(le));
+  ctx->counters_pending = false;
+  ctx->stats.paths = npix64 * p->samples;
+  if (train) TH_CHECK(hipStreamSynchronize(sm));  // the last loss has landed in `st`
+  if (stats) {
+    TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
+    TH_CHECK(hipStreamSynchronize(sm));
+    st.diffuse_vertices = cnt_host[5];
+    *stats = st;
+  }
+  return TH_OK;
+}

@@ -217,8 +217,12 @@ struct Integrator {
     return idx;
   }
 
-  __device__ v3 direct_light(const Hit& hit, const v3& albedo, uint32_t* s) {  // wgsl:406-472
-    if (P.light_count == 0u || P.total_light_weight <= 0.0f) return th::mk(0, 0, 0);
+  // direct_light (wgsl:406-472) up to its shadow ray: false where the reference returns
+  // zero before tracing; otherwise the shadow ray (so, dir, t < limit) and the term the
+  // reference returns when that ray is unoccluded.
+  __device__ bool light_candidate(const v3& hp, const v3& hn, const v3& albedo, uint32_t* s, v3* so,
+                                  v3* dir, float* limit, v3* term) {
+    if (P.light_count == 0u || P.total_light_weight <= 0.0f) return false;
     const TH_GpuLight& L = S.lights[pick_light(s)];
     v3 pos, ln;
     if (L.kind == 0u) {
@@ -236,16 +240,16 @@ struct Integrator {
                     th::smul(L.center_radius[3], ln));
     }
     const TH_GpuMaterial& em = S.materials[L.material_index];
-    v3 so = th::add(hit.p, th::muls(hit.n, TH_MIN_DISTANCE));
-    v3 tl = th::sub(pos, so);
+    *so = th::add(hp, th::muls(hn, TH_MIN_DISTANCE));
+    v3 tl = th::sub(pos, *so);
     float d2 = th::dot(tl, tl);
-    if (d2 <= TH_MIN_DISTANCE * TH_MIN_DISTANCE) return th::mk(0, 0, 0);
+    if (d2 <= TH_MIN_DISTANCE * TH_MIN_DISTANCE) return false;
     float dist = th::fsqrt(d2);
-    v3 dir = th::divs(tl, dist);
-    float sc = th::dot(dir, hit.n);
-    float lcos = th::dot(th::neg(dir), ln);
-    if (sc <= 0.0f || lcos <= 0.0f) return th::mk(0, 0, 0);
-    if (occluded(so, dir, dist - 2.0f * TH_MIN_DISTANCE)) return th::mk(0, 0, 0);
+    *dir = th::divs(tl, dist);
+    float sc = th::dot(*dir, hn);
+    float lcos = th::dot(th::neg(*dir), ln);
+    if (sc <= 0.0f || lcos <= 0.0f) return false;
+    *limit = dist - 2.0f * TH_MIN_DISTANCE;
     v3 brdf = th::divs(albedo, TH_PI);
     float area = L.area_weight_cumulative[0];
     float weight = L.area_weight_cumulative[1];
@@ -256,24 +260,43 @@ struct Integrator {
     r = th::muls(r, em.params[2]);
     r = th::muls(r, sc);
     r = th::muls(r, lcos);
-    return th::divs(r, d2 * pdf_area);
+    *term = th::divs(r, d2 * pdf_area);
+    return true;
   }
 
-  __device__ v3 direct_environment(const Hit& hit, const v3& albedo, uint32_t* s) {  // wgsl:486-505
-    if (P.background_kind != 2u) return th::mk(0, 0, 0);
-    EnvSample es = env_sample(s);
-    float sc = th::fmax_(0.0f, th::dot(hit.n, es.dir));
-    if (sc <= 0.0f || es.pdf <= 0.0f) return th::mk(0, 0, 0);
-    // blocked when the closest hit is within MAX_DISTANCE (t <= 10000 <=> t < next float)
-    if (occluded(th::add(hit.p, th::muls(hit.n, TH_MIN_DISTANCE)), es.dir, 10000.001f))
-      return th::mk(0, 0, 0);
-    float bsdf_pdf = sc / TH_PI;
-    float mis = power_heuristic(es.pdf, bsdf_pdf);
+  __device__ v3 direct_light(const Hit& hit, const v3& albedo, uint32_t* s) {  // wgsl:406-472
+    v3 so, dir, term;
+    float limit;
+    if (!light_candidate(hit.p, hit.n, albedo, s, &so, &dir, &limit, &term)) return th::mk(0, 0, 0);
+    if (occluded(so, dir, limit)) return th::mk(0, 0, 0);
+    return term;
+  }
+
+  // direct_environment (wgsl:486-505) up to its shadow ray: the environment sample and
+  // its cosine; false where the reference returns zero before tracing.
+  __device__ bool env_candidate(const v3& hn, uint32_t* s, EnvSample* es, float* sc) {
+    *es = env_sample(s);
+    *sc = th::fmax_(0.0f, th::dot(hn, es->dir));
+    return !(*sc <= 0.0f || es->pdf <= 0.0f);
+  }
+  // The unoccluded term of direct_environment for a given MIS weight.
+  __device__ static v3 env_term(const v3& albedo, const EnvSample& es, float sc, float mis) {
     v3 brdf = th::divs(albedo, TH_PI);
     v3 r = th::mul(brdf, es.rad);
     r = th::muls(r, sc);
     r = th::muls(r, mis);
     return th::divs(r, es.pdf);
+  }
+
+  __device__ v3 direct_environment(const Hit& hit, const v3& albedo, uint32_t* s) {  // wgsl:486-505
+    if (P.background_kind != 2u) return th::mk(0, 0, 0);
+    EnvSample es;
+    float sc;
+    if (!env_candidate(hit.n, s, &es, &sc)) return th::mk(0, 0, 0);
+    // blocked when the closest hit is within MAX_DISTANCE (t <= 10000 <=> t < next float)
+    if (occluded(th::add(hit.p, th::muls(hit.n, TH_MIN_DISTANCE)), es.dir, 10000.001f))
+      return th::mk(0, 0, 0);
+    return env_term(albedo, es, sc, power_heuristic(es.pdf, sc / TH_PI));
   }
 
   // ---------------------------------------------------------------- textures

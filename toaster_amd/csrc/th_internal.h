@@ -20,6 +20,7 @@ using namespace thd;
   } while (0)
 
 struct th_wavefront;    // wavefront workspace (th_wavefront.hip)
+struct th_guided_ws;    // guided wavefront workspace (th_guided.hip)
 struct th_batch_state;  // batch tables and captured graph (th_batch.hip)
 
 struct th_context {
@@ -58,6 +59,7 @@ struct th_scene {
   float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
   uint32_t max_depth = 0;
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
+  th_guided_ws* gw = nullptr;  // allocated by the first guided render
   th_tuning tuning = {0, 0, 0.0f, 0};
 };
 
@@ -73,3 +75,13 @@ int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderPara
 void th_wavefront_free(th_wavefront* wf);
 void th_batch_free(th_batch_state* b);
 int32_t th_check_render_params(const th_scene* s, const TH_GpuRenderParams* p);
+
+// Guided wavefront (th_guided.hip) and the network kernels it drives (th_guide.hip).
+void th_guided_free(th_guided_ws* w);
+int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const uint32_t* queue,
+                    const float4* gp, const float4* gn, const float4* gd, const float* bmin,
+                    const float* bmax, float* O, hipStream_t st);
+int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count,
+                    float* loss_out);
+uint32_t guide_steps(const th_guide* g);
+th_context* guide_context(const th_guide* g);

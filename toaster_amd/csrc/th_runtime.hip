@@ -769,6 +769,7 @@ void th_scene_destroy(th_scene* s) {
   (void)hipStreamSynchronize(s->ctx->stream);
   free_scene_buffers(s);
   th_wavefront_free(s->wf);
+  th_guided_free(s->gw);
   delete s;
 }
 

@@ -187,63 +187,23 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
         if (m.kind == 0u) {
           v3 albedo = it.base_color(m, rec);
           // direct light (wgsl:406-472) with the visibility test deferred
-          if (P.light_count != 0u && P.total_light_weight > 0.0f) {
-            const TH_GpuLight& L = S.lights[it.pick_light(&rng)];
-            v3 pos, ln;
-            if (L.kind == 0u) {
-              float u = th::fsqrt(th::rand01(&rng));
-              float v = th::rand01(&rng);
-              float w0 = 1.0f - u, w1 = u * (1.0f - v), w2 = u * v;
-              v3 a = th::mk(L.v0[0], L.v0[1], L.v0[2]);
-              v3 b = th::mk(L.v1[0], L.v1[1], L.v1[2]);
-              v3 c = th::mk(L.v2[0], L.v2[1], L.v2[2]);
-              pos = th::add(th::add(th::smul(w0, a), th::smul(w1, b)), th::smul(w2, c));
-              ln = th::normalize(th::cross(th::sub(b, a), th::sub(c, a)));
-            } else {
-              ln = it.random_unit_vector(&rng);
-              pos = th::add(th::mk(L.center_radius[0], L.center_radius[1], L.center_radius[2]),
-                            th::smul(L.center_radius[3], ln));
-            }
-            const TH_GpuMaterial& em = S.materials[L.material_index];
-            v3 so = th::add(rec.p, th::muls(rec.n, TH_MIN_DISTANCE));
-            v3 tl = th::sub(pos, so);
-            float d2 = th::dot(tl, tl);
-            if (d2 > TH_MIN_DISTANCE * TH_MIN_DISTANCE) {
-              float dist = th::fsqrt(d2);
-              v3 dir = th::divs(tl, dist);
-              float sc = th::dot(dir, rec.n);
-              float lcos = th::dot(th::neg(dir), ln);
-              if (sc > 0.0f && lcos > 0.0f) {
-                v3 brdf = th::divs(albedo, TH_PI);
-                float sel_pdf = L.area_weight_cumulative[1] / P.total_light_weight;
-                float surf_pdf = 1.0f / L.area_weight_cumulative[0];
-                float pdf_area = sel_pdf * surf_pdf;
-                v3 r = th::mul(brdf, th::mk(em.albedo[0], em.albedo[1], em.albedo[2]));
-                r = th::muls(r, em.params[2]);
-                r = th::muls(r, sc);
-                r = th::muls(r, lcos);
-                r = th::divs(r, d2 * pdf_area);
-                v3 c = th::mul(throughput, r);
-                B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
-                B.sh0[k] = make_float4(dir.x, dir.y, dir.z, dist - 2.0f * TH_MIN_DISTANCE);
-                B.c0[k] = make_float4(c.x, c.y, c.z, 0.0f);
-                sh_light = true;
-              }
+          {
+            v3 so, dir, term;
+            float limit;
+            if (it.light_candidate(rec.p, rec.n, albedo, &rng, &so, &dir, &limit, &term)) {
+              v3 c = th::mul(throughput, term);
+              B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
+              B.sh0[k] = make_float4(dir.x, dir.y, dir.z, limit);
+              B.c0[k] = make_float4(c.x, c.y, c.z, 0.0f);
+              sh_light = true;
             }
           }
           // direct environment (wgsl:486-505) with the visibility test deferred
           if (P.background_kind == 2u) {
-            EnvSample es = it.env_sample(&rng);
-            float sc = th::fmax_(0.0f, th::dot(rec.n, es.dir));
-            if (sc > 0.0f && es.pdf > 0.0f) {
-              float bsdf_pdf = sc / TH_PI;
-              float mis = it.power_heuristic(es.pdf, bsdf_pdf);
-              v3 brdf = th::divs(albedo, TH_PI);
-              v3 r = th::mul(brdf, es.rad);
-              r = th::muls(r, sc);
-              r = th::muls(r, mis);
-              r = th::divs(r, es.pdf);
-              v3 c = th::mul(throughput, r);
+            EnvSample es;
+            float sc;
+            if (it.env_candidate(rec.n, &rng, &es, &sc)) {
+              v3 c = th::mul(throughput, it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, sc / TH_PI)));
               v3 so = th::add(rec.p, th::muls(rec.n, TH_MIN_DISTANCE));
               B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
               B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);

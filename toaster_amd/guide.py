@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi as A
-from .renderer import GpuContext
+from .renderer import GpuContext, GpuScene
 
 
 def _ptr(a):
@@ -80,3 +80,30 @@ def mfma_selftest(ctx: GpuContext, a, b):
     c = np.empty((16, 16), np.float32)
     A.check(A.lib().th_guide_mfma_selftest(ctx.handle, _ptr(a), _ptr(b), _ptr(c)))
     return c
+
+
+def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.5, train=True, steps_per_pass=1,
+                  batch=1 << 18, progressive=True):
+    """`samples` passes of one sample per pixel with the guide (see th_render_guided);
+    with progressive=True the running average and frame seed continue like
+    GpuScene.render_progressive.  Returns the call's statistics as a dict."""
+    p = scene.params(samples=samples)
+    if not progressive:
+        p.accumulated_samples = 0
+    o = A.GuidedOptions(alpha=alpha, train=1 if train else 0, steps_per_pass=steps_per_pass,
+                        batch=batch)
+    st = A.GuidedStats()
+    A.check(A.lib().th_render_guided(scene.ctx.handle, scene._h, guide.handle, C.byref(p),
+                                     C.byref(scene.packed.camera), C.byref(o), C.byref(st)))
+    if progressive:
+        scene.accumulated_samples = p.accumulated_samples + samples
+        scene.frame_index += 1
+    return {"diffuse_vertices": st.diffuse_vertices, "records": st.records,
+            "train_steps": st.train_steps, "last_loss": st.last_loss}
+
+
+def rel_rmse(image, reference):
+    """sqrt(mean over pixels and RGB channels of (I - R)^2 / (R^2 + 0.01))."""
+    i = np.asarray(image, np.float64)[..., :3]
+    r = np.asarray(reference, np.float64)[..., :3]
+    return float(np.sqrt(np.mean((i - r) ** 2 / (r * r + 0.01))))

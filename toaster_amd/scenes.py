@@ -212,6 +212,75 @@ def mixed_environment(width=1280, height=720, samples=64, max_bounces=8, n_trian
                  environment=env, name="mixed_environment")
 
 
+def _hash01(ix, iz, octave, seed):
+    """Lattice value in [-1, 1) from a 64-bit mix of (ix, iz, octave, seed)."""
+    with np.errstate(over="ignore"):
+        h = (ix.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+             ^ iz.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)
+             ^ np.uint64((octave * 0x165667B19E3779F9 + seed) & 0xFFFFFFFFFFFFFFFF))
+        h ^= h >> np.uint64(33)
+        h *= np.uint64(0xFF51AFD7ED558CCD)
+        h ^= h >> np.uint64(33)
+        h *= np.uint64(0xC4CEB9FE1A85EC53)
+        h ^= h >> np.uint64(33)
+    return (h >> np.uint64(40)).astype(np.float64) * (2.0 / (1 << 24)) - 1.0
+
+
+def fbm(x, z, seed, octaves=6, lacunarity=2.0, gain=0.5, frequency=0.75):
+    """Fractional Brownian motion of value noise (smoothstep-interpolated lattice values),
+    normalised by the summed octave amplitudes to [-1, 1]."""
+    x, z = np.asarray(x, np.float64), np.asarray(z, np.float64)
+    total, amp, norm, f = np.zeros(np.broadcast(x, z).shape), 1.0, 0.0, frequency
+    for o in range(octaves):
+        X, Z = x * f, z * f
+        ix, iz = np.floor(X), np.floor(Z)
+        tx, tz = X - ix, Z - iz
+        sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+        ix, iz = ix.astype(np.int64), iz.astype(np.int64)
+        v00, v10 = _hash01(ix, iz, o, seed), _hash01(ix + 1, iz, o, seed)
+        v01, v11 = _hash01(ix, iz + 1, o, seed), _hash01(ix + 1, iz + 1, o, seed)
+        v = (v00 * (1 - sx) + v10 * sx) * (1 - sz) + (v01 * (1 - sx) + v11 * sx) * sz
+        total += amp * v
+        norm += amp
+        amp *= gain
+        f *= lacunarity
+    return total / norm
+
+
+def terrain(width=1280, height=720, samples=256, max_bounces=8, cells=(512, 256), env_size=(1024, 512),
+            amplitude=0.6):
+    """Workload 5 (indirect-dominated light for path guiding): a height field of
+    cells[0] x cells[1] cells (two triangles each; 262,144 at 512 x 256) over x in [-4, 4],
+    z in [-2, 2], height = amplitude * 6-octave value-noise fBm (lacunarity 2, gain 0.5).
+    A roof quad at y = 1.6 covers the field with a 0.5 margin and a wall closes the back
+    (z = -2.5); the other three sides are open.  The synthetic sky + sun HDR has its sun
+    at elevation 60 degrees on the closed side, so the roof blocks direct sun and the
+    field is lit by sky light through the openings and inter-reflection."""
+    nx, nz = cells
+    xs = np.linspace(-4.0, 4.0, nx + 1)
+    zs = np.linspace(-2.0, 2.0, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    Y = amplitude * fbm(X, Z, seed=workload_seed(5))
+    P = np.stack([X, Y, Z], -1).astype(np.float32)  # [nx+1, nz+1, 3]
+    v00, v10, v01, v11 = P[:-1, :-1], P[1:, :-1], P[:-1, 1:], P[1:, 1:]
+    t0 = np.stack([v00, v01, v11], -2).reshape(-1, 3, 3)
+    t1 = np.stack([v00, v11, v10], -2).reshape(-1, 3, 3)
+    field = np.stack([t0, t1], 1).reshape(-1, 3, 3)
+    m, yr, zb = 4.5, 1.6, -2.5
+    roof = _quad((-m, yr, zb), (m, yr, zb), (m, yr, -zb), (-m, yr, -zb))
+    wall = _quad((-m, -1.0, zb), (m, -1.0, zb), (m, yr, zb), (-m, yr, zb))
+    extra = np.asarray(roof + wall, np.float32)
+    tris = np.zeros(len(field) + 4, TRIANGLE_IN_DTYPE)
+    tris["v"] = np.concatenate([field, extra])
+    tris["material_index"] = np.concatenate([np.zeros(len(field), np.int64), [1, 1, 2, 2]])
+    mats = _materials([diffuse((0.55, 0.5, 0.45)), diffuse((0.7, 0.7, 0.7)), diffuse((0.6, 0.6, 0.6))])
+    env = EnvironmentMap(sky_environment(*env_size, sun_elevation=60.0, sun_azimuth=-90.0), 1.0, 0.0)
+    return Scene(camera_position=(0.0, 1.2, 2.2), camera_look_at=(0.0, -0.1, -1.0), fov_degrees=60.0,
+                 width=width, height=height, samples=samples, max_bounces=max_bounces,
+                 background=A.TH_BG_ENVIRONMENT, materials=mats, spheres=np.zeros(0, SPHERE_IN_DTYPE),
+                 triangles=tris, environment=env, name=f"terrain_{2 * nx * nz}")
+
+
 def no_attributes(n):
     return np.zeros(n, ATTR_IN_DTYPE)
 
