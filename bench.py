@@ -11,6 +11,14 @@ reference traversal reads for this frame (48 B per node visited, 8 B per primiti
 reference, 64/32 B per triangle/sphere test, 48+80 B per closest hit, 96 B per light
 record scanned, 16 B per output pixel), counted by the oracle on the same seeds, divided
 by the kernel's mean duration from HIP events around each launch.
+
+equal_time compares the guided tracer with the plain tracer on the indirect-lit terrain
+(262,144 triangles, 1280x720, 8 bounces): a plain reference at --reference-spp, the
+plain tracer at 64 and 256 spp, and the guided tracer at 256 spp trained online from
+scratch (all training inside the timed region).  From the two plain points,
+relRMSE^2 = C / spp + E (E = the reference's own noise); the plain time to reach the
+guided error is T_plain(256) * (C / (relRMSE_guided^2 - E)) / 256, and time_ratio is
+the guided time over that.
 """
 import argparse
 import json
@@ -74,6 +82,90 @@ def cpu_baseline(sc, budget_s=8.0):
     }, counters
 
 
+def equal_time_guided(reference_spp):
+    """Equal-time relRMSE of guided vs plain tracing on the terrain workload."""
+    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
+    from toaster_amd.guide import GUIDED_DEFAULTS, Guide, rel_rmse, render_guided
+    sc = scenes.terrain(1280, 720, samples=16, max_bounces=8)
+    ctx = GpuContext(0)
+    gs = GpuScene.from_scene(ctx, sc)
+    px = sc.width * sc.height
+
+    def timed(fn):
+        ctx.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        ctx.synchronize()
+        return time.perf_counter() - t, r
+
+    # the plain estimator (per-sample streams) through each plain kernel; the fastest is
+    # the baseline and renders the reference
+    g0 = Guide(ctx, seed=1)
+    plain = {
+        "quad_megakernel": lambda n, f: gs.render(samples=n, frame_index=f, accumulated_samples=0,
+                                                  options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                                        traversal=A.TH_TRAVERSE_WIDE)),
+        "wavefront": lambda n, f: gs.render(samples=n, frame_index=f, accumulated_samples=0,
+                                            options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                                  traversal=A.TH_TRAVERSE_WIDE,
+                                                                  kernel=A.TH_KERNEL_WAVEFRONT)),
+        "wavefront_1spp_passes": lambda n, f: (setattr(gs, "frame_index", f),
+                                               render_guided(gs, g0, n, alpha=0.0, train=False,
+                                                             progressive=False, warmup_steps=0)),
+    }
+    speed = {}
+    for name, fn in plain.items():
+        fn(4, 1)
+        speed[name] = timed(lambda: fn(16, 2))[0] / 16
+    best = min(speed, key=speed.get)
+    fn = plain[best]
+    chunk = 256
+    t_ref = 0.0
+    for c in range(max(1, reference_spp // chunk)):  # progressive chunks into one average
+        if best == "wavefront_1spp_passes":
+            gs.frame_index = 1000 + c
+            gs.accumulated_samples = c * chunk
+            t_ref += timed(lambda: render_guided(gs, g0, chunk, alpha=0.0, train=False, warmup_steps=0))[0]
+        else:
+            t_ref += timed(lambda: gs.render(samples=chunk, frame_index=1000 + c, accumulated_samples=c * chunk,
+                                             options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                                   traversal=A.TH_TRAVERSE_WIDE,
+                                                                   kernel=(A.TH_KERNEL_WAVEFRONT if best == "wavefront"
+                                                                           else A.TH_KERNEL_MEGA))))[0]
+    ref = gs.readback()
+    pts = {}
+    for n in (64, 256):
+        dt, _ = timed(lambda: fn(n, 7 + n))
+        pts[n] = (dt, rel_rmse(gs.readback(), ref))
+    g = Guide(ctx, seed=1)
+    gs.frame_index = 3
+    dtg, st = timed(lambda: render_guided(gs, g, 256, progressive=False, **GUIDED_DEFAULTS))
+    relg = rel_rmse(gs.readback(), ref)
+    (t64, r64), (t256, r256) = pts[64], pts[256]
+    C = (r64 ** 2 - r256 ** 2) / (1 / 64 - 1 / 256)
+    E = max(r256 ** 2 - C / 256, 0.0)
+    n_eq = C / max(relg ** 2 - E, 1e-12)
+    t_eq = t256 * n_eq / 256
+    g.close()
+    g0.close()
+    gs.close()
+    ctx.close()
+    return {
+        "workload": "terrain_262144: 512x256-cell fBm height field under a roof, sky + sun HDR "
+                    "(sun blocked), 1280x720, 8 bounces",
+        "reference_spp": reference_spp, "reference_seconds": round(t_ref, 2),
+        "plain_kernel": best, "plain_ms_per_spp": {k: round(v * 1e3, 3) for k, v in speed.items()},
+        "plain": {"spp": 256, "seconds": round(t256, 4), "rel_rmse": round(r256, 5),
+                  "mpaths_s": round(256 * px / t256 / 1e6, 2), "rel_rmse_64spp": round(r64, 5)},
+        "guided": {"spp": 256, "seconds": round(dtg, 4), "rel_rmse": round(relg, 5),
+                   "mpaths_s": round(256 * px / dtg / 1e6, 2), "train_steps": st["train_steps"],
+                   "records": st["records"], "settings": GUIDED_DEFAULTS},
+        "reference_noise_rel_rmse": round(E ** 0.5, 5),
+        "plain_seconds_to_guided_error": round(t_eq, 4),
+        "time_ratio": round(dtg / t_eq, 4), "target_time_ratio": 0.5,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +175,8 @@ def main():
     ap.add_argument("--traversal", choices=["reference", "ordered", "wide"], default="wide")
     ap.add_argument("--rng", choices=["reference", "per_sample"], default="reference")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-guided", action="store_true")
+    ap.add_argument("--reference-spp", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported",
@@ -145,6 +239,7 @@ def main():
     ms_step = dt * 1e3 / args.steps
     kernel_ms = st["total_kernel_ms"] / max(1, st["launches"])
 
+    equal_time = None if args.no_guided else equal_time_guided(args.reference_spp)
     cpu, counters = (None, None)
     if not args.no_cpu:
         cpu, counters = cpu_baseline(sc)
@@ -163,7 +258,7 @@ def main():
                     "kernel": "k_render_quad" if args.traversal == "wide" else "k_render_pixel",
                     "kernel_ms": round(kernel_ms, 5)}
     line = {
-        "metric": "Mpaths/s (plain HIP tracer, BVH-stress workload)",
+        "metric": "Mpaths/s + equal-time relRMSE: guided vs plain HIP tracer, 1 MI355X",
         "value": round(args.steps * paths / dt / 1e6, 3),
         "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -178,6 +273,7 @@ def main():
         "kernel_ms_mean": round(kernel_ms, 5),
         "roofline": roofline,
         "variants": variants,
+        "equal_time": equal_time,
         "cpu_baseline": cpu,
         "host": platform.node(),
     }

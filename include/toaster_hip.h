@@ -377,15 +377,23 @@ int32_t th_guide_train_batch(th_guide* guide, const float* features, const float
  * density in the environment MIS, so it stays unbiased for any network.  With train != 0
  * every pass records diffuse vertices (features, sampled direction, and the weight
  * f cos L_i / pdf once the path completes) and runs Adam steps on them; alpha is treated
- * as 0 until the guide has taken its first step.  With alpha = 0 and train = 0 the image
+ * as 0 until the guide has taken its first step.  Every schedule depends only on the
+ * guide's step count, so the estimator of each pass is fixed before the pass runs.  With alpha = 0 and train = 0 the image
  * equals th_render's TH_RNG_PER_SAMPLE image with one sample group. */
 typedef struct th_guided_options {
     float alpha;              /* guide probability at diffuse vertices, [0, 0.95]      */
     uint32_t train;           /* 1 = record samples and train after every pass        */
     uint32_t steps_per_pass;  /* Adam steps per pass (0 = 1)                           */
     uint32_t batch;           /* samples per step (0 = 2^18)                           */
-    uint32_t reserved[4];
+    uint32_t flags;           /* TH_GUIDED_*                                           */
+    uint32_t warmup_steps;    /* while the guide has taken fewer Adam steps than this,  */
+    uint32_t warmup_rate;     /*   take warmup_rate steps per pass instead             */
+    uint32_t alpha_ramp;      /* alpha grows linearly over the guide's first alpha_ramp */
+                              /*   steps (0 = full alpha after the first step)         */
 } th_guided_options;
+/* At guided vertices, estimate environment light by the continuation alone (no
+ * environment sample and shadow ray; an escaping continuation counts with weight 1). */
+#define TH_GUIDED_NO_ENV_NEE 1u
 
 typedef struct th_guided_stats {
     uint64_t diffuse_vertices; /* deferred diffuse vertices over the call               */

@@ -40,7 +40,8 @@ def test_alpha_zero_matches_oracle(gpu_ctx, case, train):
     gs = GpuScene.from_scene(gpu_ctx, sc)
     g = Guide(gpu_ctx, seed=2)
     print(f"case guided alpha=0 {case} train={train}", flush=True)
-    st = render_guided(gs, g, sc.samples, alpha=0.0, train=train, batch=1024, progressive=False)
+    st = render_guided(gs, g, sc.samples, alpha=0.0, train=train, batch=1024, progressive=False,
+                       warmup_steps=0)
     img = gs.readback()
     ref, _ = oracle.OracleScene(sc).render(threads=8, rng_mode=A.TH_RNG_PER_SAMPLE, group=1)
     frac = _within(img, ref)
@@ -56,7 +57,8 @@ def _terrain_small():
     return scenes.terrain(64, 36, samples=16, max_bounces=6, cells=(128, 64), env_size=(512, 256))
 
 
-def test_guided_estimator_is_unbiased(gpu_ctx):
+@pytest.mark.parametrize("env_nee", [True, False])
+def test_guided_estimator_is_unbiased(gpu_ctx, env_nee):
     sc = _terrain_small()
     gs = GpuScene.from_scene(gpu_ctx, sc)
     print("case guided unbiased: plain reference", flush=True)
@@ -64,15 +66,16 @@ def test_guided_estimator_is_unbiased(gpu_ctx):
                                                   traversal=A.TH_TRAVERSE_WIDE), frame_index=77)
     ref = gs.readback()[..., :3].mean(-1)
     g = Guide(gpu_ctx, seed=4)
-    render_guided(gs, g, 32, alpha=0.5, train=True, batch=4096, steps_per_pass=2, progressive=False)
+    kw = dict(alpha=0.7, env_nee=env_nee, batch=4096, warmup_steps=16, warmup_rate=4, alpha_ramp=16)
+    render_guided(gs, g, 32, train=True, progressive=False, **kw)
     means = []
     for c in range(16):
         gs.frame_index = 1000 + c
-        render_guided(gs, g, 32, alpha=0.5, train=True, batch=4096, progressive=False)
+        render_guided(gs, g, 32, train=True, progressive=False, **kw)
         means.append(float(gs.readback()[..., :3].mean()))
     m, se = float(np.mean(means)), float(np.std(means, ddof=1) / np.sqrt(len(means)))
     r = float(ref.mean())
-    print(f"  guided mean {m:.5f} +- {se:.5f}, plain 4096 spp {r:.5f}")
+    print(f"  env_nee={env_nee}: guided mean {m:.5f} +- {se:.5f}, plain 4096 spp {r:.5f}")
     assert abs(m - r) <= 4 * se + 2e-3 * r
     gs.close()
     g.close()
@@ -88,10 +91,11 @@ def test_trained_guide_lowers_error_at_equal_samples(gpu_ctx):
     gs.render(samples=64, options=o, frame_index=5)
     plain = rel_rmse(gs.readback(), ref)
     g = Guide(gpu_ctx, seed=6)
+    kw = dict(batch=1 << 14, warmup_steps=64, warmup_rate=4, alpha_ramp=64)
     gs.frame_index = 11
-    render_guided(gs, g, 64, alpha=0.5, train=True, batch=1 << 14, steps_per_pass=4, progressive=False)
+    render_guided(gs, g, 64, train=True, progressive=False, **kw)
     gs.frame_index = 12
-    st = render_guided(gs, g, 64, alpha=0.5, train=True, batch=1 << 14, progressive=False)
+    st = render_guided(gs, g, 64, train=True, progressive=False, **kw)
     guided = rel_rmse(gs.readback(), ref)
     print(f"  relRMSE at 64 spp: plain {plain:.4f} guided {guided:.4f} stats {st}")
     assert guided < plain

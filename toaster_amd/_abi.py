@@ -26,6 +26,7 @@ TH_KERNEL_MEGA, TH_KERNEL_WAVEFRONT = 0, 1
 TH_TRAVERSE_REFERENCE, TH_TRAVERSE_ORDERED, TH_TRAVERSE_WIDE = 0, 1, 2
 
 TH_GUIDE_FEATURES, TH_GUIDE_OUTPUTS, TH_GUIDE_PARAMS = 48, 40, 14056
+TH_GUIDED_NO_ENV_NEE = 1
 
 TH_CHANGE_CAMERA = 1
 TH_CHANGE_SPHERES = 2
@@ -149,7 +150,7 @@ class Stats(C.Structure):
 P = C.c_void_p
 class GuidedOptions(C.Structure):
     _fields_ = [("alpha", f32), ("train", u32), ("steps_per_pass", u32), ("batch", u32),
-                ("reserved", u32 * 4)]
+                ("flags", u32), ("warmup_steps", u32), ("warmup_rate", u32), ("alpha_ramp", u32)]
 
 
 class GuidedStats(C.Structure):

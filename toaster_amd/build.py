@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -36,26 +37,32 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
-    """Compiles every HIP and host source and links libtoaster_hip.so."""
-    objdir = os.path.join(HERE, "_build")
+def build(verbose=False, defines=(), lib_path=LIB, jobs=None):
+    """Compiles every HIP and host source (in parallel) and links libtoaster_hip.so.
+    `defines` adds -D options to every compile (alternative builds go to `lib_path`)."""
+    tag = "".join("_" + d.replace("=", "") for d in defines)
+    objdir = os.path.join(HERE, "_build" + tag)
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    dflags = ["-D" + d for d in defines]
+    cmds, objs = [], []
     for src in HOST_SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append(["g++"] + HOST_FLAGS + dflags + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + dflags + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    workers = jobs or min(len(cmds), max(1, (os.cpu_count() or 2) // 2), 8)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        list(pool.map(_run, cmds))
+    tmp = lib_path + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-lpthread"])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib_path)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":

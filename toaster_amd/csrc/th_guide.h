@@ -53,23 +53,34 @@ TH_HD float fexp(float x) {
 #endif
 }
 
+// Encoding layout: TG_POS_OCT frequency octaves per position axis (sin and cos each)
+// and TG_DIR_BINS one-blob bins per normal / outgoing-direction component.
+#ifndef TG_POS_OCT
+#define TG_POS_OCT 4
+#endif
+#ifndef TG_DIR_BINS
+#define TG_DIR_BINS 4
+#endif
+static_assert(6 * TG_POS_OCT + 6 * TG_DIR_BINS == TG_IN, "encoding width");
+
 // Feature j (0..47) of the encoding of a vertex: scene-bounds-normalised position p01
-// in [0,1]^3 -> sin/cos(2 pi 2^k x) for k = 0..3 per axis (features 0..23); the normal
-// and outgoing direction components c -> u = (c + 1)/2 spread over 4 Gaussian bins at
-// (b + 1/2)/4 with sigma 1/4 (features 24..47).  Each feature is computed on its own,
+// in [0,1]^3 -> sin/cos(2 pi 2^k x) for k < TG_POS_OCT per axis; the normal and outgoing
+// direction components c -> u = (c + 1)/2 spread over TG_DIR_BINS Gaussian bins at
+// (b + 1/2)/TG_DIR_BINS with sigma 1/TG_DIR_BINS.  Each feature is computed on its own,
 // so lanes can share the work of one row.
 TH_HD float encode_feature(int j, v3 p01, v3 n, v3 wo) {
-  if (j < 24) {
-    int a = j >> 3, k = (j & 7) >> 1;
+  if (j < 6 * TG_POS_OCT) {
+    int a = j / (2 * TG_POS_OCT), k = (j % (2 * TG_POS_OCT)) >> 1;
     float x = a == 0 ? p01.x : (a == 1 ? p01.y : p01.z);
     float s, c;
     th::sincos_(6.2831855f * (float)(1 << k) * x, &s, &c);
     return (j & 1) ? c : s;
   }
-  int c = (j - 24) >> 2, b = (j - 24) & 3;
+  int jj = j - 6 * TG_POS_OCT;
+  int c = jj / TG_DIR_BINS, b = jj % TG_DIR_BINS;
   float v = c == 0 ? n.x : c == 1 ? n.y : c == 2 ? n.z : c == 3 ? wo.x : c == 4 ? wo.y : wo.z;
   float u = 0.5f * (v + 1.0f);
-  float d = (u - ((float)b + 0.5f) * 0.25f) * 4.0f;
+  float d = (u - ((float)b + 0.5f) * (1.0f / (float)TG_DIR_BINS)) * (float)TG_DIR_BINS;
   return fexp(-0.5f * d * d);
 }
 

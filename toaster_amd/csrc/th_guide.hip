@@ -149,15 +149,27 @@ __global__ void __launch_bounds__(64) k_gemm_wgrad(const uint16_t* __restrict__ 
 }
 
 // Loss gradient of one batch: L = -sum_b w_b log q_b(dir_b) / sum_b w_b.  The weight sum
-// is one block with a fixed reduction order, so gradients are reproducible call to call.
-__global__ void __launch_bounds__(1024) k_weight_sum(const float* __restrict__ w, uint32_t n,
-                                                     float* __restrict__ sums) {
-  __shared__ float red[1024];
+// is reduced in a fixed order (256 block partials, then one block), so gradients are
+// reproducible call to call.
+__global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict__ w, uint32_t n,
+                                                        float* __restrict__ part) {
+  __shared__ float red[256];
   float s = 0.0f;
-  for (uint32_t i = threadIdx.x; i < n; i += 1024u) s += w[i];
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) s += w[i];
   red[threadIdx.x] = s;
   __syncthreads();
-  for (uint32_t off = 512; off > 0; off >>= 1) {
+  for (uint32_t off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_weight_total(const float* __restrict__ part, float* __restrict__ sums) {
+  __shared__ float red[256];
+  red[threadIdx.x] = part[threadIdx.x];
+  __syncthreads();
+  for (uint32_t off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
@@ -177,8 +189,17 @@ __global__ void k_loss_grad(const float* __restrict__ O, const float* __restrict
   const bool live = b < rows && b < count && sums[0] > 0.0f;
   float dzv[TG_OUT];
   float lterm = 0.0f;
+  float o[TG_OUT];
   if (live) {
-    const float* o = O + (size_t)b * TG_OUTP;
+    const float4* o4 = (const float4*)(O + (size_t)b * TG_OUTP);
+#pragma unroll
+    for (int j = 0; j < TG_OUT / 4; ++j) {
+      float4 v = o4[j];
+      o[4 * j] = v.x;
+      o[4 * j + 1] = v.y;
+      o[4 * j + 2] = v.z;
+      o[4 * j + 3] = v.w;
+    }
     tg::Mixture m;
     tg::decode(o, &m);
     const v3 om = th::mk(dirs[3 * b], dirs[3 * b + 1], dirs[3 * b + 2]);
@@ -210,18 +231,43 @@ __global__ void k_loss_grad(const float* __restrict__ O, const float* __restrict
   for (int off = 32; off > 0; off >>= 1) lterm += __shfl_xor(lterm, off, 64);
   if ((threadIdx.x & 63u) == 0u && lterm != 0.0f) atomicAdd(loss, lterm);
   if (b >= rows) return;
-  uint16_t* dz = dZ + (size_t)b * TG_OUTP;
-  for (int j = 0; j < TG_OUTP; ++j) dz[j] = (live && j < TG_OUT) ? tg::f2bf(dzv[j]) : (uint16_t)0;
+  uint4* dz = (uint4*)(dZ + (size_t)b * TG_OUTP);
+#pragma unroll
+  for (int j = 0; j < TG_OUTP / 8; ++j) {
+    uint32_t h[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c0 = 8 * j + 2 * t, c1 = c0 + 1;
+      const uint32_t lo = (live && c0 < TG_OUT) ? tg::f2bf(dzv[c0 < TG_OUT ? c0 : 0]) : 0u;
+      const uint32_t hi = (live && c1 < TG_OUT) ? tg::f2bf(dzv[c1 < TG_OUT ? c1 : 0]) : 0u;
+      h[t] = lo | (hi << 16);
+    }
+    dz[j] = make_uint4(h[0], h[1], h[2], h[3]);
+  }
 }
 
 __global__ void k_clear(float* p, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0f;
 }
 
-// Adam on the FP32 master parameters (sum of the per-block partials as gradient, or
-// only the gradient when lr == 0), then both padded BF16 copies and biases.
+// First stage of the fixed-order sum of the per-block weight-gradient partials: group j
+// of 16 sums blocks j, j + 16, j + 32, ... of every layer element.
+#define TG_GROUPS 16
+__global__ void __launch_bounds__(256) k_reduce_partials(const float* __restrict__ partials, uint32_t nblocks,
+                                                         float* __restrict__ groups) {
+  const uint32_t e = blockIdx.x * 256u + threadIdx.x, j = blockIdx.y;
+  if (e >= 4u * TG_PART) return;
+  const uint32_t l = e / TG_PART, off = e % TG_PART;
+  const float* base = partials + (size_t)l * 256u * TG_PART + off;
+  float s = 0.0f;
+  for (uint32_t b = j; b < nblocks; b += TG_GROUPS) s += base[(size_t)b * TG_PART];
+  groups[(size_t)j * 4u * TG_PART + e] = s;
+}
+
+// Adam on the FP32 master parameters (gradient = the sum of the TG_GROUPS group sums,
+// or only the gradient when lr == 0), then both padded BF16 copies and biases.
 __global__ void k_adam(float* __restrict__ master, float* __restrict__ m1, float* __restrict__ m2,
-                       const float* __restrict__ partials, uint32_t nblocks, float lr, float b1,
+                       const float* __restrict__ groups, uint32_t ngroups, float lr, float b1,
                        float b2, float eps, float bc1, float bc2, float* __restrict__ grad_out,
                        uint16_t* __restrict__ woi, uint16_t* __restrict__ wio, float* __restrict__ bias) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -237,11 +283,9 @@ __global__ void k_adam(float* __restrict__ master, float* __restrict__ m1, float
     n = ((int)i - D.woff) / D.kin;
     k = ((int)i - D.woff) % D.kin;
   }
-  // partials: layer l at l * 256 * TG_PART, block stride TG_PART
-  const size_t lbase = (size_t)l * 256u * TG_PART;
-  const size_t off = is_bias ? (size_t)64 * 64 + n : (size_t)n * 64 + k;
+  const size_t e = (size_t)l * TG_PART + (is_bias ? (size_t)64 * 64 + n : (size_t)n * 64 + k);
   float gsum = 0.0f;
-  for (uint32_t b = 0; b < nblocks; ++b) gsum += partials[lbase + (size_t)b * TG_PART + off];
+  for (uint32_t j = 0; j < ngroups; ++j) gsum += groups[(size_t)j * 4u * TG_PART + e];
   if (grad_out) grad_out[i] = gsum;
   float p = master[i];
   if (lr > 0.0f) {
@@ -379,11 +423,12 @@ struct th_guide {
   uint16_t *X0 = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr;
   uint16_t *dZ4 = nullptr, *dZ3 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
   float *O = nullptr, *feat = nullptr, *dirs = nullptr, *w = nullptr, *partials = nullptr, *scal = nullptr;
+  float *groups = nullptr, *wpart = nullptr;
   uint32_t nblocks = 0;
   std::vector<void*> ws;
 };
 
-static const uint32_t kWgradBlocks = 256;  // partial slots per layer (k_adam assumes 256)
+static const uint32_t kWgradBlocks = 256;  // partial slots per layer (k_reduce_partials assumes 256)
 
 static int32_t guide_reserve(th_guide* g, uint32_t rows) {
   if (g->cap >= rows) return TH_OK;
@@ -407,7 +452,8 @@ static int32_t guide_reserve(th_guide* g, uint32_t rows) {
       (rc = al((void**)&g->feat, (size_t)cap * TG_IN * 4)) || (rc = al((void**)&g->dirs, (size_t)cap * 12)) ||
       (rc = al((void**)&g->w, (size_t)cap * 4)) ||
       (rc = al((void**)&g->partials, (size_t)kWgradBlocks * 4 * TG_PART * 4)) ||
-      (rc = al((void**)&g->scal, 64)))
+      (rc = al((void**)&g->groups, (size_t)TG_GROUPS * 4 * TG_PART * 4)) ||
+      (rc = al((void**)&g->wpart, 256 * 4)) || (rc = al((void**)&g->scal, 64)))
     return rc;
   g->cap = cap;
   return TH_OK;
@@ -440,7 +486,8 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
   int32_t rc = th_guide_forward_rows(g, X, nullptr, rows, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, g->scal, 16u);
-  hipLaunchKernelGGL(k_weight_sum, dim3(1), dim3(1024), 0, st, w, count, g->scal);
+  hipLaunchKernelGGL(k_weight_partial, dim3(256), dim3(256), 0, st, w, count, g->wpart);
+  hipLaunchKernelGGL(k_weight_total, dim3(1), dim3(256), 0, st, g->wpart, g->scal);
   hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, g->O, dirs, w, count,
                      rows, g->scal, g->dZ4, g->scal + 1);
   uint32_t rows_per = round_up((rows + kWgradBlocks - 1) / kWgradBlocks, 32);
@@ -467,9 +514,11 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
     bc2 = 1.0f - powf(g->beta2, (float)g->step);
   }
   // partials of layer l: kWgradBlocks slots of TG_PART floats; the first nb are summed
+  hipLaunchKernelGGL(k_reduce_partials, dim3((4 * TG_PART + 255) / 256, TG_GROUPS), dim3(256), 0, st, P, nb,
+                     g->groups);
   hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 255) / 256), dim3(256), 0, st, g->d_master,
-                     g->d_m1, g->d_m2, P, nb, lr, g->beta1, g->beta2, g->eps, bc1, bc2, g->d_grad,
-                     g->d_woi, g->d_wio, g->d_bias);
+                     g->d_m1, g->d_m2, g->groups, std::min<uint32_t>(nb, TG_GROUPS), lr, g->beta1, g->beta2,
+                     g->eps, bc1, bc2, g->d_grad, g->d_woi, g->d_wio, g->d_bias);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -519,7 +568,7 @@ static void host_init_params(uint32_t seed, std::vector<float>& p) {
 
 static int32_t guide_refresh(th_guide* g) {  // bf16 copies from the master weights
   hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 255) / 256), dim3(256), 0, g->ctx->stream, g->d_master,
-                     g->d_m1, g->d_m2, g->partials, 0u, 0.0f, g->beta1, g->beta2, g->eps, 1.0f, 1.0f,
+                     g->d_m1, g->d_m2, g->groups, 0u, 0.0f, g->beta1, g->beta2, g->eps, 1.0f, 1.0f,
                      (float*)nullptr, g->d_woi, g->d_wio, g->d_bias);
   TH_CHECK(hipGetLastError());
   return TH_OK;

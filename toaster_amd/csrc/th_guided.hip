@@ -50,14 +50,17 @@ struct GdBufs {
   float4 *sho, *sh0, *c0, *sh1, *c1;
   uint32_t* vis;
   uint32_t *q0, *q1, *qs, *qg;
-  uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [4] records, [5] diffuse total
+  uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [4] records offered,
+                  // [5] diffuse total, [8 + c] records of chunk c
   float* O;       // guide outputs, TG_OUT per guide-queue row
   uint16_t* rX;   // training records: BF16 features, 64 wide (zero beyond 48)
   float *rdir, *rw;
   float4 *rbeta, *rL;
   uint32_t* rpath;
-  uint32_t rcap;
+  uint32_t rcap;  // chunks x chunk capacity
 };
+
+#define GD_MAX_CHUNKS 16u
 
 struct GdArgs {
   uint32_t n;           // paths per pass (pixels)
@@ -68,6 +71,9 @@ struct GdArgs {
   uint32_t record;      // record training samples
   uint32_t rec_thresh;  // record an eligible vertex when its hash < rec_thresh
   uint32_t rec_all;     // record every eligible vertex
+  uint32_t env_nee;     // environment sample + shadow ray at diffuse vertices
+  uint32_t nchunks;     // records are spread uniformly at random over this many chunks
+  uint32_t chunk_cap;   // capacity of one chunk (chunk c holds rows c * chunk_cap ...)
   float bmin[3], bscale[3];
 };
 
@@ -88,7 +94,8 @@ __device__ __forceinline__ uint32_t gd_push(bool live, uint32_t value, uint32_t*
 }
 
 __global__ void gd_reset(GdBufs B, uint32_t all) {
-  if (threadIdx.x < 5u || (all && threadIdx.x < 8u)) B.cnt[threadIdx.x] = 0u;
+  const uint32_t t = threadIdx.x;
+  if (t < 5u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[t] = 0u;
 }
 
 // ---------------------------------------------------------------- generate
@@ -181,8 +188,8 @@ __global__ void __launch_bounds__(64) gd_shadow(GdBufs B, DevScene S, TH_GpuRend
 __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, false>& it,
                                               const TH_GpuRenderParams& P, uint32_t k, const v3& p,
                                               const v3& n, const v3& albedo, const tg::Mixture* m,
-                                              float alpha, v3& throughput, uint32_t& rng, v3& nd,
-                                              float& pdf, bool& sh_light, bool& sh_env) {
+                                              float alpha, bool env_nee, v3& throughput, uint32_t& rng,
+                                              v3& nd, float& pdf, bool& sh_light, bool& sh_env) {
   const bool guided = m != nullptr && alpha > 0.0f;
   {
     v3 so, dir, term;
@@ -195,7 +202,7 @@ __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, fal
       sh_light = true;
     }
   }
-  if (P.background_kind == 2u) {
+  if (P.background_kind == 2u && env_nee) {
     EnvSample es;
     float sc;
     if (it.env_candidate(n, &rng, &es, &sc)) {
@@ -273,8 +280,8 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
           radiance = th::add(radiance, th::mul(throughput, th::mix(th::mk(1.0f, 1.0f, 1.0f),
                                                                     th::mk(0.35f, 0.65f, 1.0f), lt)));
         } else if (P.background_kind == 2u) {
-          float mis = 1.0f;
-          if (flags & GD_PREV_DIFFUSE) mis = it.power_heuristic(prev_pdf, it.env_pdf(d));
+          float mis = 1.0f;  // escaping after a diffuse vertex: MIS with its environment sample
+          if ((flags & GD_PREV_DIFFUSE) && A.env_nee) mis = it.power_heuristic(prev_pdf, it.env_pdf(d));
           radiance = th::add(radiance, th::muls(th::mul(throughput, it.env_lookup(d)), mis));
         }
         done = true;
@@ -297,8 +304,8 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
             park = true;
           } else {
             float pdf;
-            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, throughput, rng, nd, pdf,
-                          sh_light, sh_env);
+            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, true, throughput, rng, nd,
+                          pdf, sh_light, sh_env);
             flags = GD_PREV_DIFFUSE;
             o_next = make_float4(rec.p.x, rec.p.y, rec.p.z, pdf);
             if (A.bounce + 1u >= P.max_bounces) done = true;
@@ -387,8 +394,8 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       tg::Mixture mix;
       if (use_guide) tg::decode(B.O + (size_t)i * TG_OUT, &mix);
       float pdf;
-      bool cont = shade_diffuse(B, it, P, k, p, nrm, albedo, use_guide ? &mix : nullptr, A.alpha, throughput,
-                                rng, nd, pdf, sh_light, sh_env);
+      bool cont = shade_diffuse(B, it, P, k, p, nrm, albedo, use_guide ? &mix : nullptr, A.alpha,
+                                A.env_nee != 0u, throughput, rng, nd, pdf, sh_light, sh_env);
       bool done = !cont || A.bounce + 1u >= P.max_bounces;
       if (A.record && !done) {
         uint32_t h = th::pcg_hash((k * 0x9E3779B9u) ^ (A.bounce * 0x85EBCA6Bu) ^
@@ -404,8 +411,28 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
     }
     gd_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
     gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
-    uint32_t r = gd_push(rec_ok, k, B.rpath, &B.cnt[4], B.rcap);
-    if (rec_ok && r < B.rcap) {
+    gd_push(rec_ok, 0u, nullptr, &B.cnt[4]);  // records offered
+    // chunk drawn at random so every training step sees an unbiased subset; slots are
+    // handed out with one atomic per (wave, chunk)
+    uint32_t r = 0xffffffffu;
+    const uint32_t myc =
+        rec_ok ? th::pcg_hash(k ^ (A.bounce << 27) ^ (A.smp * 0x2545F491u)) % A.nchunks : 0xffffffffu;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t pending = __ballot(rec_ok); pending != 0;) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const uint32_t cc = __shfl(myc, leader, 64);
+      const uint64_t m = __ballot(myc == cc);
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(&B.cnt[8u + cc], (uint32_t)__popcll(m));
+      base = __shfl(base, leader, 64);
+      if (myc == cc) {
+        const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (idx < A.chunk_cap) r = cc * A.chunk_cap + idx;
+      }
+      pending &= ~m;
+    }
+    if (r != 0xffffffffu) {
+      B.rpath[r] = k;
       v3 p01 = th::mk((p.x - A.bmin[0]) * A.bscale[0], (p.y - A.bmin[1]) * A.bscale[1],
                       (p.z - A.bmin[2]) * A.bscale[2]);
       uint16_t* x = B.rX + (size_t)r * 64u;
@@ -468,9 +495,11 @@ __global__ void __launch_bounds__(64) gd_finish(GdBufs B, TH_GpuRenderParams P, 
 
 // ---------------------------------------------------------------- records, resolve
 // Weight of each record of the pass: lum((L - L_v) / beta_v), zero where not finite.
-__global__ void __launch_bounds__(256) gd_weights(GdBufs B) {
-  const uint32_t n = min(B.cnt[4], B.rcap);
+__global__ void __launch_bounds__(256) gd_weights(GdBufs B, uint32_t nchunks, uint32_t chunk_cap) {
+  const uint32_t n = nchunks * chunk_cap;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint32_t c = r / chunk_cap;
+    if (r - c * chunk_cap >= min(B.cnt[8u + c], chunk_cap)) continue;
     float4 L = B.rad[B.rpath[r]], Lv = B.rL[r], b = B.rbeta[r];
     float v0 = b.x > 0.0f ? (L.x - Lv.x) / b.x : 0.0f;
     float v1 = b.y > 0.0f ? (L.y - Lv.y) / b.y : 0.0f;
@@ -543,7 +572,7 @@ static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
       (rc = gd_alloc(w, (void**)&b.sh1, f4)) || (rc = gd_alloc(w, (void**)&b.c1, f4)) ||
       (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, u)) ||
       (rc = gd_alloc(w, (void**)&b.q1, u)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * u)) ||
-      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.cnt, 16 * sizeof(uint32_t))) ||
+      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.cnt, 32 * sizeof(uint32_t))) ||
       (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.rX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
       (rc = gd_alloc(w, (void**)&b.rdir, (size_t)rcap * 3 * sizeof(float))) ||
@@ -571,14 +600,19 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     return th_set_error(TH_E_INVALID, "alpha must lie in [0, 0.95]");
   const uint32_t batch = opt->batch ? opt->batch : (1u << 18);
   const uint32_t steps = opt->steps_per_pass ? opt->steps_per_pass : 1u;
-  if (batch % 32u != 0u || batch > (1u << 22) || steps > 64u)
+  const uint32_t wrate = opt->warmup_steps ? std::max<uint32_t>(opt->warmup_rate, 1u) : steps;
+  if (batch % 32u != 0u || batch > (1u << 22) || steps > 64u || wrate > 64u)
     return th_set_error(TH_E_INVALID, "batch must be a multiple of 32 up to 2^22, steps at most 64");
   const uint64_t npix64 = (uint64_t)p->width * p->height;
   if (npix64 > (1u << 26)) return th_set_error(TH_E_UNSUPPORTED, "image too large for the guided wavefront");
   const uint32_t npix = (uint32_t)npix64;
   const bool train = opt->train != 0u;
-  const uint64_t target = (uint64_t)batch * steps;
-  const uint32_t rcap = train ? (uint32_t)((2u * target + 31u) / 32u * 32u) : 32u;
+  const uint32_t max_chunks = std::max(steps, wrate);
+  if (max_chunks > GD_MAX_CHUNKS) return th_set_error(TH_E_INVALID, "at most 16 steps per pass");
+  // one chunk per step: capacity 1.5 x batch absorbs the spread of the random split
+  const uint32_t chunk_cap = (batch + batch / 2u + 31u) / 32u * 32u;
+  if (train && (uint64_t)chunk_cap * max_chunks > (1u << 26)) return th_set_error(TH_E_INVALID, "batch x steps too large");
+  const uint32_t rcap = train ? chunk_cap * max_chunks : 32u;
   TH_CHECK(hipSetDevice(ctx->device));
   if ((rc = gd_reserve(s, npix, rcap))) return rc;
   th_guided_ws* w = s->gw;
@@ -606,13 +640,21 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if ((rc = begin_timed(ctx, &evp))) return rc;
   hipStream_t sm = ctx->stream;
   hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 1u);
-  uint32_t cnt_host[8];
+  uint32_t cnt_host[32];
   for (uint32_t smp = 0; smp < p->samples; ++smp) {
-    const bool trained = guide_steps(g) > 0u;
+    const uint32_t taken = guide_steps(g);
+    const uint32_t pass_steps = taken < opt->warmup_steps ? wrate : steps;
+    const uint64_t target = (uint64_t)batch * pass_steps;
     A.smp = smp;
-    A.alpha = trained ? opt->alpha : 0.0f;
+    A.nchunks = pass_steps;
+    A.chunk_cap = chunk_cap;
+    A.alpha = taken == 0u ? 0.0f
+              : (opt->alpha_ramp && taken < opt->alpha_ramp ? opt->alpha * (float)taken / (float)opt->alpha_ramp
+                                                             : opt->alpha);
     A.record = train ? 1u : 0u;
     A.defer = (A.alpha > 0.0f || train) ? 1u : 0u;
+    // the reference's environment sampling stays on until the guide is in use
+    A.env_nee = (A.alpha > 0.0f && (opt->flags & TH_GUIDED_NO_ENV_NEE)) ? 0u : 1u;
     if (train) {
       double pr = (double)target / (double)std::max<uint64_t>(w->diffuse_per_pass, 1);
       A.rec_all = pr >= 1.0 ? 1u : 0u;
@@ -639,19 +681,19 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
       hipLaunchKernelGGL(gd_finish, dim3(grid_scalar), dim3(block), 0, sm, B, *p, li);
     }
     if (train) {
-      hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((rcap + 255) / 256, 2048u)), dim3(256), 0, sm, B);
+      hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((pass_steps * chunk_cap + 255) / 256, 4096u)),
+                         dim3(256), 0, sm, B, pass_steps, chunk_cap);
       TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
       TH_CHECK(hipStreamSynchronize(sm));
-      const uint32_t recs = std::min(cnt_host[4], rcap);
       const uint64_t diffuse_total = cnt_host[5];
       w->diffuse_per_pass = std::max<uint64_t>(diffuse_total - diffuse_before, 1);
       diffuse_before = diffuse_total;
-      st.records += recs;
-      uint32_t nsteps = recs >= batch ? std::min(steps, recs / batch) : (recs >= 1024u ? 1u : 0u);
-      for (uint32_t j = 0; j < nsteps; ++j) {
-        const uint32_t off = j * batch, cnt = std::min(batch, recs - off);
-        if ((rc = guide_train(g, B.rX + (size_t)off * 64u, B.rdir + (size_t)off * 3u, B.rw + off, cnt,
-                              &st.last_loss))) {
+      for (uint32_t j = 0; j < pass_steps; ++j) {
+        const uint32_t cnt = std::min(cnt_host[8 + j], chunk_cap);
+        st.records += cnt;
+        if (cnt < 1024u) continue;  // too few samples for a step
+        const size_t off = (size_t)j * chunk_cap;
+        if ((rc = guide_train(g, B.rX + off * 64u, B.rdir + off * 3u, B.rw + off, cnt, &st.last_loss))) {
           (void)end_timed(ctx, evp);
           return rc;
         }

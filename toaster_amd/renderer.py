@@ -6,7 +6,9 @@ per-frame dispatches that fold `samples` new samples into the progressive
 running average (wgsl:538-547, path_tracer.rs:655-674), linear-HDR readback
 (readback.rs:8-25) and the exposure/ACES/sRGB display stage (image_output.rs:59-92).
 """
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -35,11 +37,28 @@ def device_count():
     return n.value
 
 
+# Live handles, released in dependency order (scenes and guides before their contexts)
+# while the interpreter is still whole at exit.
+_dependents = weakref.WeakSet()
+_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for group in (_dependents, _contexts):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 class GpuContext:
     def __init__(self, device=0):
         self._lib = A.lib()
         self._h = C.c_void_p()
         A.check(self._lib.th_context_create(int(device), C.byref(self._h)))
+        _contexts.add(self)
 
     @property
     def handle(self):
@@ -84,6 +103,7 @@ class GpuScene:
         self._h = C.c_void_p()
         d = packed.desc()
         A.check(self._lib.th_scene_create(ctx.handle, C.byref(d), int(node_capacity), C.byref(self._h)))
+        _dependents.add(self)
         self.width, self.height = packed.params.width, packed.params.height
         self.accumulated_samples = 0
         self.frame_index = 0
