@@ -57,10 +57,9 @@ def cpu_baseline(sc, budget_s=8.0):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
-    # single-thread figure on a row band (reference CPU renderer is single-threaded)
-    rows = max(1, sc.height // 8)
+    # single-thread figure on one full frame (the reference CPU renderer is single-threaded)
     t1 = time.perf_counter()
-    oc.render(threads=1, rows=(0, rows))
+    oc.render(threads=1)
     dt1 = time.perf_counter() - t1
     oc.close()
     model = ""
@@ -77,7 +76,7 @@ def cpu_baseline(sc, budget_s=8.0):
         "value": round(frames * paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads,
         "kind": "port",
         "sample": f"{frames} full frames of the same workload (oracle, pcg mode, OpenMP over pixels)",
-        "single_thread_mpaths_s": round(rows * sc.width * sc.samples / dt1 / 1e6, 4),
+        "single_thread_mpaths_s": round(sc.paths_per_frame / dt1 / 1e6, 4),
         "cpu_model": model, "nproc": os.cpu_count(),
     }, counters
 
