@@ -81,6 +81,48 @@ def cpu_baseline(sc, budget_s=8.0):
     }, counters
 
 
+def reference_suite(warmup=2, runs=5):
+    """The reference's benchmark scenes (scenes/benchmarks/001-007) as its benchmark mode
+    runs them (frame seed 0, 2 warm-up + 5 measured frames, median): dispatch+wait =
+    render + readback of the linear-HDR output, the span the published
+    `dispatch_wait_ms` covers (dispatch.rs:44-57), next to the RTX 2080 Ti medians."""
+    import numpy as np
+    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A
+    from toaster_amd.loader import PUBLISHED_DISPATCH_WAIT_MS, load_scene, suite_paths
+    ctx = GpuContext(0)
+    out = {}
+    for path in suite_paths():
+        sc = load_scene(path)
+        gs = GpuScene.from_scene(ctx, sc)
+        row = {}
+        for mode, opts in (("reference_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE)),
+                           ("per_sample_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
+                                                            rng_mode=A.TH_RNG_PER_SAMPLE))):
+            buf = np.empty((sc.height, sc.width, 4), np.float32)
+            times, kms = [], []
+            for i in range(warmup + runs):
+                ctx.synchronize()
+                ctx.reset_stats()
+                t = time.perf_counter()
+                gs.render(options=opts, frame_index=0, accumulated_samples=0)
+                buf[...] = gs.readback()
+                dt = time.perf_counter() - t
+                if i >= warmup:
+                    times.append(dt * 1e3)
+                    st = ctx.stats()
+                    kms.append(st["total_kernel_ms"])
+            med = float(np.median(times))
+            row[mode] = {"dispatch_wait_ms": round(med, 4), "kernel_ms": round(float(np.median(kms)), 4),
+                         "mpaths_s": round(sc.paths_per_frame / med / 1e3, 2)}
+        pub = PUBLISHED_DISPATCH_WAIT_MS[sc.name]
+        row["rtx2080ti_dispatch_wait_ms"] = pub
+        row["speedup_vs_rtx2080ti"] = round(pub / row["reference_rng"]["dispatch_wait_ms"], 3)
+        out[sc.name] = row
+        gs.close()
+    ctx.close()
+    return out
+
+
 def equal_time_guided(reference_spp):
     """Equal-time relRMSE of guided vs plain tracing on the terrain workload."""
     from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
@@ -175,6 +217,7 @@ def main():
     ap.add_argument("--rng", choices=["reference", "per_sample"], default="reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true")
+    ap.add_argument("--no-suite", action="store_true")
     ap.add_argument("--reference-spp", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -211,6 +254,8 @@ def main():
     for name, o in (("megakernel_reference_traversal", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_REFERENCE)),
                     ("megakernel_ordered", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_ORDERED)),
                     ("quad_wide", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE)),
+                    ("quad_wide_dynamic_schedule", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE,
+                                                                 schedule=A.TH_SCHEDULE_DYNAMIC)),
                     ("quad_wide_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
                                                                traversal=A.TH_TRAVERSE_WIDE)),
                     ("megakernel_ordered_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
@@ -239,6 +284,7 @@ def main():
     kernel_ms = st["total_kernel_ms"] / max(1, st["launches"])
 
     equal_time = None if args.no_guided else equal_time_guided(args.reference_spp)
+    suite = None if args.no_suite else reference_suite()
     cpu, counters = (None, None)
     if not args.no_cpu:
         cpu, counters = cpu_baseline(sc)
@@ -273,6 +319,7 @@ def main():
         "roofline": roofline,
         "variants": variants,
         "equal_time": equal_time,
+        "reference_suite": suite,
         "cpu_baseline": cpu,
         "host": platform.node(),
     }

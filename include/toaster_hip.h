@@ -255,8 +255,15 @@ typedef struct th_render_options {
     uint32_t traversal;      /* TH_TRAVERSE_*                                    */
     uint32_t block_size;     /* 0 = autotuned / default                          */
     uint32_t collect_stats;  /* 1 = count rays / nodes / tests (slower launch)   */
-    uint32_t reserved[3];
+    uint32_t schedule;       /* TH_SCHEDULE_*                                    */
+    uint32_t reserved[2];
 } th_render_options;
+
+/* Pixel-to-wave assignment of the 4-lane kernel with the reference RNG.
+ * TH_SCHEDULE_STATIC: one 4x4 pixel tile per wave.
+ * TH_SCHEDULE_DYNAMIC: persistent quads take the next pixel from a counter when done. */
+#define TH_SCHEDULE_STATIC 0
+#define TH_SCHEDULE_DYNAMIC 1
 
 typedef struct th_stats {
     double last_kernel_ms;    /* device time of the last th_render (hipEvents)    */

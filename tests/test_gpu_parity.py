@@ -52,6 +52,22 @@ def test_image_matches_oracle(gpu_ctx, name, make, traversal):
     assert frac >= TOL_FRAC
 
 
+@pytest.mark.parametrize("name,make", [
+    ("cornell", lambda: scenes.cornell()),
+    ("stress2048", lambda: scenes.bvh_stress(2048)),
+])
+def test_dynamic_schedule_matches_oracle(gpu_ctx, name, make):
+    """The 4-lane kernel with persistent quads taking pixels from a counter (the default
+    is one static tile per wave): same per-pixel work, same image."""
+    sc = make()
+    print(f"case {name} wide dynamic schedule", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_DYNAMIC))
+    ref, _ = oracle.OracleScene(sc).render(threads=8)
+    frac, exact = _compare(img, ref)
+    print(f"{name} dynamic: within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert frac >= TOL_FRAC
+
+
 @pytest.mark.parametrize("traversal", [A.TH_TRAVERSE_ORDERED, A.TH_TRAVERSE_WIDE])
 @pytest.mark.parametrize("samples", [1, 8, 16])
 def test_per_sample_rng_matches_oracle(gpu_ctx, traversal, samples):

@@ -24,6 +24,7 @@ TH_BG_SKY, TH_BG_BLACK, TH_BG_ENVIRONMENT = 0, 1, 2
 TH_RNG_REFERENCE, TH_RNG_PER_SAMPLE = 0, 1
 TH_KERNEL_MEGA, TH_KERNEL_WAVEFRONT = 0, 1
 TH_TRAVERSE_REFERENCE, TH_TRAVERSE_ORDERED, TH_TRAVERSE_WIDE = 0, 1, 2
+TH_SCHEDULE_STATIC, TH_SCHEDULE_DYNAMIC = 0, 1
 
 TH_GUIDE_FEATURES, TH_GUIDE_OUTPUTS, TH_GUIDE_PARAMS = 48, 40, 14056
 TH_GUIDED_NO_ENV_NEE = 1
@@ -135,7 +136,7 @@ class BatchItem(C.Structure):
 
 class RenderOptions(C.Structure):
     _fields_ = [("rng_mode", u32), ("kernel", u32), ("traversal", u32), ("block_size", u32),
-                ("collect_stats", u32), ("reserved", u32 * 3)]
+                ("collect_stats", u32), ("schedule", u32), ("reserved", u32 * 2)]
 
 
 class Stats(C.Structure):

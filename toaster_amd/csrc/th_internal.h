@@ -33,6 +33,9 @@ struct th_context {
   th_stats stats;
   float* d_srgb_lut = nullptr;
   th_batch_state* batch = nullptr;
+  uint32_t* d_work = nullptr;  // work counters of persistent launches
+  int cu_count = 0;            // compute units of the device
+  int dyn_blocks_per_cu = 0;   // resident blocks of k_render_quad_dyn (0 = not yet queried)
 };
 
 struct th_scene {
@@ -60,6 +63,7 @@ struct th_scene {
   uint32_t max_depth = 0;
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
   th_guided_ws* gw = nullptr;  // allocated by the first guided render
+  float* h_stage = nullptr;    // pinned readback staging (allocated by the first readback)
   th_tuning tuning = {0, 0, 0.0f, 0};
 };
 

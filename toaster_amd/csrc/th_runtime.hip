@@ -272,6 +272,72 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   }
 }
 
+// Reference RNG with dynamic pixel assignment.  A pixel's samples share one random
+// stream, so its samples run in sequence and a statically assigned wave lasts as long as
+// its slowest pixel.  Here every quad is persistent: when its pixel is done it takes the
+// next one (4x4-tile order, one atomic per wave for the quads that fetch together), so a
+// wave's time is its share of the frame.  Same per-pixel work and order as k_render_quad.
+__global__ void __launch_bounds__(256) k_render_quad_dyn(DevScene S, TH_GpuRenderParams P,
+                                                         TH_GpuCamera cam, float4* __restrict__ accum,
+                                                         const float* __restrict__ srgb_lut,
+                                                         uint32_t cached, uint32_t* __restrict__ work) {
+  extern __shared__ uint2 lds_raw[];
+  Child4* lds_nodes = (Child4*)lds_raw;
+  for (uint32_t i = threadIdx.x; i < 4u * cached; i += blockDim.x)
+    lds_nodes[i] = ((const Child4*)S.nodes4)[i];
+  __syncthreads();
+  uint2* lds_stack = lds_raw + 4u * cached * (sizeof(Child4) / sizeof(uint2));
+  const uint32_t lane = threadIdx.x & 63u, lq = lane & 3u;
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  LaneCounters<false> lc;
+  Integrator<3, false> it{S, P, srgb_lut, st, lc, lds_nodes, cached};
+  const uint32_t tiles_x = (P.width + 3u) / 4u;
+  const uint32_t n_tiled = tiles_x * ((P.height + 3u) / 4u) * 16u;
+  for (;;) {
+    // quads fetching now (all four lanes active): one atomic for the wave
+    const uint64_t m = __ballot(lq == 0u);
+    const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(work, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)first, 64);
+    uint32_t p = base + (uint32_t)__popcll(m & ((1ull << (lane & ~3u)) - 1ull));
+    p = __shfl(p, (int)(lane & ~3u), 64);
+    if (p >= n_tiled) break;
+    const uint32_t tile = p >> 4, w = p & 15u;
+    const uint32_t x = (tile % tiles_x) * 4u + (w & 3u), y = (tile / tiles_x) * 4u + (w >> 2);
+    if (x >= P.width || y >= P.height) continue;
+    const uint32_t idx = y * P.width + x;
+    uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
+    for (uint32_t smp = 0; smp < P.samples; ++smp) {
+      v3 o, d;
+      camera_ray(cam, P, x, y, &s, &o, &d);
+      v3 L = it.ray_color(o, d, &s);
+      cr = cr + L.x;
+      cg = cg + L.y;
+      cb = cb + L.z;
+      ca = ca + 1.0f;
+    }
+    if (lq == 0u) {
+      float ns = (float)P.samples;
+      float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
+      if (P.accumulated_samples == 0u) {
+        accum[idx] = b;
+      } else {
+        float prev = (float)P.accumulated_samples;
+        float tot = prev + ns;
+        float4 o = accum[idx];
+        accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                                 (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+      }
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 __global__ void __launch_bounds__(64) k_primary_quad(DevScene S, TH_GpuRenderParams P,
                                                      TH_GpuCamera cam, uint32_t* __restrict__ slot_out,
                                                      float* __restrict__ t_out) {
@@ -632,6 +698,8 @@ int32_t th_context_create(int32_t device, th_context** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipMalloc((void**)&c->d_counters, 16 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_work, 16 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipMalloc((void**)&c->d_srgb_lut, 512 * sizeof(float));
   if (e == hipSuccess) {
     float lut[512];
@@ -652,6 +720,7 @@ void th_context_destroy(th_context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_counters) (void)hipFree(c->d_counters);
+  if (c->d_work) (void)hipFree(c->d_work);
   if (c->d_srgb_lut) (void)hipFree(c->d_srgb_lut);
   th_batch_free(c->batch);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -770,6 +839,7 @@ void th_scene_destroy(th_scene* s) {
   free_scene_buffers(s);
   th_wavefront_free(s->wf);
   th_guided_free(s->gw);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
   delete s;
 }
 
@@ -782,7 +852,7 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   th_render_options o;
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
-  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u)
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 1u)
     return th_set_error(TH_E_INVALID, "unknown render option");
   if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
     return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
@@ -816,7 +886,19 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
     dim3 grid((waves + 3) / 4), blk(qblock);
 #define TH_LAUNCHQ(PS, GG) \
   hipLaunchKernelGGL((k_render_quad<PS, GG>), grid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cached)
-    if (o.rng_mode == TH_RNG_REFERENCE) {
+    if (o.rng_mode == TH_RNG_REFERENCE && o.schedule == TH_SCHEDULE_DYNAMIC) {
+      if (ctx->dyn_blocks_per_cu == 0) {
+        int nb = 0;
+        TH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_quad_dyn, (int)qblock, lds4));
+        ctx->dyn_blocks_per_cu = std::max(nb, 1);
+      }
+      // persistent: as many blocks as stay resident, but no more quads than pixels
+      uint32_t pblocks = (uint32_t)(std::max(ctx->cu_count, 1) * ctx->dyn_blocks_per_cu);
+      pblocks = std::min<uint32_t>(pblocks, (p->width * p->height + 63u) / 64u);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_work, 1u);
+      hipLaunchKernelGGL(k_render_quad_dyn, dim3(std::max(pblocks, 1u)), blk, lds4, ctx->stream, S, *p, *cam,
+                         s->d_accum, ctx->d_srgb_lut, cached, ctx->d_work);
+    } else if (o.rng_mode == TH_RNG_REFERENCE) {
       TH_LAUNCHQ(false, 1);
     } else {
       switch (G) {
@@ -881,8 +963,11 @@ int32_t th_readback(th_context* ctx, th_scene* s, float* rgba, size_t bytes) {
   size_t need = (size_t)s->width * s->height * 4 * sizeof(float);
   if (bytes != need) return th_set_error(TH_E_INVALID, "readback buffer size mismatch");
   TH_CHECK(hipSetDevice(ctx->device));
-  TH_CHECK(hipMemcpyAsync(rgba, s->d_accum, need, hipMemcpyDeviceToHost, ctx->stream));
+  // DMA into a pinned staging buffer of the scene (allocated once), then one host copy
+  if (!s->h_stage) TH_CHECK(hipHostMalloc((void**)&s->h_stage, need, hipHostMallocDefault));
+  TH_CHECK(hipMemcpyAsync(s->h_stage, s->d_accum, need, hipMemcpyDeviceToHost, ctx->stream));
   TH_CHECK(hipStreamSynchronize(ctx->stream));
+  memcpy(rgba, s->h_stage, need);
   return TH_OK;
 }
 

@@ -23,11 +23,13 @@ class RenderOptions:
     kernel: int = A.TH_KERNEL_MEGA
     traversal: int = A.TH_TRAVERSE_REFERENCE
     collect_stats: bool = False
+    schedule: int = A.TH_SCHEDULE_STATIC
 
     def struct(self):
         o = A.RenderOptions()
         o.rng_mode, o.kernel, o.traversal = self.rng_mode, self.kernel, self.traversal
         o.collect_stats = 1 if self.collect_stats else 0
+        o.schedule = self.schedule
         return o
 
 
