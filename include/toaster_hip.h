@@ -286,6 +286,7 @@ typedef struct th_stats {
 
 int32_t th_device_count(int32_t* count);
 int32_t th_context_create(int32_t device, th_context** out);
+/* Scenes and guides of a context must be destroyed before the context. */
 void th_context_destroy(th_context* ctx);
 /* Message of the last failure on this thread (never NULL). */
 const char* th_last_error(void);
@@ -397,6 +398,9 @@ typedef struct th_guided_options {
     uint32_t warmup_rate;     /*   take warmup_rate steps per pass instead             */
     uint32_t alpha_ramp;      /* alpha grows linearly over the guide's first alpha_ramp */
                               /*   steps (0 = full alpha after the first step)         */
+    uint32_t train_every;     /* after the warm-up, record and train on every n-th pass */
+                              /*   of the call only (0 or 1 = every pass)              */
+    uint32_t reserved[3];
 } th_guided_options;
 /* At guided vertices, estimate environment light by the continuation alone (no
  * environment sample and shadow ray; an escaping continuation counts with weight 1). */

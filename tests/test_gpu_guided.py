@@ -91,7 +91,7 @@ def test_trained_guide_lowers_error_at_equal_samples(gpu_ctx):
     gs.render(samples=64, options=o, frame_index=5)
     plain = rel_rmse(gs.readback(), ref)
     g = Guide(gpu_ctx, seed=6)
-    kw = dict(batch=1 << 14, warmup_steps=64, warmup_rate=4, alpha_ramp=64)
+    kw = dict(batch=1 << 14, warmup_steps=64, warmup_rate=4, alpha_ramp=64, train_every=1)
     gs.frame_index = 11
     render_guided(gs, g, 64, train=True, progressive=False, **kw)
     gs.frame_index = 12

@@ -151,7 +151,8 @@ class Stats(C.Structure):
 P = C.c_void_p
 class GuidedOptions(C.Structure):
     _fields_ = [("alpha", f32), ("train", u32), ("steps_per_pass", u32), ("batch", u32),
-                ("flags", u32), ("warmup_steps", u32), ("warmup_rate", u32), ("alpha_ramp", u32)]
+                ("flags", u32), ("warmup_steps", u32), ("warmup_rate", u32), ("alpha_ramp", u32),
+                ("train_every", u32), ("reserved", u32 * 3)]
 
 
 class GuidedStats(C.Structure):

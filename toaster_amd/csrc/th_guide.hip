@@ -7,10 +7,12 @@
 // against a host product): lane l = 16 g + r holds A[r][8g..8g+7], B[8g..8g+7][r] and
 // C[4g..4g+3][r].  A wave owns 16 rows of a batch; weights are kept in two padded
 // BF16 layouts, out-major ([N][K], forward) and in-major ([K][N], backward), so every
-// B fragment is one 16-byte load.  Weight gradients dW[N x K] = dZ^T X reduce over the
-// batch inside the MFMA K dimension (32 rows per instruction) with the 32-row chunk
-// staged through LDS; each block writes one partial, and the Adam kernel sums the
-// partials, updates the FP32 master weights and refreshes both BF16 copies.
+// B fragment is one 16-byte load.  Inference (k_guide_infer) and a training step
+// (k_train_fused) each run in one kernel with the activations in LDS; weight gradients
+// dW[N x K] = dZ^T X reduce over the batch inside the MFMA K dimension (32 samples per
+// instruction); each block writes one partial, a fixed-order reduction sums the
+// partials, and the Adam kernel updates the FP32 master weights and refreshes both BF16
+// copies.
 #include <algorithm>
 #include <vector>
 
@@ -46,16 +48,14 @@ __device__ __forceinline__ tg_f32x4 mfma(tg_bf16x8 a, tg_bf16x8 b, tg_f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Y = epilogue(X * M + bias) with M given as MT[N][K]; one wave per 16 rows.
-// MODE 0: ReLU, BF16 out.  MODE 1: linear, FP32 out.  MODE 2: gated by gate > 0
-// (ReLU derivative of the layer input), BF16 out, no bias.
+// Y = epilogue(X * M + bias) with M given as MT[N][K]; one wave per 16 rows (the
+// host-facing forward, th_guide_forward).  MODE 0: ReLU, BF16 out.  MODE 1: linear, FP32 out.
 template <int K, int N, int MODE>
 __global__ void __launch_bounds__(256) k_gemm_rows(const uint16_t* __restrict__ X,
                                                    const uint16_t* __restrict__ MT,
                                                    const float* __restrict__ bias,
                                                    const uint32_t* __restrict__ count_ptr,
-                                                   uint32_t count_static, void* __restrict__ Y,
-                                                   const uint16_t* __restrict__ gate) {
+                                                   uint32_t count_static, void* __restrict__ Y) {
   const uint32_t rows = count_ptr ? *count_ptr : count_static;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t row0 = wave * 16u;
@@ -71,81 +71,17 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const uint16_t* __restrict__ 
     for (int kb = 0; kb < K / 32; ++kb)
       acc = mfma(a[kb], ld8(MT + (size_t)(nb * 16 + r) * K + kb * 32 + g * 8), acc);
     const uint32_t col = nb * 16 + r;
-    const float bv = (MODE != 2 && bias) ? bias[col] : 0.0f;
+    const float bv = bias ? bias[col] : 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t row = row0 + g * 4 + i;
       float v = acc[i] + bv;
-      if (MODE == 0) {
+      if (MODE == 0)
         ((uint16_t*)Y)[(size_t)row * N + col] = tg::f2bf(v > 0.0f ? v : 0.0f);
-      } else if (MODE == 1) {
+      else
         ((float*)Y)[(size_t)row * N + col] = v;
-      } else {
-        uint16_t gb = gate[(size_t)row * N + col];
-        ((uint16_t*)Y)[(size_t)row * N + col] = tg::f2bf((gb & 0x7fffu) ? v : 0.0f);
-      }
     }
   }
-}
-
-// Partial weight/bias gradient of one block: dW[n][k] = sum_b dZ[b][n] X[b][k],
-// db[n] = sum_b dZ[b][n], over rows [blk * rows_per, (blk+1) * rows_per).
-template <int K, int N>
-__global__ void __launch_bounds__(64) k_gemm_wgrad(const uint16_t* __restrict__ X,
-                                                   const uint16_t* __restrict__ dZ,
-                                                   uint32_t rows_per, uint32_t rows_total,
-                                                   float* __restrict__ partial) {
-  __shared__ __align__(16) uint16_t xs[32 * K];
-  __shared__ __align__(16) uint16_t zs[32 * N];
-  const uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
-  tg_f32x4 acc[N / 16][K / 16];
-#pragma unroll
-  for (int a = 0; a < N / 16; ++a)
-#pragma unroll
-    for (int b = 0; b < K / 16; ++b) acc[a][b] = tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  float db = 0.0f;
-  const uint32_t r0 = blockIdx.x * rows_per;
-  const uint32_t r1 = min(r0 + rows_per, rows_total);
-  for (uint32_t c0 = r0; c0 < r1; c0 += 32u) {
-    // stage 32 rows of X and dZ (16-byte pieces), rows past the end read as zero
-    for (uint32_t i = l; i < 32u * K / 8u; i += 64u) {
-      uint32_t row = c0 + (i * 8u) / K;
-      uint4 v = row < r1 ? *(const uint4*)(X + (size_t)c0 * K + i * 8u) : make_uint4(0, 0, 0, 0);
-      *(uint4*)(xs + i * 8u) = v;
-    }
-    for (uint32_t i = l; i < 32u * N / 8u; i += 64u) {
-      uint32_t row = c0 + (i * 8u) / N;
-      uint4 v = row < r1 ? *(const uint4*)(dZ + (size_t)c0 * N + i * 8u) : make_uint4(0, 0, 0, 0);
-      *(uint4*)(zs + i * 8u) = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < N / 16; ++nb) {
-      uint16_t av[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) av[j] = zs[(g * 8 + j) * N + nb * 16 + r];
-      tg_bf16x8 A = __builtin_bit_cast(tg_bf16x8, *(uint4*)av);
-#pragma unroll
-      for (int kb = 0; kb < K / 16; ++kb) {
-        uint16_t bv[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) bv[j] = xs[(g * 8 + j) * K + kb * 16 + r];
-        tg_bf16x8 B = __builtin_bit_cast(tg_bf16x8, *(uint4*)bv);
-        acc[nb][kb] = mfma(A, B, acc[nb][kb]);
-      }
-    }
-    if (l < (uint32_t)N)
-      for (int j = 0; j < 32; ++j) db += tg::bf2f(zs[j * N + l]);
-    __syncthreads();
-  }
-  float* out = partial + (size_t)blockIdx.x * (N * K + N);
-#pragma unroll
-  for (int nb = 0; nb < N / 16; ++nb)
-#pragma unroll
-    for (int kb = 0; kb < K / 16; ++kb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) out[(nb * 16 + g * 4 + i) * K + kb * 16 + r] = acc[nb][kb][i];
-  if (l < (uint32_t)N) out[N * K + l] = db;
 }
 
 // Loss gradient of one batch: L = -sum_b w_b log q_b(dir_b) / sum_b w_b.  The weight sum
@@ -176,78 +112,190 @@ __global__ void __launch_bounds__(256) k_weight_total(const float* __restrict__ 
   if (threadIdx.x == 0) sums[0] = red[0];
 }
 
-// dL/dO per sample (BF16, 64 wide, zero beyond the 40 outputs) and the loss itself.
-// With g_k = w_k f_k / q: d log q / d r_k = g_k kappa_k (w - mu_k (mu_k.w)) / |r_k|,
-// d log q / d kappa_k = g_k (1/kappa_k - 2 e^{-2 kappa_k} / (1 - e^{-2 kappa_k}) + mu_k.w - 1)
-// (times the softplus slope sigmoid(o), zero where the clamp is active) and
-// d log q / d logit_k = g_k - w_k.
-__global__ void k_loss_grad(const float* __restrict__ O, const float* __restrict__ dirs,
-                            const float* __restrict__ w, uint32_t count, uint32_t rows,
-                            const float* __restrict__ sums, uint16_t* __restrict__ dZ,
-                            float* __restrict__ loss) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = b < rows && b < count && sums[0] > 0.0f;
-  float dzv[TG_OUT];
-  float lterm = 0.0f;
-  float o[TG_OUT];
-  if (live) {
-    const float4* o4 = (const float4*)(O + (size_t)b * TG_OUTP);
-#pragma unroll
-    for (int j = 0; j < TG_OUT / 4; ++j) {
-      float4 v = o4[j];
-      o[4 * j] = v.x;
-      o[4 * j + 1] = v.y;
-      o[4 * j + 2] = v.z;
-      o[4 * j + 3] = v.w;
-    }
-    tg::Mixture m;
-    tg::decode(o, &m);
-    const v3 om = th::mk(dirs[3 * b], dirs[3 * b + 1], dirs[3 * b + 2]);
-    float f[TG_LOBES], q = 0.0f;
-    for (int k = 0; k < TG_LOBES; ++k) {
-      f[k] = tg::vmf_pdf(m.mu[k], m.kappa[k], om);
-      q += m.w[k] * f[k];
-    }
-    q = th::fmax_(q, 1e-30f);
-    const float scale = -w[b] / sums[0];
-    lterm = scale * tg::flog(q);
-    for (int k = 0; k < TG_LOBES; ++k) {
-      const float gk = m.w[k] * f[k] / q;
-      const v3 r = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
-      const float len = th::fsqrt(th::dot(r, r) + 1e-8f);
-      const float c = th::dot(m.mu[k], om);
-      const float kap = m.kappa[k];
-      const v3 dr = th::muls(th::sub(om, th::muls(m.mu[k], c)), gk * kap / len);
-      const float e2 = tg::fexp(-2.0f * kap);
-      const float dk = gk * (1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c - 1.0f));
-      const float sig = kap >= 1000.0f ? 0.0f : 1.0f / (1.0f + tg::fexp(-o[24 + k]));
-      dzv[3 * k] = scale * dr.x;
-      dzv[3 * k + 1] = scale * dr.y;
-      dzv[3 * k + 2] = scale * dr.z;
-      dzv[24 + k] = scale * dk * sig;
-      dzv[32 + k] = scale * (gk - m.w[k]);
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) lterm += __shfl_xor(lterm, off, 64);
-  if ((threadIdx.x & 63u) == 0u && lterm != 0.0f) atomicAdd(loss, lterm);
-  if (b >= rows) return;
-  uint4* dz = (uint4*)(dZ + (size_t)b * TG_OUTP);
-#pragma unroll
-  for (int j = 0; j < TG_OUTP / 8; ++j) {
-    uint32_t h[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int c0 = 8 * j + 2 * t, c1 = c0 + 1;
-      const uint32_t lo = (live && c0 < TG_OUT) ? tg::f2bf(dzv[c0 < TG_OUT ? c0 : 0]) : 0u;
-      const uint32_t hi = (live && c1 < TG_OUT) ? tg::f2bf(dzv[c1 < TG_OUT ? c1 : 0]) : 0u;
-      h[t] = lo | (hi << 16);
-    }
-    dz[j] = make_uint4(h[0], h[1], h[2], h[3]);
-  }
-}
-
 __global__ void k_clear(float* p, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0f;
+}
+
+// Fused training step.  One block (4 waves) walks `rows_per` consecutive samples in
+// chunks of 64 (16 per wave) with every activation in LDS: forward through the four
+// layers (A fragments from LDS tiles, B fragments = out-major weights from L2), the
+// vMF-mixture loss gradient with 4 lanes per sample (2 lobes each, quad reductions), then
+// layer by layer the chunk's weight gradient (wave w owns output rows 16w..16w+15 of
+// every layer's dW, accumulated across chunks in registers; the MFMA K dimension runs
+// over 32 samples) and the gated back-propagation with the in-major weights.  Each block
+// writes one partial gradient per layer for the fixed-order reduction and Adam.
+#define TF_LD 72  // LDS row stride (BF16 elements) of a 64-wide activation tile
+__device__ __forceinline__ tg_bf16x8 lds8(const uint16_t* p) {
+  return __builtin_bit_cast(tg_bf16x8, *(const uint4*)p);
+}
+__device__ __forceinline__ float quad_sum(float v) {
+  return (qbcast_f<0>(v) + qbcast_f<1>(v)) + (qbcast_f<2>(v) + qbcast_f<3>(v));
+}
+__device__ __forceinline__ float quad_max(float v) {
+  return th::fmax_(th::fmax_(qbcast_f<0>(v), qbcast_f<1>(v)), th::fmax_(qbcast_f<2>(v), qbcast_f<3>(v)));
+}
+
+__global__ void __launch_bounds__(256) k_train_fused(const uint16_t* __restrict__ X,
+                                                     const float* __restrict__ dirs,
+                                                     const float* __restrict__ w, uint32_t count,
+                                                     uint32_t rows_per, const float* __restrict__ sums,
+                                                     const uint16_t* __restrict__ woi,
+                                                     const uint16_t* __restrict__ wio,
+                                                     const float* __restrict__ bias,
+                                                     float* __restrict__ partials, float* __restrict__ loss) {
+  __shared__ __align__(16) uint16_t tX[64 * TF_LD], tH1[64 * TF_LD], tH2[64 * TF_LD], tH3[64 * TF_LD];
+  __shared__ __align__(16) uint16_t tDa[64 * TF_LD], tDb[64 * TF_LD];
+  __shared__ __align__(16) float tO[64 * TG_OUT];
+  const uint32_t tid = threadIdx.x, wv = tid >> 6, l = tid & 63u, r = l & 15u, g = l >> 4;
+  const uint32_t r0 = blockIdx.x * rows_per, r1 = min(r0 + rows_per, count);
+  const float S = sums[0];
+  const uint16_t* hin[4] = {tX, tH1, tH2, tH3};
+  tg_f32x4 acc[4][4];
+#pragma unroll
+  for (int L = 0; L < 4; ++L)
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc[L][kb] = tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float dbias[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // column 16 wv + (l & 15), rows (l >> 4) * 16 ...
+  float lsum = 0.0f;
+  for (uint32_t c0 = r0; c0 < r1; c0 += 64u) {  // uniform per block
+    for (uint32_t i = tid; i < 64u * 8u; i += 256u) {
+      const uint32_t row = i >> 3, seg = i & 7u, gr = c0 + row;
+      uint4 v = gr < r1 ? ((const uint4*)(X + (size_t)gr * 64u))[seg] : make_uint4(0, 0, 0, 0);
+      *(uint4*)(tX + row * TF_LD + seg * 8u) = v;
+    }
+    __syncthreads();
+    // ---- forward: wave wv owns rows 16 wv .. 16 wv + 15 of the chunk
+    const uint32_t arow = (16u * wv + r) * TF_LD;
+#pragma unroll
+    for (int L = 0; L < 4; ++L) {
+      const uint16_t* in = hin[L];
+      const tg_bf16x8 a0 = lds8(in + arow + g * 8u), a1 = lds8(in + arow + 32u + g * 8u);
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        if (L == 3 && nb == 3) continue;  // the last layer has 40 outputs
+        const uint16_t* wr = woi + L * 4096 + (nb * 16 + r) * 64;
+        tg_f32x4 c = mfma(a0, ld8(wr + g * 8u), tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        c = mfma(a1, ld8(wr + 32u + g * 8u), c);
+        const uint32_t col = nb * 16 + r;
+        const float bv = bias[L * 64 + col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t row = 16u * wv + 4u * g + i;
+          const float v = c[i] + bv;
+          if (L < 3)
+            ((uint16_t*)hin[L + 1])[row * TF_LD + col] = tg::f2bf(v > 0.0f ? v : 0.0f);
+          else if (col < (uint32_t)TG_OUT)
+            tO[row * TG_OUT + col] = v;
+        }
+      }
+      __syncthreads();
+    }
+    // ---- loss gradient: 4 lanes per sample, lobes 2p and 2p + 1
+    {
+      const uint32_t rr = 16u * wv + (l >> 2), p = l & 3u, gr = c0 + rr;
+      const bool live = gr < r1 && S > 0.0f;
+      const float* o = tO + rr * TG_OUT;
+      const int k0 = 2 * (int)p;
+      const float lg0 = o[32 + k0], lg1 = o[33 + k0];
+      const float mx = quad_max(th::fmax_(lg0, lg1));
+      const float e0 = tg::fexp(lg0 - mx), e1 = tg::fexp(lg1 - mx);
+      const float esum = quad_sum(e0 + e1);
+      const float wk[2] = {e0 / esum, e1 / esum};
+      const v3 om = live ? th::mk(dirs[3u * gr], dirs[3u * gr + 1u], dirs[3u * gr + 2u]) : th::mk(0.0f, 0.0f, 1.0f);
+      v3 mu[2];
+      float len[2], kap[2], f[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k = k0 + j;
+        const v3 rv = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
+        len[j] = th::fsqrt(th::dot(rv, rv) + 1e-8f);
+        mu[j] = th::divs(rv, len[j]);
+        kap[j] = th::fmin_(tg::softplus(o[24 + k]) + 0.01f, 1000.0f);
+        f[j] = tg::vmf_pdf(mu[j], kap[j], om);
+      }
+      const float q = th::fmax_(quad_sum(wk[0] * f[0] + wk[1] * f[1]), 1e-30f);
+      const float scale = live ? -w[gr] / S : 0.0f;
+      if (p == 0u && live) lsum += scale * tg::flog(q);
+      uint16_t* dz = tDa + rr * TF_LD;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k = k0 + j;
+        const float gk = wk[j] * f[j] / q;
+        const float c = th::dot(mu[j], om);
+        const v3 dr = th::muls(th::sub(om, th::muls(mu[j], c)), gk * kap[j] / len[j]);
+        const float e2 = tg::fexp(-2.0f * kap[j]);
+        const float dk = gk * (1.0f / kap[j] - 2.0f * e2 / (1.0f - e2) + (c - 1.0f));
+        const float sig = kap[j] >= 1000.0f ? 0.0f : 1.0f / (1.0f + tg::fexp(-o[24 + k]));
+        dz[3 * k] = tg::f2bf(scale * dr.x);
+        dz[3 * k + 1] = tg::f2bf(scale * dr.y);
+        dz[3 * k + 2] = tg::f2bf(scale * dr.z);
+        dz[24 + k] = tg::f2bf(scale * dk * sig);
+        dz[32 + k] = tg::f2bf(scale * (gk - wk[j]));
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) dz[TG_OUT + 6 * p + j] = 0;
+    }
+    __syncthreads();
+    // ---- backward, layer by layer
+    uint16_t* dcur = tDa;
+    uint16_t* dnext = tDb;
+#pragma unroll
+    for (int L = 3; L >= 0; --L) {
+      // weight gradient: dW[n][k] += sum_rows dZ[row][n] hin[L][row][k], n = 16 wv + r
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        uint16_t av[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) av[j] = dcur[(ks * 32 + g * 8 + j) * TF_LD + 16u * wv + r];
+        const tg_bf16x8 A = __builtin_bit_cast(tg_bf16x8, *(uint4*)av);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+          uint16_t bv[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) bv[j] = hin[L][(ks * 32 + g * 8 + j) * TF_LD + kb * 16 + r];
+          acc[L][kb] = mfma(A, __builtin_bit_cast(tg_bf16x8, *(uint4*)bv), acc[L][kb]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) dbias[L] += tg::bf2f(dcur[((l >> 4) * 16 + j) * TF_LD + 16u * wv + (l & 15u)]);
+      if (L > 0) {  // dZ of the layer below: (dZ W) gated by the ReLU of this layer's input
+        const tg_bf16x8 a0 = lds8(dcur + arow + g * 8u), a1 = lds8(dcur + arow + 32u + g * 8u);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+          const uint16_t* wr = wio + L * 4096 + (nb * 16 + r) * 64;
+          tg_f32x4 c = mfma(a0, ld8(wr + g * 8u), tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+          c = mfma(a1, ld8(wr + 32u + g * 8u), c);
+          const uint32_t col = nb * 16 + r;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint32_t row = 16u * wv + 4u * g + i;
+            const uint16_t gate = hin[L][row * TF_LD + col];
+            dnext[row * TF_LD + col] = tg::f2bf((gate & 0x7fffu) ? c[i] : 0.0f);
+          }
+        }
+        __syncthreads();  // the whole chunk's dZ is written and the old one fully read
+        uint16_t* t = dcur;
+        dcur = dnext;
+        dnext = t;
+      }
+    }
+    __syncthreads();  // tiles are reloaded by the next chunk
+  }
+  // ---- per-block partials
+#pragma unroll
+  for (int L = 0; L < 4; ++L) {
+    float* out = partials + (size_t)L * 256u * TG_PART + (size_t)blockIdx.x * TG_PART;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[(16u * wv + 4u * g + i) * 64u + kb * 16 + r] = acc[L][kb][i];
+    float b = dbias[L];
+    b += __shfl_xor(b, 16, 64);
+    b += __shfl_xor(b, 32, 64);
+    if (l < 16u) out[64 * 64 + 16u * wv + l] = b;
+  }
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
+  if (l == 0u && lsum != 0.0f) atomicAdd(loss, lsum);
 }
 
 // First stage of the fixed-order sum of the per-block weight-gradient partials: group j
@@ -421,7 +469,6 @@ struct th_guide {
   // batch workspace
   uint32_t cap = 0;
   uint16_t *X0 = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr;
-  uint16_t *dZ4 = nullptr, *dZ3 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
   float *O = nullptr, *feat = nullptr, *dirs = nullptr, *w = nullptr, *partials = nullptr, *scal = nullptr;
   float *groups = nullptr, *wpart = nullptr;
   uint32_t nblocks = 0;
@@ -446,8 +493,7 @@ static int32_t guide_reserve(th_guide* g, uint32_t rows) {
   size_t h = (size_t)cap * 64 * 2;
   int32_t rc;
   if ((rc = al((void**)&g->X0, h)) || (rc = al((void**)&g->H1, h)) || (rc = al((void**)&g->H2, h)) ||
-      (rc = al((void**)&g->H3, h)) || (rc = al((void**)&g->dZ4, (size_t)cap * TG_OUTP * 2)) ||
-      (rc = al((void**)&g->dZ3, h)) || (rc = al((void**)&g->dZ2, h)) || (rc = al((void**)&g->dZ1, h)) ||
+      (rc = al((void**)&g->H3, h)) ||
       (rc = al((void**)&g->O, (size_t)cap * TG_OUTP * 4)) ||
       (rc = al((void**)&g->feat, (size_t)cap * TG_IN * 4)) || (rc = al((void**)&g->dirs, (size_t)cap * 12)) ||
       (rc = al((void**)&g->w, (size_t)cap * 4)) ||
@@ -465,13 +511,13 @@ static int32_t th_guide_forward_rows(th_guide* g, const uint16_t* X, const uint3
   uint32_t grid = (round_up(rows, 64) / 64);  // 4 waves x 16 rows per block
   if (grid == 0) return TH_OK;
   hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, X, g->d_woi + 0 * 4096,
-                     g->d_bias + 0, d_count, rows, (void*)g->H1, nullptr);
+                     g->d_bias + 0, d_count, rows, (void*)g->H1);
   hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->H1, g->d_woi + 1 * 4096,
-                     g->d_bias + 64, d_count, rows, (void*)g->H2, nullptr);
+                     g->d_bias + 64, d_count, rows, (void*)g->H2);
   hipLaunchKernelGGL((k_gemm_rows<64, 64, 0>), dim3(grid), dim3(256), 0, st, g->H2, g->d_woi + 2 * 4096,
-                     g->d_bias + 128, d_count, rows, (void*)g->H3, nullptr);
+                     g->d_bias + 128, d_count, rows, (void*)g->H3);
   hipLaunchKernelGGL((k_gemm_rows<64, 64, 1>), dim3(grid), dim3(256), 0, st, g->H3, g->d_woi + 3 * 4096,
-                     g->d_bias + 192, d_count, rows, (void*)g->O, nullptr);
+                     g->d_bias + 192, d_count, rows, (void*)g->O);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -482,30 +528,15 @@ static int32_t th_guide_forward_rows(th_guide* g, const uint16_t* X, const uint3
 static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, const float* w,
                           uint32_t count, float lr) {
   hipStream_t st = g->ctx->stream;
-  uint32_t rows = round_up(count, 32);
-  int32_t rc = th_guide_forward_rows(g, X, nullptr, rows, st);
-  if (rc) return rc;
   hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, g->scal, 16u);
   hipLaunchKernelGGL(k_weight_partial, dim3(256), dim3(256), 0, st, w, count, g->wpart);
   hipLaunchKernelGGL(k_weight_total, dim3(1), dim3(256), 0, st, g->wpart, g->scal);
-  hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, g->O, dirs, w, count,
-                     rows, g->scal, g->dZ4, g->scal + 1);
-  uint32_t rows_per = round_up((rows + kWgradBlocks - 1) / kWgradBlocks, 32);
-  uint32_t nb = (rows + rows_per - 1) / rows_per;
-  uint32_t grid = round_up(rows, 64) / 64;
+  // 64-sample chunks, at most kWgradBlocks blocks (one partial slot each)
+  const uint32_t rows_per = round_up((count + kWgradBlocks - 1) / kWgradBlocks, 64);
+  const uint32_t nb = (count + rows_per - 1) / rows_per;
   float* P = g->partials;
-  const size_t s3 = (size_t)kWgradBlocks * TG_PART;
-  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H3, g->dZ4, rows_per, rows,
-                     P + 3 * s3);
-  hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ4, g->d_wio + 3 * 4096,
-                     nullptr, nullptr, rows, (void*)g->dZ3, g->H3);
-  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H2, g->dZ3, rows_per, rows, P + 2 * s3);
-  hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ3, g->d_wio + 2 * 4096,
-                     nullptr, nullptr, rows, (void*)g->dZ2, g->H2);
-  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, g->H1, g->dZ2, rows_per, rows, P + 1 * s3);
-  hipLaunchKernelGGL((k_gemm_rows<64, 64, 2>), dim3(grid), dim3(256), 0, st, g->dZ2, g->d_wio + 1 * 4096,
-                     nullptr, nullptr, rows, (void*)g->dZ1, g->H1);
-  hipLaunchKernelGGL((k_gemm_wgrad<64, 64>), dim3(nb), dim3(64), 0, st, X, g->dZ1, rows_per, rows, P + 0 * s3);
+  hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(256), 0, st, X, dirs, w, count, rows_per, g->scal, g->d_woi,
+                     g->d_wio, g->d_bias, P, g->scal + 1);
   TH_CHECK(hipGetLastError());
   float bc1 = 1.0f, bc2 = 1.0f;
   if (lr > 0.0f) {

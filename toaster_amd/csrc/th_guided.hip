@@ -435,9 +435,19 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       B.rpath[r] = k;
       v3 p01 = th::mk((p.x - A.bmin[0]) * A.bscale[0], (p.y - A.bmin[1]) * A.bscale[1],
                       (p.z - A.bmin[2]) * A.bscale[2]);
-      uint16_t* x = B.rX + (size_t)r * 64u;
-      for (int j = 0; j < TG_IN; ++j) x[j] = tg::f2bf(tg::encode_feature(j, p01, nrm, wo));
-      for (int j = TG_IN; j < 64; ++j) x[j] = 0;
+      uint4* x = (uint4*)(B.rX + (size_t)r * 64u);  // 128-byte row, eight 16-byte stores
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        uint32_t h[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int j0 = 8 * q + 2 * t;
+          const uint32_t lo = j0 < TG_IN ? tg::f2bf(tg::encode_feature(j0, p01, nrm, wo)) : 0u;
+          const uint32_t hi = j0 + 1 < TG_IN ? tg::f2bf(tg::encode_feature(j0 + 1, p01, nrm, wo)) : 0u;
+          h[t] = lo | (hi << 16);
+        }
+        x[q] = make_uint4(h[0], h[1], h[2], h[3]);
+      }
       B.rdir[3u * r] = nd.x;
       B.rdir[3u * r + 1u] = nd.y;
       B.rdir[3u * r + 2u] = nd.z;
@@ -643,7 +653,10 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   uint32_t cnt_host[32];
   for (uint32_t smp = 0; smp < p->samples; ++smp) {
     const uint32_t taken = guide_steps(g);
-    const uint32_t pass_steps = taken < opt->warmup_steps ? wrate : steps;
+    const bool warm = taken < opt->warmup_steps;
+    const uint32_t pass_steps = warm ? wrate : steps;
+    // after the warm-up, train on every train_every-th pass only
+    const bool train_pass = train && (warm || opt->train_every <= 1u || (smp % opt->train_every) == 0u);
     const uint64_t target = (uint64_t)batch * pass_steps;
     A.smp = smp;
     A.nchunks = pass_steps;
@@ -651,11 +664,11 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     A.alpha = taken == 0u ? 0.0f
               : (opt->alpha_ramp && taken < opt->alpha_ramp ? opt->alpha * (float)taken / (float)opt->alpha_ramp
                                                              : opt->alpha);
-    A.record = train ? 1u : 0u;
-    A.defer = (A.alpha > 0.0f || train) ? 1u : 0u;
+    A.record = train_pass ? 1u : 0u;
+    A.defer = (A.alpha > 0.0f || train_pass) ? 1u : 0u;
     // the reference's environment sampling stays on until the guide is in use
     A.env_nee = (A.alpha > 0.0f && (opt->flags & TH_GUIDED_NO_ENV_NEE)) ? 0u : 1u;
-    if (train) {
+    if (train_pass) {
       double pr = (double)target / (double)std::max<uint64_t>(w->diffuse_per_pass, 1);
       A.rec_all = pr >= 1.0 ? 1u : 0u;
       A.rec_thresh = (uint32_t)std::min(pr * 4294967296.0, 4294967295.0);
@@ -680,7 +693,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
       hipLaunchKernelGGL(gd_shadow, dim3(grid_shadow), dim3(block), lds, sm, B, S, *p);
       hipLaunchKernelGGL(gd_finish, dim3(grid_scalar), dim3(block), 0, sm, B, *p, li);
     }
-    if (train) {
+    if (train_pass) {
       hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((pass_steps * chunk_cap + 255) / 256, 4096u)),
                          dim3(256), 0, sm, B, pass_steps, chunk_cap);
       TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));

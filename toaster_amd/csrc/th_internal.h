@@ -63,7 +63,6 @@ struct th_scene {
   uint32_t max_depth = 0;
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
   th_guided_ws* gw = nullptr;  // allocated by the first guided render
-  float* h_stage = nullptr;    // pinned readback staging (allocated by the first readback)
   th_tuning tuning = {0, 0, 0.0f, 0};
 };
 

@@ -839,7 +839,6 @@ void th_scene_destroy(th_scene* s) {
   free_scene_buffers(s);
   th_wavefront_free(s->wf);
   th_guided_free(s->gw);
-  if (s->h_stage) (void)hipHostFree(s->h_stage);
   delete s;
 }
 
@@ -963,11 +962,8 @@ int32_t th_readback(th_context* ctx, th_scene* s, float* rgba, size_t bytes) {
   size_t need = (size_t)s->width * s->height * 4 * sizeof(float);
   if (bytes != need) return th_set_error(TH_E_INVALID, "readback buffer size mismatch");
   TH_CHECK(hipSetDevice(ctx->device));
-  // DMA into a pinned staging buffer of the scene (allocated once), then one host copy
-  if (!s->h_stage) TH_CHECK(hipHostMalloc((void**)&s->h_stage, need, hipHostMallocDefault));
-  TH_CHECK(hipMemcpyAsync(s->h_stage, s->d_accum, need, hipMemcpyDeviceToHost, ctx->stream));
+  TH_CHECK(hipMemcpyAsync(rgba, s->d_accum, need, hipMemcpyDeviceToHost, ctx->stream));
   TH_CHECK(hipStreamSynchronize(ctx->stream));
-  memcpy(rgba, s->h_stage, need);
   return TH_OK;
 }
 

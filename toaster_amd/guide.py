@@ -24,6 +24,7 @@ class Guide:
         A.check(A.lib().th_guide_create(ctx.handle, seed, C.byref(h)))
         self._h = h
         renderer._dependents.add(self)
+        ctx.adopt(self)
 
     @property
     def handle(self):
@@ -86,13 +87,15 @@ def mfma_selftest(ctx: GpuContext, a, b):
 
 # Defaults measured on the indirect-lit terrain (1280x720, 262,144 triangles): guide
 # probability 0.7, environment light by the continuation alone at guided vertices, a
-# warm-up of 128 Adam steps at 8 steps per pass with alpha ramped over the same steps.
-GUIDED_DEFAULTS = dict(alpha=0.7, env_nee=False, warmup_steps=128, warmup_rate=8, alpha_ramp=128)
+# warm-up of 384 Adam steps at 16 steps per pass with alpha ramped over the same steps,
+# then one step on every 3rd pass.
+GUIDED_DEFAULTS = dict(alpha=0.7, env_nee=False, warmup_steps=384, warmup_rate=16, alpha_ramp=384,
+                       train_every=3)
 
 
 def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.7, train=True, steps_per_pass=1,
-                  batch=1 << 18, progressive=True, env_nee=False, warmup_steps=128, warmup_rate=8,
-                  alpha_ramp=128):
+                  batch=1 << 18, progressive=True, env_nee=False, warmup_steps=384, warmup_rate=16,
+                  alpha_ramp=384, train_every=3):
     """`samples` passes of one sample per pixel with the guide (see th_render_guided);
     with progressive=True the running average and frame seed continue like
     GpuScene.render_progressive.  Returns the call's statistics as a dict."""
@@ -101,7 +104,8 @@ def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.7, train=True,
         p.accumulated_samples = 0
     o = A.GuidedOptions(alpha=alpha, train=1 if train else 0, steps_per_pass=steps_per_pass,
                         batch=batch, flags=0 if env_nee else A.TH_GUIDED_NO_ENV_NEE,
-                        warmup_steps=warmup_steps, warmup_rate=warmup_rate, alpha_ramp=alpha_ramp)
+                        warmup_steps=warmup_steps, warmup_rate=warmup_rate, alpha_ramp=alpha_ramp,
+                        train_every=train_every)
     st = A.GuidedStats()
     A.check(A.lib().th_render_guided(scene.ctx.handle, scene._h, guide.handle, C.byref(p),
                                      C.byref(scene.packed.camera), C.byref(o), C.byref(st)))

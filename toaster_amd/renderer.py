@@ -60,6 +60,7 @@ class GpuContext:
         self._lib = A.lib()
         self._h = C.c_void_p()
         A.check(self._lib.th_context_create(int(device), C.byref(self._h)))
+        self._children = weakref.WeakSet()  # scenes and guides, released before the context
         _contexts.add(self)
 
     @property
@@ -77,8 +78,14 @@ class GpuContext:
     def reset_stats(self):
         A.check(self._lib.th_reset_stats(self._h))
 
+    def adopt(self, child):
+        """Registers an object (scene, guide) that must be released before this context."""
+        self._children.add(child)
+
     def close(self):
         if self._h:
+            for child in list(self._children):
+                child.close()
             self._lib.th_context_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -106,6 +113,7 @@ class GpuScene:
         d = packed.desc()
         A.check(self._lib.th_scene_create(ctx.handle, C.byref(d), int(node_capacity), C.byref(self._h)))
         _dependents.add(self)
+        ctx.adopt(self)
         self.width, self.height = packed.params.width, packed.params.height
         self.accumulated_samples = 0
         self.frame_index = 0
