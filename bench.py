@@ -15,11 +15,11 @@ by the kernel's mean duration from HIP events around each launch.
 equal_time compares the guided tracer with the plain tracer on the indirect-lit terrain
 (262,144 triangles, 1280x720, 8 bounces): a plain reference at --reference-spp, the
 plain tracer at 64 and 256 spp, and the guided tracer at 256 spp trained online from
-scratch (all training inside the timed region), three times with independent guides
+scratch (all training inside the timed region), five times with independent guides
 and sample streams.  From the two plain points, relRMSE^2 = C / spp + E (E = the
 reference's own noise); the plain time to reach a guided run's error is
 T_plain(256) * (C / (relRMSE_guided^2 - E)) / 256, and time_ratio is the guided time over
-that, averaged over the three runs.
+that, averaged over the five runs.
 """
 import argparse
 import json
@@ -186,7 +186,7 @@ def equal_time_guided(reference_spp):
     C = (r64 ** 2 - r256 ** 2) / (1 / 64 - 1 / 256)
     E = max(r256 ** 2 - C / 256, 0.0)
     runs = []
-    for seed in (1, 2, 3):  # independent guides and sample streams: the ratio's spread
+    for seed in (1, 2, 3, 4, 5):  # independent guides and sample streams: the ratio's spread
         g = Guide(ctx, seed=seed)
         gs.frame_index = 2 + seed
         dtg, st = timed(lambda: render_guided(gs, g, 256, progressive=False, **GUIDED_DEFAULTS))

@@ -63,6 +63,22 @@ TH_HD float fexp(float x) {
 #endif
 static_assert(6 * TG_POS_OCT + 6 * TG_DIR_BINS == TG_IN, "encoding width");
 
+// Position features j and j + 1 (j even, j < 6 TG_POS_OCT): sin and cos of
+// 2 pi 2^k x for axis a = j / (2 TG_POS_OCT), octave k.  The device uses the hardware
+// sine/cosine (angles stay below 2 pi 2^TG_POS_OCT, well inside its range); training
+// records and inference both encode on the device, so they see the same features.
+TH_HD void encode_pos_pair(int j, v3 p01, float* s, float* c) {
+  int a = j / (2 * TG_POS_OCT), k = (j % (2 * TG_POS_OCT)) >> 1;
+  float x = a == 0 ? p01.x : (a == 1 ? p01.y : p01.z);
+  float ang = 6.2831855f * (float)(1 << k) * x;
+#if defined(__HIP_DEVICE_COMPILE__)
+  __sincosf(ang, s, c);
+#else
+  *s = sinf(ang);
+  *c = cosf(ang);
+#endif
+}
+
 // Feature j (0..47) of the encoding of a vertex: scene-bounds-normalised position p01
 // in [0,1]^3 -> sin/cos(2 pi 2^k x) for k < TG_POS_OCT per axis; the normal and outgoing
 // direction components c -> u = (c + 1)/2 spread over TG_DIR_BINS Gaussian bins at
@@ -70,10 +86,8 @@ static_assert(6 * TG_POS_OCT + 6 * TG_DIR_BINS == TG_IN, "encoding width");
 // so lanes can share the work of one row.
 TH_HD float encode_feature(int j, v3 p01, v3 n, v3 wo) {
   if (j < 6 * TG_POS_OCT) {
-    int a = j / (2 * TG_POS_OCT), k = (j % (2 * TG_POS_OCT)) >> 1;
-    float x = a == 0 ? p01.x : (a == 1 ? p01.y : p01.z);
     float s, c;
-    th::sincos_(6.2831855f * (float)(1 << k) * x, &s, &c);
+    encode_pos_pair(j & ~1, p01, &s, &c);
     return (j & 1) ? c : s;
   }
   int jj = j - 6 * TG_POS_OCT;
@@ -82,6 +96,12 @@ TH_HD float encode_feature(int j, v3 p01, v3 n, v3 wo) {
   float u = 0.5f * (v + 1.0f);
   float d = (u - ((float)b + 0.5f) * (1.0f / (float)TG_DIR_BINS)) * (float)TG_DIR_BINS;
   return fexp(-0.5f * d * d);
+}
+
+// The whole row, each sin/cos pair computed once.
+TH_HD void encode_row(v3 p01, v3 n, v3 wo, float* out /* TG_IN */) {
+  for (int j = 0; j < 6 * TG_POS_OCT; j += 2) encode_pos_pair(j, p01, &out[j], &out[j + 1]);
+  for (int j = 6 * TG_POS_OCT; j < TG_IN; ++j) out[j] = encode_feature(j, p01, n, wo);
 }
 
 // ------------------------------------------------------------------ mixture

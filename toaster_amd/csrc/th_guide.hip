@@ -376,19 +376,18 @@ __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
 }
 
 // Fused inference over a queue of deferred diffuse vertices, one wave per 16-row tile
-// and persistent over the queue.  Lane (r, g) encodes features 12g..12g+11 of row r into
-// an LDS tile; the four layers then run on MFMA with every weight fragment held in
-// registers (loaded once per wave) and activations passed through the same tile (BF16,
-// row stride 72 elements = 144 bytes, 16-byte aligned rows); each row's 40 outputs are
-// written in FP32.  Queue entry i names the path whose vertex is (gp.xyz, gn.xyz) with
-// outgoing direction -gd.xyz.
+// and persistent over the queue.  Row i of the queue is the vertex (p, n, wo) stored in
+// queue order (vp, vn, vw).  Lane (r, g) encodes features 12g..12g+11 of row r into an
+// LDS tile (each sin/cos pair once); the four layers then run on MFMA with every weight
+// fragment held in registers (loaded once per wave) and activations passed through the
+// same tile (BF16, row stride 72 elements = 144 bytes, 16-byte aligned rows); each row's
+// 40 outputs are written in FP32.
 __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__ woi,
                                                     const float* __restrict__ bias,
                                                     const uint32_t* __restrict__ count_ptr,
-                                                    const uint32_t* __restrict__ queue,
-                                                    const float4* __restrict__ gp,
-                                                    const float4* __restrict__ gn,
-                                                    const float4* __restrict__ gd, float4 bmin,
+                                                    const float4* __restrict__ vp,
+                                                    const float4* __restrict__ vn,
+                                                    const float4* __restrict__ vw, float4 bmin,
                                                     float4 bscale, float* __restrict__ O) {
   __shared__ __align__(16) uint16_t T[16 * 72];
   const uint32_t count = *count_ptr;
@@ -408,14 +407,23 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
     const uint32_t row = base + r;
     v3 p01 = th::mk(0.5f, 0.5f, 0.5f), n = th::mk(0.0f, 1.0f, 0.0f), wo = n;
     if (row < count) {
-      const uint32_t k = queue[row];
-      const float4 a = gp[k], b = gn[k], c = gd[k];
+      const float4 a = vp[row], b = vn[row], c = vw[row];
       p01 = th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z);
       n = th::mk(b.x, b.y, b.z);
-      wo = th::mk(-c.x, -c.y, -c.z);
+      wo = th::mk(c.x, c.y, c.z);
     }
 #pragma unroll
-    for (int j = 0; j < 12; ++j) T[r * 72 + 12 * g + j] = tg::f2bf(tg::encode_feature(12 * (int)g + j, p01, n, wo));
+    for (int jj = 0; jj < 12; jj += 2) {
+      const int j = 12 * (int)g + jj;
+      float f0, f1;
+      if (j < 6 * TG_POS_OCT) {
+        tg::encode_pos_pair(j, p01, &f0, &f1);
+      } else {
+        f0 = tg::encode_feature(j, p01, n, wo);
+        f1 = tg::encode_feature(j + 1, p01, n, wo);
+      }
+      *(uint32_t*)(T + r * 72 + j) = (uint32_t)tg::f2bf(f0) | ((uint32_t)tg::f2bf(f1) << 16);
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) T[r * 72 + 48 + 4 * g + j] = 0;
     __syncthreads();
@@ -555,9 +563,9 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
 }
 
 // ====================================================================== renderer interface
-int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const uint32_t* queue,
-                    const float4* gp, const float4* gn, const float4* gd, const float* bmin,
-                    const float* bmax, float* O, hipStream_t st) {
+int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const float4* vp,
+                    const float4* vn, const float4* vw, const float* bmin, const float* bmax, float* O,
+                    hipStream_t st) {
   uint32_t grid = std::min<uint32_t>((max_rows + 15u) / 16u, 8192u);
   if (grid == 0) return TH_OK;
   float4 lo = make_float4(bmin[0], bmin[1], bmin[2], 0.0f), sc;
@@ -565,8 +573,8 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, con
   for (int a = 0; a < 3; ++a) e[a] = bmax[a] - bmin[a];
   sc = make_float4(e[0] > 0.0f ? 1.0f / e[0] : 0.0f, e[1] > 0.0f ? 1.0f / e[1] : 0.0f,
                    e[2] > 0.0f ? 1.0f / e[2] : 0.0f, 0.0f);
-  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, st, g->d_woi, g->d_bias, d_count, queue, gp,
-                     gn, gd, lo, sc, O);
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, st, g->d_woi, g->d_bias, d_count, vp, vn, vw,
+                     lo, sc, O);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -584,6 +592,10 @@ uint32_t guide_steps(const th_guide* g) { return g->step; }
 th_context* guide_context(const th_guide* g) { return g->ctx; }
 
 // ====================================================================== C ABI
+// He-uniform hidden weights (bound sqrt(6 / fan_in)) from a seeded PCG stream, zero hidden
+// biases; the output layer starts at a tenth of that scale with biases that make the
+// initial mixture cover the sphere evenly: lobe means on a Fibonacci sphere, kappa = 2,
+// equal weights.
 static void host_init_params(uint32_t seed, std::vector<float>& p) {
   p.assign(TG_NPARAMS, 0.0f);
   uint32_t s = seed * 2654435761u + 12345u;
@@ -594,6 +606,17 @@ static void host_init_params(uint32_t seed, std::vector<float>& p) {
       s = th::pcg_hash(s);
       p[D.woff + i] = ((float)(s >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * a;
     }
+  }
+  float* b4 = p.data() + layer_dims(3).boff;
+  const float golden = 2.39996323f;  // pi (3 - sqrt 5)
+  for (int k = 0; k < TG_LOBES; ++k) {
+    float z = 1.0f - (2.0f * (float)k + 1.0f) / (float)TG_LOBES;
+    float r = sqrtf(std::max(0.0f, 1.0f - z * z));
+    b4[3 * k] = r * cosf(golden * (float)k);
+    b4[3 * k + 1] = z;
+    b4[3 * k + 2] = r * sinf(golden * (float)k);
+    b4[24 + k] = logf(expf(2.0f - 0.01f) - 1.0f);  // softplus^-1(kappa - 0.01), kappa = 2
+    b4[32 + k] = 0.0f;
   }
 }
 

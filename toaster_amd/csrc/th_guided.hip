@@ -45,8 +45,7 @@ struct GdBufs {
   float4 *o, *d, *thr, *rad, *acc;  // o.w previous continuation pdf, d.w flags, thr.w record
   uint32_t* rng;
   uint2* hit;
-  float4 *gp, *gn;  // parked diffuse vertex: (p, albedo.x), (n, albedo.y)
-  float* gz;        // albedo.z
+  float4 *vp, *vn, *vw;  // guide-queue row: (p, albedo.x), (n, albedo.y), (wo, albedo.z)
   float4 *sho, *sh0, *c0, *sh1, *c1;
   uint32_t* vis;
   uint32_t *q0, *q1, *qs, *qg;
@@ -260,6 +259,7 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
     if (!__any(act)) break;
     bool sh_light = false, sh_env = false, park = false;
     uint32_t k = 0;
+    float4 pk_p, pk_n, pk_w;  // the parked vertex
     if (act) {
       k = q[i];
       float4 o4 = B.o[k], d4 = B.d[k], t4 = B.thr[k], r4 = B.rad[k];
@@ -298,9 +298,9 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
         if (m.kind == 0u) {
           v3 albedo = it.base_color(m, rec);
           if (A.defer) {
-            B.gp[k] = make_float4(rec.p.x, rec.p.y, rec.p.z, albedo.x);
-            B.gn[k] = make_float4(rec.n.x, rec.n.y, rec.n.z, albedo.y);
-            B.gz[k] = albedo.z;
+            pk_p = make_float4(rec.p.x, rec.p.y, rec.p.z, albedo.x);
+            pk_n = make_float4(rec.n.x, rec.n.y, rec.n.z, albedo.y);
+            pk_w = make_float4(-d.x, -d.y, -d.z, albedo.z);
             park = true;
           } else {
             float pdf;
@@ -361,7 +361,12 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
     }
     gd_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
     gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
-    gd_push(park, k, B.qg, &B.cnt[3]);
+    const uint32_t gi = gd_push(park, k, B.qg, &B.cnt[3]);
+    if (park) {  // vertex data in queue order: contiguous rows for inference and shade B
+      B.vp[gi] = pk_p;
+      B.vn[gi] = pk_n;
+      B.vw[gi] = pk_w;
+    }
   }
 }
 
@@ -383,11 +388,11 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
     v3 p, nrm, wo, nd, beta;
     if (act) {
       k = B.qg[i];
-      float4 gp = B.gp[k], gn = B.gn[k], d4 = B.d[k], t4 = B.thr[k];
+      const float4 gp = B.vp[i], gn = B.vn[i], gw = B.vw[i], t4 = B.thr[k];
       p = th::mk(gp.x, gp.y, gp.z);
       nrm = th::mk(gn.x, gn.y, gn.z);
-      wo = th::mk(-d4.x, -d4.y, -d4.z);
-      v3 albedo = th::mk(gp.w, gn.w, B.gz[k]);
+      wo = th::mk(gw.x, gw.y, gw.z);
+      v3 albedo = th::mk(gp.w, gn.w, gw.w);
       v3 throughput = th::mk(t4.x, t4.y, t4.z);
       beta = throughput;
       uint32_t rng = B.rng[k];
@@ -435,6 +440,8 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       B.rpath[r] = k;
       v3 p01 = th::mk((p.x - A.bmin[0]) * A.bscale[0], (p.y - A.bmin[1]) * A.bscale[1],
                       (p.z - A.bmin[2]) * A.bscale[2]);
+      float feat[TG_IN];
+      tg::encode_row(p01, nrm, wo, feat);
       uint4* x = (uint4*)(B.rX + (size_t)r * 64u);  // 128-byte row, eight 16-byte stores
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -442,8 +449,8 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int j0 = 8 * q + 2 * t;
-          const uint32_t lo = j0 < TG_IN ? tg::f2bf(tg::encode_feature(j0, p01, nrm, wo)) : 0u;
-          const uint32_t hi = j0 + 1 < TG_IN ? tg::f2bf(tg::encode_feature(j0 + 1, p01, nrm, wo)) : 0u;
+          const uint32_t lo = j0 < TG_IN ? tg::f2bf(feat[j0 < TG_IN ? j0 : 0]) : 0u;
+          const uint32_t hi = j0 + 1 < TG_IN ? tg::f2bf(feat[j0 + 1 < TG_IN ? j0 + 1 : 0]) : 0u;
           h[t] = lo | (hi << 16);
         }
         x[q] = make_uint4(h[0], h[1], h[2], h[3]);
@@ -576,8 +583,8 @@ static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
       (rc = gd_alloc(w, (void**)&b.thr, f4)) || (rc = gd_alloc(w, (void**)&b.rad, f4)) ||
       (rc = gd_alloc(w, (void**)&b.acc, f4)) || (rc = gd_alloc(w, (void**)&b.rng, u)) ||
       (rc = gd_alloc(w, (void**)&b.hit, (size_t)n * sizeof(uint2))) ||
-      (rc = gd_alloc(w, (void**)&b.gp, f4)) || (rc = gd_alloc(w, (void**)&b.gn, f4)) ||
-      (rc = gd_alloc(w, (void**)&b.gz, u)) || (rc = gd_alloc(w, (void**)&b.sho, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.vp, f4)) || (rc = gd_alloc(w, (void**)&b.vn, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.vw, f4)) || (rc = gd_alloc(w, (void**)&b.sho, f4)) ||
       (rc = gd_alloc(w, (void**)&b.sh0, f4)) || (rc = gd_alloc(w, (void**)&b.c0, f4)) ||
       (rc = gd_alloc(w, (void**)&b.sh1, f4)) || (rc = gd_alloc(w, (void**)&b.c1, f4)) ||
       (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, u)) ||
@@ -683,7 +690,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
                          (const float*)ctx->d_srgb_lut, li);
       if (A.defer) {
         if (A.alpha > 0.0f &&
-            (rc = guide_infer(g, B.cnt + 3, npix, B.qg, B.gp, B.gn, B.d, s->root_min, s->root_max, B.O, sm))) {
+            (rc = guide_infer(g, B.cnt + 3, npix, B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm))) {
           (void)end_timed(ctx, evp);
           return rc;
         }

@@ -86,14 +86,14 @@ def mfma_selftest(ctx: GpuContext, a, b):
 
 
 # Defaults measured on the indirect-lit terrain (1280x720, 262,144 triangles): guide
-# probability 0.7, environment light by the continuation alone at guided vertices, a
+# probability 0.8, environment light by the continuation alone at guided vertices, a
 # warm-up of 384 Adam steps at 16 steps per pass with alpha ramped over the same steps,
 # then one step on every 3rd pass.
-GUIDED_DEFAULTS = dict(alpha=0.7, env_nee=False, warmup_steps=384, warmup_rate=16, alpha_ramp=384,
+GUIDED_DEFAULTS = dict(alpha=0.8, env_nee=False, warmup_steps=384, warmup_rate=16, alpha_ramp=384,
                        train_every=3)
 
 
-def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.7, train=True, steps_per_pass=1,
+def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.8, train=True, steps_per_pass=1,
                   batch=1 << 18, progressive=True, env_nee=False, warmup_steps=384, warmup_rate=16,
                   alpha_ramp=384, train_every=3):
     """`samples` passes of one sample per pixel with the guide (see th_render_guided);
