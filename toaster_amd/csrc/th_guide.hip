@@ -403,11 +403,26 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb) W[L][nb][kb] = ld8(woi + L * 4096 + (nb * 16 + r) * 64 + kb * 32 + g * 8);
     }
-  for (uint32_t base = blockIdx.x * 16u; base < count; base += gridDim.x * 16u) {
+  // the next tile's vertex rows are loaded while the current tile runs its layers
+  const float4 zero4 = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
+  uint32_t base = blockIdx.x * 16u;
+  float4 na = zero4, nbv = zero4, nc = zero4;
+  if (base + r < count) {
+    na = vp[base + r];
+    nbv = vn[base + r];
+    nc = vw[base + r];
+  }
+  for (; base < count; base += gridDim.x * 16u) {
     const uint32_t row = base + r;
+    const float4 a = na, b = nbv, c = nc;
+    const uint32_t nrow = row + gridDim.x * 16u;
+    if (nrow < count) {
+      na = vp[nrow];
+      nbv = vn[nrow];
+      nc = vw[nrow];
+    }
     v3 p01 = th::mk(0.5f, 0.5f, 0.5f), n = th::mk(0.0f, 1.0f, 0.0f), wo = n;
     if (row < count) {
-      const float4 a = vp[row], b = vn[row], c = vw[row];
       p01 = th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z);
       n = th::mk(b.x, b.y, b.z);
       wo = th::mk(c.x, c.y, c.z);

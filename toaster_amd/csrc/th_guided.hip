@@ -10,10 +10,13 @@
 //   shade B  diffuse vertices: direct light and environment terms with deferred
 //            visibility, then the continuation by one-sample MIS between the guide
 //            (probability alpha) and the reference's cosine lobe;
-//   shadow   any hit, 4 lanes per ray;
-//   finish   visible direct terms, completed samples, compaction of live paths.
-// After each pass the training records of the pass receive their weights and the guide
-// takes its Adam steps on them.
+//   shadow   any hit, 4 lanes per ray.
+// The shade kernels compact continuing paths straight into the next live queue and
+// finished ones into a done queue; a path's deferred direct terms are added (in the
+// reference's order, light then environment) when the next shade A picks it up, or by
+// the retire kernel that closes the pass for finished paths.  After each pass the
+// training records of the pass receive their weights and the guide takes its Adam
+// steps on them.
 //
 // Estimator.  The continuation density is p(w) = alpha q(w) + (1 - alpha) cos/pi, the
 // throughput takes albedo (cos/pi) / p(w), and both the environment term's MIS weight
@@ -48,9 +51,9 @@ struct GdBufs {
   float4 *vp, *vn, *vw;  // guide-queue row: (p, albedo.x), (n, albedo.y), (wo, albedo.z)
   float4 *sho, *sh0, *c0, *sh1, *c1;
   uint32_t* vis;
-  uint32_t *q0, *q1, *qs, *qg;
+  uint32_t *q0, *q1, *qs, *qg, *qd;
   uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [4] records offered,
-                  // [5] diffuse total, [8 + c] records of chunk c
+                  // [5] diffuse total, [6] done, [8 + c] records of chunk c
   float* O;       // guide outputs, TG_OUT per guide-queue row
   uint16_t* rX;   // training records: BF16 features, 64 wide (zero beyond 48)
   float *rdir, *rw;
@@ -60,6 +63,7 @@ struct GdBufs {
 };
 
 #define GD_MAX_CHUNKS 16u
+#define GD_C(c) ((c) * 32u)  // queue counters 128 bytes apart: no two share a cache line
 
 struct GdArgs {
   uint32_t n;           // paths per pass (pixels)
@@ -94,7 +98,7 @@ __device__ __forceinline__ uint32_t gd_push(bool live, uint32_t value, uint32_t*
 
 __global__ void gd_reset(GdBufs B, uint32_t all) {
   const uint32_t t = threadIdx.x;
-  if (t < 5u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[t] = 0u;
+  if (t < 5u || t == 6u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[GD_C(t)] = 0u;
 }
 
 // ---------------------------------------------------------------- generate
@@ -119,17 +123,18 @@ __global__ void __launch_bounds__(256) gd_generate(GdBufs B, GdArgs A, TH_GpuRen
     }
     B.acc[k] = a;
   }
-  gd_push(live, k, B.q0, &B.cnt[0]);
+  gd_push(live, k, B.q0, &B.cnt[GD_C(0)]);
 }
 
 // ---------------------------------------------------------------- extend / shadow
 __global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRenderParams P, uint32_t li) {
   extern __shared__ uint2 lds_stack[];
   const uint32_t* q = li ? B.q1 : B.q0;
-  const uint32_t n = B.cnt[li];
+  const uint32_t n = B.cnt[GD_C(li)];
   if (blockIdx.x == 0 && threadIdx.x == 0) {  // queues filled later in this iteration
-    B.cnt[2] = 0;
-    B.cnt[3] = 0;
+    B.cnt[GD_C(2)] = 0;
+    B.cnt[GD_C(3)] = 0;
+    B.cnt[GD_C(li ^ 1u)] = 0;
   }
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
   const uint32_t stride = gridDim.x * blockDim.x / 4u;
@@ -155,7 +160,7 @@ __global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRend
 
 __global__ void __launch_bounds__(64) gd_shadow(GdBufs B, DevScene S, TH_GpuRenderParams P) {
   extern __shared__ uint2 lds_stack[];
-  const uint32_t n = B.cnt[2];
+  const uint32_t n = B.cnt[GD_C(2)];
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
   const uint32_t stride = gridDim.x * blockDim.x / 4u;
   for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / 4u;; i += stride) {
@@ -244,12 +249,32 @@ __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, fal
   return true;
 }
 
+// Adds the deferred direct terms of the previous vertex (light first, then environment,
+// wgsl:846-847) and stores the radiance a pending training record waits for.
+__device__ __forceinline__ void apply_pending(const GdBufs& B, uint32_t k, uint32_t flags, float4& r4,
+                                              float4& t4) {
+  if (flags & (GD_PEND_LIGHT | GD_PEND_ENV)) {
+    if ((flags & GD_PEND_LIGHT) && B.vis[2u * k]) {
+      const float4 c = B.c0[k];
+      r4 = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, 0.0f);
+    }
+    if ((flags & GD_PEND_ENV) && B.vis[2u * k + 1u]) {
+      const float4 c = B.c1[k];
+      r4 = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, 0.0f);
+    }
+  }
+  const uint32_t rec = __float_as_uint(t4.w);
+  if (rec != GD_NO_RECORD) {
+    B.rL[rec] = r4;
+    t4.w = __uint_as_float(GD_NO_RECORD);
+  }
+}
+
 // ---------------------------------------------------------------- shade A
 __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
                                                  const float* __restrict__ srgb_lut, uint32_t li) {
   const uint32_t* q = li ? B.q1 : B.q0;
-  const uint32_t n = B.cnt[li];
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[li ^ 1u] = 0;  // next live queue
+  const uint32_t n = B.cnt[GD_C(li)];
   Stack st{nullptr, 0, 0};
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
@@ -257,12 +282,13 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
-    bool sh_light = false, sh_env = false, park = false;
+    bool sh_light = false, sh_env = false, park = false, live = false, fin = false;
     uint32_t k = 0;
     float4 pk_p, pk_n, pk_w;  // the parked vertex
     if (act) {
       k = q[i];
       float4 o4 = B.o[k], d4 = B.d[k], t4 = B.thr[k], r4 = B.rad[k];
+      apply_pending(B, k, __float_as_uint(d4.w), r4, t4);
       uint32_t rng = B.rng[k];
       uint2 h = B.hit[k];
       v3 d = th::mk(d4.x, d4.y, d4.z);
@@ -355,13 +381,17 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
         B.o[k] = o_next;
         B.d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(f));
         B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, t4.w);
-        B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
         B.rng[k] = rng;
+        live = !done;
+        fin = done;
       }
+      B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
     }
-    gd_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
-    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
-    const uint32_t gi = gd_push(park, k, B.qg, &B.cnt[3]);
+    gd_push(sh_light, 2u * k, B.qs, &B.cnt[GD_C(2)]);
+    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
+    gd_push(live, k, li ? B.q0 : B.q1, &B.cnt[GD_C(li ^ 1u)]);
+    gd_push(fin, k, B.qd, &B.cnt[GD_C(6)]);
+    const uint32_t gi = gd_push(park, k, B.qg, &B.cnt[GD_C(3)]);
     if (park) {  // vertex data in queue order: contiguous rows for inference and shade B
       B.vp[gi] = pk_p;
       B.vn[gi] = pk_n;
@@ -372,9 +402,9 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
 
 // ---------------------------------------------------------------- shade B
 __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
-                                                 const float* __restrict__ srgb_lut) {
-  const uint32_t n = B.cnt[3];
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.cnt[5], n);
+                                                 const float* __restrict__ srgb_lut, uint32_t li) {
+  const uint32_t n = B.cnt[GD_C(3)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.cnt[GD_C(5)], n);
   Stack st{nullptr, 0, 0};
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
@@ -383,7 +413,7 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
-    bool sh_light = false, sh_env = false, rec_ok = false;
+    bool sh_light = false, sh_env = false, rec_ok = false, live = false, fin = false;
     uint32_t k = 0;
     v3 p, nrm, wo, nd, beta;
     if (act) {
@@ -413,10 +443,14 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       B.d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(f));
       B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, __uint_as_float(GD_NO_RECORD));
       B.rng[k] = rng;
+      live = !done;
+      fin = done;
     }
-    gd_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
-    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
-    gd_push(rec_ok, 0u, nullptr, &B.cnt[4]);  // records offered
+    gd_push(sh_light, 2u * k, B.qs, &B.cnt[GD_C(2)]);
+    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
+    gd_push(live, k, li ? B.q0 : B.q1, &B.cnt[GD_C(li ^ 1u)]);
+    gd_push(fin, k, B.qd, &B.cnt[GD_C(6)]);
+    gd_push(rec_ok, 0u, nullptr, &B.cnt[GD_C(4)]);  // records offered
     // chunk drawn at random so every training step sees an unbiased subset; slots are
     // handed out with one atomic per (wave, chunk)
     uint32_t r = 0xffffffffu;
@@ -428,7 +462,7 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       const uint32_t cc = __shfl(myc, leader, 64);
       const uint64_t m = __ballot(myc == cc);
       uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(&B.cnt[8u + cc], (uint32_t)__popcll(m));
+      if ((int)lane == leader) base = atomicAdd(&B.cnt[GD_C(8u + cc)], (uint32_t)__popcll(m));
       base = __shfl(base, leader, 64);
       if (myc == cc) {
         const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -464,49 +498,18 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
   }
 }
 
-// ---------------------------------------------------------------- finish
-__global__ void __launch_bounds__(64) gd_finish(GdBufs B, TH_GpuRenderParams P, uint32_t li) {
-  const uint32_t* q = li ? B.q1 : B.q0;
-  uint32_t* qn = li ? B.q0 : B.q1;
-  const uint32_t n = B.cnt[li];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
-    bool act = i < n;
-    if (!__any(act)) break;
-    bool live = false;
-    uint32_t k = 0;
-    if (act) {
-      k = q[i];
-      float4 d4 = B.d[k];
-      uint32_t f = __float_as_uint(d4.w);
-      float4 r4 = B.rad[k];
-      if (f & (GD_PEND_LIGHT | GD_PEND_ENV)) {
-        // radiance += throughput * direct term, light first then environment (wgsl:846-847)
-        if ((f & GD_PEND_LIGHT) && B.vis[2u * k]) {
-          float4 c = B.c0[k];
-          r4 = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, 0.0f);
-        }
-        if ((f & GD_PEND_ENV) && B.vis[2u * k + 1u]) {
-          float4 c = B.c1[k];
-          r4 = make_float4(r4.x + c.x, r4.y + c.y, r4.z + c.z, 0.0f);
-        }
-        B.rad[k] = r4;
-      }
-      uint32_t rec = __float_as_uint(B.thr[k].w);
-      if (rec != GD_NO_RECORD) {
-        B.rL[rec] = r4;
-        B.thr[k].w = __uint_as_float(GD_NO_RECORD);
-      }
-      if (f & GD_DONE) {
-        // color += vec4(ray_color, 1) (wgsl:536)
-        float4 a = B.acc[k];
-        B.acc[k] = make_float4(a.x + r4.x, a.y + r4.y, a.z + r4.z, a.w + 1.0f);
-      } else {
-        B.d[k] = make_float4(d4.x, d4.y, d4.z, __uint_as_float(f & (GD_INCLUDE_EMISSIVE | GD_PREV_DIFFUSE)));
-        live = true;
-      }
-    }
-    gd_push(live, k, qn, &B.cnt[li ^ 1u]);
+// ---------------------------------------------------------------- retire
+// Paths finished during the pass: their last deferred direct terms, then
+// color += vec4(ray_color, 1) (wgsl:536).
+__global__ void __launch_bounds__(256) gd_retire(GdBufs B) {
+  const uint32_t n = B.cnt[GD_C(6)];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t k = B.qd[i];
+    float4 r4 = B.rad[k], t4 = B.thr[k];
+    apply_pending(B, k, __float_as_uint(B.d[k].w), r4, t4);
+    B.rad[k] = r4;
+    const float4 a = B.acc[k];
+    B.acc[k] = make_float4(a.x + r4.x, a.y + r4.y, a.z + r4.z, a.w + 1.0f);
   }
 }
 
@@ -516,7 +519,7 @@ __global__ void __launch_bounds__(256) gd_weights(GdBufs B, uint32_t nchunks, ui
   const uint32_t n = nchunks * chunk_cap;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
     const uint32_t c = r / chunk_cap;
-    if (r - c * chunk_cap >= min(B.cnt[8u + c], chunk_cap)) continue;
+    if (r - c * chunk_cap >= min(B.cnt[GD_C(8u + c)], chunk_cap)) continue;
     float4 L = B.rad[B.rpath[r]], Lv = B.rL[r], b = B.rbeta[r];
     float v0 = b.x > 0.0f ? (L.x - Lv.x) / b.x : 0.0f;
     float v1 = b.y > 0.0f ? (L.y - Lv.y) / b.y : 0.0f;
@@ -589,7 +592,7 @@ static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
       (rc = gd_alloc(w, (void**)&b.sh1, f4)) || (rc = gd_alloc(w, (void**)&b.c1, f4)) ||
       (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, u)) ||
       (rc = gd_alloc(w, (void**)&b.q1, u)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * u)) ||
-      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.cnt, 32 * sizeof(uint32_t))) ||
+      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.qd, u)) || (rc = gd_alloc(w, (void**)&b.cnt, GD_C(32) * sizeof(uint32_t))) ||
       (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.rX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
       (rc = gd_alloc(w, (void**)&b.rdir, (size_t)rcap * 3 * sizeof(float))) ||
@@ -657,7 +660,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if ((rc = begin_timed(ctx, &evp))) return rc;
   hipStream_t sm = ctx->stream;
   hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 1u);
-  uint32_t cnt_host[32];
+  uint32_t cnt_host[GD_C(32)];
   for (uint32_t smp = 0; smp < p->samples; ++smp) {
     const uint32_t taken = guide_steps(g);
     const bool warm = taken < opt->warmup_steps;
@@ -690,26 +693,26 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
                          (const float*)ctx->d_srgb_lut, li);
       if (A.defer) {
         if (A.alpha > 0.0f &&
-            (rc = guide_infer(g, B.cnt + 3, npix, B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm))) {
+            (rc = guide_infer(g, B.cnt + GD_C(3), npix, B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm))) {
           (void)end_timed(ctx, evp);
           return rc;
         }
         hipLaunchKernelGGL(gd_shade_b, dim3(grid_scalar), dim3(block), 0, sm, B, A, S, *p,
-                           (const float*)ctx->d_srgb_lut);
+                           (const float*)ctx->d_srgb_lut, li);
       }
       hipLaunchKernelGGL(gd_shadow, dim3(grid_shadow), dim3(block), lds, sm, B, S, *p);
-      hipLaunchKernelGGL(gd_finish, dim3(grid_scalar), dim3(block), 0, sm, B, *p, li);
     }
+    hipLaunchKernelGGL(gd_retire, dim3(std::min<uint32_t>((npix + 255) / 256, 2048u)), dim3(256), 0, sm, B);
     if (train_pass) {
       hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((pass_steps * chunk_cap + 255) / 256, 4096u)),
                          dim3(256), 0, sm, B, pass_steps, chunk_cap);
       TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
       TH_CHECK(hipStreamSynchronize(sm));
-      const uint64_t diffuse_total = cnt_host[5];
+      const uint64_t diffuse_total = cnt_host[GD_C(5)];
       w->diffuse_per_pass = std::max<uint64_t>(diffuse_total - diffuse_before, 1);
       diffuse_before = diffuse_total;
       for (uint32_t j = 0; j < pass_steps; ++j) {
-        const uint32_t cnt = std::min(cnt_host[8 + j], chunk_cap);
+        const uint32_t cnt = std::min(cnt_host[GD_C(8 + j)], chunk_cap);
         st.records += cnt;
         if (cnt < 1024u) continue;  // too few samples for a step
         const size_t off = (size_t)j * chunk_cap;
@@ -731,7 +734,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if (stats) {
     TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
     TH_CHECK(hipStreamSynchronize(sm));
-    st.diffuse_vertices = cnt_host[5];
+    st.diffuse_vertices = cnt_host[GD_C(5)];
     *stats = st;
   }
   return TH_OK;
