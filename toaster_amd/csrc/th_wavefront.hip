@@ -22,6 +22,7 @@ namespace {
 #define WF_DONE 4u
 #define WF_PEND_LIGHT 8u
 #define WF_PEND_ENV 16u
+#define WF_C(c) ((c) * 32u)  // queue counters 128 bytes apart: no two share a cache line
 
 struct WfBufs {
   float4 *o, *d, *thr, *rad, *acc;  // o.w = prev bsdf pdf, d.w = flag bits
@@ -75,7 +76,7 @@ __device__ __forceinline__ void wf_push(bool live, uint32_t value, uint32_t* que
 }
 
 __global__ void wf_reset(WfBufs B) {
-  if (threadIdx.x < 4u) B.cnt[threadIdx.x] = 0u;
+  if (threadIdx.x < 4u) B.cnt[WF_C(threadIdx.x)] = 0u;
 }
 
 // ---------------------------------------------------------------- generate
@@ -83,8 +84,8 @@ __global__ void __launch_bounds__(256) wf_generate(WfBufs B, WfArgs A, TH_GpuRen
                                                    TH_GpuCamera cam) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k == 0) {
-    B.cnt[1] = 0;
-    B.cnt[2] = 0;
+    B.cnt[WF_C(1)] = 0;
+    B.cnt[WF_C(2)] = 0;
   }
   bool live = false;
   if (k < A.n_slots) {
@@ -98,7 +99,7 @@ __global__ void __launch_bounds__(256) wf_generate(WfBufs B, WfArgs A, TH_GpuRen
       B.st[k] = make_uint4(0u, smp, 0u, pixel);
     }
   }
-  wf_push(live, k, B.q0, &B.cnt[0]);
+  wf_push(live, k, B.q0, &B.cnt[WF_C(0)]);
 }
 
 // ---------------------------------------------------------------- extend
@@ -107,8 +108,8 @@ __global__ void __launch_bounds__(64) wf_extend(WfBufs B, DevScene S, TH_GpuRend
                                                 uint32_t live_idx, uint32_t per_quad) {
   extern __shared__ uint2 lds_stack[];
   const uint32_t* q = live_idx ? B.q1 : B.q0;
-  uint32_t n = B.cnt[live_idx];
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[2] = 0;  // shadow queue of this iteration
+  uint32_t n = B.cnt[WF_C(live_idx)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[WF_C(2)] = 0;  // shadow queue of this iteration
   uint32_t lanes_per = per_quad ? 4u : 1u;
   Stack st{lds_stack + (per_quad ? (threadIdx.x >> 2) : threadIdx.x),
            (int)(per_quad ? (blockDim.x >> 2) : blockDim.x), 0};
@@ -140,8 +141,8 @@ __global__ void __launch_bounds__(64) wf_extend(WfBufs B, DevScene S, TH_GpuRend
 __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRenderParams P,
                                                const float* __restrict__ srgb_lut, uint32_t live_idx) {
   const uint32_t* q = live_idx ? B.q1 : B.q0;
-  uint32_t n = B.cnt[live_idx];
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[live_idx ^ 1u] = 0;  // next live queue
+  uint32_t n = B.cnt[WF_C(live_idx)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[WF_C(live_idx ^ 1u)] = 0;  // next live queue
   Stack st{nullptr, 0, 0};
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
@@ -262,8 +263,8 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
       B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
       B.st[k] = make_uint4(rng, s4.y, bounce, s4.w);
     }
-    wf_push(sh_light, 2u * k, B.qs, &B.cnt[2]);
-    wf_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[2]);
+    wf_push(sh_light, 2u * k, B.qs, &B.cnt[WF_C(2)]);
+    wf_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[WF_C(2)]);
   }
 }
 
@@ -272,7 +273,7 @@ template <int TMODE>
 __global__ void __launch_bounds__(64) wf_shadow(WfBufs B, DevScene S, TH_GpuRenderParams P,
                                                 uint32_t per_quad) {
   extern __shared__ uint2 lds_stack[];
-  uint32_t n = B.cnt[2];
+  uint32_t n = B.cnt[WF_C(2)];
   uint32_t lanes_per = per_quad ? 4u : 1u;
   Stack st{lds_stack + (per_quad ? (threadIdx.x >> 2) : threadIdx.x),
            (int)(per_quad ? (blockDim.x >> 2) : blockDim.x), 0};
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(64) wf_finish(WfBufs B, WfArgs A, TH_GpuRender
                                                 TH_GpuCamera cam, uint32_t live_idx) {
   const uint32_t* q = live_idx ? B.q1 : B.q0;
   uint32_t* qn = live_idx ? B.q0 : B.q1;
-  uint32_t n = B.cnt[live_idx];
+  uint32_t n = B.cnt[WF_C(live_idx)];
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; ; i += stride) {
     bool act = i < n;
@@ -354,7 +355,7 @@ __global__ void __launch_bounds__(64) wf_finish(WfBufs B, WfArgs A, TH_GpuRender
         live = true;
       }
     }
-    wf_push(live, k, qn, &B.cnt[live_idx ^ 1u]);
+    wf_push(live, k, qn, &B.cnt[WF_C(live_idx ^ 1u)]);
   }
 }
 
@@ -425,7 +426,7 @@ static int32_t wf_reserve(th_scene* s, uint32_t n) {
       (rc = wf_alloc(wf, (void**)&b.q0, (size_t)n * sizeof(uint32_t))) ||
       (rc = wf_alloc(wf, (void**)&b.q1, (size_t)n * sizeof(uint32_t))) ||
       (rc = wf_alloc(wf, (void**)&b.qs, 2 * (size_t)n * sizeof(uint32_t))) ||
-      (rc = wf_alloc(wf, (void**)&b.cnt, 16 * sizeof(uint32_t)))) {
+      (rc = wf_alloc(wf, (void**)&b.cnt, WF_C(16) * sizeof(uint32_t)))) {
     th_wavefront_free(wf);
     return rc;
   }
