@@ -12,6 +12,17 @@
  *   estimator    shaders/pathtrace_spheres.wgsl:189-937 ("pcg" mode: per-pixel PCG
  *                stream, draw order and formulas of the reference GPU kernel)
  *   display      crates/toaster-core/src/color.rs:23-75, toaster-gpu/src/image_output.rs:59-92
+ *
+ * Pinning (tests/test_oracle_known_answers.py, tests/test_reference_scenes.py): the
+ * reference's own known-answer tests for intersection, AABB, BVH, lights, environment,
+ * integrator, camera, packing and display, plus closed forms (BVH == brute force, env pdf
+ * integrates to 1, white furnace, transcendental ulp bounds).
+ * Parity unpinned:
+ *   pcg_hash constants (wgsl:186-199) -- no reference test holds a hash value; only the
+ *     bijection property is checked.
+ *   or_render_ex rng_mode 1 (per-sample streams) -- this project's extension, no
+ *     reference counterpart; it shares every formula with rng_mode 0.
+ *   toaster-cpu's StdRng (rand 0.9.4 / rand_chacha 0.9.0) stream -- not restated.
  */
 #ifndef TOASTER_ORACLE_H
 #define TOASTER_ORACLE_H
