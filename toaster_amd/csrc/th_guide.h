@@ -109,6 +109,7 @@ struct Mixture {
   v3 mu[TG_LOBES];
   float kappa[TG_LOBES];
   float w[TG_LOBES];
+  float wn[TG_LOBES];  // w_k times the lobe's normaliser kappa / (2 pi (1 - e^{-2 kappa}))
 };
 
 TH_HD float softplus(float x) {
@@ -138,7 +139,11 @@ TH_HD void decode(const float* o, Mixture* m) {
     m->w[k] = fexp(o[32 + k] - mx);
     sum += m->w[k];
   }
-  for (int k = 0; k < TG_LOBES; ++k) m->w[k] = m->w[k] / sum;
+  const float INV_2PI = 0.15915494f;
+  for (int k = 0; k < TG_LOBES; ++k) {
+    m->w[k] = m->w[k] / sum;
+    m->wn[k] = m->w[k] * (m->kappa[k] * INV_2PI / (1.0f - fexp(-2.0f * m->kappa[k])));
+  }
 }
 
 TH_HD float vmf_pdf(v3 mu, float kappa, v3 w) {
@@ -149,7 +154,7 @@ TH_HD float vmf_pdf(v3 mu, float kappa, v3 w) {
 
 TH_HD float mixture_pdf(const Mixture& m, v3 w) {
   float p = 0.0f;
-  for (int k = 0; k < TG_LOBES; ++k) p += m.w[k] * vmf_pdf(m.mu[k], m.kappa[k], w);
+  for (int k = 0; k < TG_LOBES; ++k) p += m.wn[k] * fexp(m.kappa[k] * (th::dot(m.mu[k], w) - 1.0f));
   return p;
 }
 

@@ -427,7 +427,19 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       beta = throughput;
       uint32_t rng = B.rng[k];
       tg::Mixture mix;
-      if (use_guide) tg::decode(B.O + (size_t)i * TG_OUT, &mix);
+      if (use_guide) {
+        float o[TG_OUT];
+        const float4* o4 = (const float4*)(B.O + (size_t)i * TG_OUT);  // 160-byte rows
+#pragma unroll
+        for (int j = 0; j < TG_OUT / 4; ++j) {
+          const float4 v = o4[j];
+          o[4 * j] = v.x;
+          o[4 * j + 1] = v.y;
+          o[4 * j + 2] = v.z;
+          o[4 * j + 3] = v.w;
+        }
+        tg::decode(o, &mix);
+      }
       float pdf;
       bool cont = shade_diffuse(B, it, P, k, p, nrm, albedo, use_guide ? &mix : nullptr, A.alpha,
                                 A.env_nee != 0u, throughput, rng, nd, pdf, sh_light, sh_env);
