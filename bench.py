@@ -97,7 +97,8 @@ def reference_suite(warmup=5, runs=21):
         sc = load_scene(path)
         gs = GpuScene.from_scene(ctx, sc)
         row = {}
-        for mode, opts in (("reference_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE)),
+        for mode, opts in (("reference_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
+                                                           schedule=A.TH_SCHEDULE_OCTET)),
                            ("per_sample_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
                                                             rng_mode=A.TH_RNG_PER_SAMPLE))):
             times, kms, rb = [], [], []
@@ -222,6 +223,8 @@ def main():
     ap.add_argument("--triangles", type=int, default=65536)
     ap.add_argument("--traversal", choices=["reference", "ordered", "wide"], default="wide")
     ap.add_argument("--rng", choices=["reference", "per_sample"], default="reference")
+    ap.add_argument("--schedule", choices=["static", "dynamic", "octet"], default="octet",
+                    help="pixel-to-lane assignment of the wide kernel with the reference RNG")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true")
     ap.add_argument("--no-suite", action="store_true")
@@ -237,7 +240,16 @@ def main():
     trav = {"reference": A.TH_TRAVERSE_REFERENCE, "ordered": A.TH_TRAVERSE_ORDERED,
             "wide": A.TH_TRAVERSE_WIDE}[args.traversal]
     rng = A.TH_RNG_REFERENCE if args.rng == "reference" else A.TH_RNG_PER_SAMPLE
-    opts = RenderOptions(rng_mode=rng, traversal=trav)
+    sched = {"static": A.TH_SCHEDULE_STATIC, "dynamic": A.TH_SCHEDULE_DYNAMIC,
+             "octet": A.TH_SCHEDULE_OCTET}[args.schedule]
+    opts = RenderOptions(rng_mode=rng, traversal=trav, schedule=sched)
+    if args.traversal == "wide" and args.rng == "reference":
+        kernel_name = {"static": "k_render_quad", "dynamic": "k_render_quad_dyn",
+                       "octet": "k_render_octet"}[args.schedule]
+    elif args.traversal == "wide":
+        kernel_name = "k_render_quad"
+    else:
+        kernel_name = "k_render_pixel" if args.rng == "reference" else "k_render_sample"
     ctx = GpuContext(0)
     gs = GpuScene.from_scene(ctx, sc)
     # one counting pass (outside the timed region) for rays per frame; rays do not
@@ -261,6 +273,8 @@ def main():
     for name, o in (("megakernel_reference_traversal", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_REFERENCE)),
                     ("megakernel_ordered", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_ORDERED)),
                     ("quad_wide", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE)),
+                    ("octet_wide", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE,
+                                                 schedule=A.TH_SCHEDULE_OCTET)),
                     ("quad_wide_dynamic_schedule", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_WIDE,
                                                                  schedule=A.TH_SCHEDULE_DYNAMIC)),
                     ("quad_wide_per_sample_rng", RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
@@ -307,7 +321,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes_per_launch": bytes_launch,
-                    "kernel": "k_render_quad" if args.traversal == "wide" else "k_render_pixel",
+                    "kernel": kernel_name,
                     "kernel_ms": round(kernel_ms, 5)}
     line = {
         "metric": "Mpaths/s + equal-time relRMSE: guided vs plain HIP tracer, 1 MI355X",
@@ -319,8 +333,8 @@ def main():
                                "1 emissive sphere, 320x180, 4 spp, 4 bounces, median-split BVH",
                    "triangles": args.triangles, "width": sc.width, "height": sc.height,
                    "spp": sc.samples, "max_bounces": sc.max_bounces, "paths_per_step": paths,
-                   "rng": args.rng, "traversal": args.traversal,
-                   "kernel": "quad" if args.traversal == "wide" else "megakernel"},
+                   "rng": args.rng, "traversal": args.traversal, "schedule": args.schedule,
+                   "kernel": kernel_name},
         "rays_per_s": round(args.steps * rays_per_frame / dt, 1),
         "kernel_ms_mean": round(kernel_ms, 5),
         "roofline": roofline,

@@ -261,9 +261,12 @@ typedef struct th_render_options {
 
 /* Pixel-to-wave assignment of the 4-lane kernel with the reference RNG.
  * TH_SCHEDULE_STATIC: one 4x4 pixel tile per wave.
- * TH_SCHEDULE_DYNAMIC: persistent quads take the next pixel from a counter when done. */
+ * TH_SCHEDULE_DYNAMIC: persistent quads take the next pixel from a counter when done.
+ * TH_SCHEDULE_OCTET: one 4x2 pixel tile per wave, eight lanes per pixel: one quad traces
+ *   the path while the other traces the shadow rays of the path's previous vertex. */
 #define TH_SCHEDULE_STATIC 0
 #define TH_SCHEDULE_DYNAMIC 1
+#define TH_SCHEDULE_OCTET 2
 
 typedef struct th_stats {
     double last_kernel_ms;    /* device time of the last th_render (hipEvents)    */

@@ -52,20 +52,50 @@ def test_image_matches_oracle(gpu_ctx, name, make, traversal):
     assert frac >= TOL_FRAC
 
 
+@pytest.mark.parametrize("schedule", [A.TH_SCHEDULE_DYNAMIC, A.TH_SCHEDULE_OCTET])
 @pytest.mark.parametrize("name,make", [
     ("cornell", lambda: scenes.cornell()),
     ("stress2048", lambda: scenes.bvh_stress(2048)),
+    ("mixed_env", lambda: scenes.mixed_environment(width=160, height=90, samples=4, max_bounces=8,
+                                                   env_size=(256, 128))),
 ])
-def test_dynamic_schedule_matches_oracle(gpu_ctx, name, make):
-    """The 4-lane kernel with persistent quads taking pixels from a counter (the default
-    is one static tile per wave): same per-pixel work, same image."""
+def test_schedule_matches_oracle(gpu_ctx, name, make, schedule):
+    """The 4-lane kernel with other pixel-to-lane assignments (the default is one static
+    tile per wave): persistent quads taking pixels from a counter, or eight lanes per
+    pixel with the shadow rays traced beside the path.  Same per-pixel work, same image."""
     sc = make()
-    print(f"case {name} wide dynamic schedule", flush=True)
-    img = _render(gpu_ctx, sc, RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_DYNAMIC))
+    print(f"case {name} wide schedule={schedule}", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=schedule))
     ref, _ = oracle.OracleScene(sc).render(threads=8)
     frac, exact = _compare(img, ref)
-    print(f"{name} dynamic: within tol {frac:.5f} bit-exact {exact:.5f}")
+    print(f"{name} schedule {schedule}: within tol {frac:.5f} bit-exact {exact:.5f}")
     assert frac >= TOL_FRAC
+
+
+@pytest.mark.parametrize("name,make", [
+    ("cornell_1spp", lambda: scenes.cornell(samples=1)),
+    ("cornell_0_bounces", lambda: scenes.cornell(max_bounces=0)),
+    ("cornell_1_bounce", lambda: scenes.cornell(max_bounces=1)),
+    ("stress8192_odd_size", lambda: scenes.bvh_stress(8192, width=123, height=77, samples=3)),
+    ("mixed_env", lambda: scenes.mixed_environment(width=160, height=90, samples=5, max_bounces=8,
+                                                   env_size=(256, 128))),
+])
+def test_octet_is_bit_identical_to_quad(gpu_ctx, name, make):
+    """The octet kernel traces a vertex's shadow rays beside the next bounce (and a
+    sample's last shadow rays beside the next sample's primary ray) but adds every term in
+    the reference's order: its image equals the 4-lane kernel's bit for bit, also when it
+    folds into an accumulated image."""
+    sc = make()
+    print(f"case octet vs quad {name}", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    out = []
+    for sched in (A.TH_SCHEDULE_STATIC, A.TH_SCHEDULE_OCTET):
+        o = RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=sched)
+        gs.render(options=o, frame_index=0, accumulated_samples=0)
+        gs.render(options=o, frame_index=1, accumulated_samples=sc.samples)
+        out.append(gs.readback().copy())
+    gs.close()
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
 
 
 @pytest.mark.parametrize("traversal", [A.TH_TRAVERSE_ORDERED, A.TH_TRAVERSE_WIDE])

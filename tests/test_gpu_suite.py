@@ -11,13 +11,16 @@ from toaster_amd.loader import load_scene, suite_paths
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("traversal", [A.TH_TRAVERSE_REFERENCE, A.TH_TRAVERSE_WIDE])
+@pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_REFERENCE, A.TH_SCHEDULE_STATIC),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_STATIC),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET)])
 @pytest.mark.parametrize("index", range(7))
-def test_suite_scene_matches_oracle(gpu_ctx, index, traversal):
+def test_suite_scene_matches_oracle(gpu_ctx, index, traversal, schedule):
     sc = load_scene(suite_paths()[index])
-    print(f"case suite {sc.name} traversal={traversal}", flush=True)
+    print(f"case suite {sc.name} traversal={traversal} schedule={schedule}", flush=True)
     gs = GpuScene.from_scene(gpu_ctx, sc)
-    gs.render(options=RenderOptions(traversal=traversal), frame_index=0, accumulated_samples=0)
+    gs.render(options=RenderOptions(traversal=traversal, schedule=schedule), frame_index=0,
+              accumulated_samples=0)
     img = gs.readback()
     gs.close()
     ref, _ = oracle.OracleScene(sc).render(threads=8)

@@ -102,6 +102,25 @@ struct Integrator {
     return t < limit;
   }
 
+  // One traversal step of an octet (two quads of one path, octet kernels): the quad with
+  // shadow == false looks for the closest hit along (o, d) when `ext`; the other quad
+  // tests (so, sdir) for an occluder before `slim` when `shd`.  Both run in the same
+  // loop, so the two rays cost one traversal's latency.  Afterwards both quads hold the
+  // closest hit (*t, *slot) and the occlusion result.
+  __device__ bool octet_step(bool shadow, bool ext, const v3& o, const v3& d, bool shd, const v3& so,
+                             const v3& sdir, float slim, float* t, uint32_t* slot) {
+    float tt;
+    uint32_t ss;
+    bool r = traverse_quad_rt(S, (const Child4*)S.nodes4, S.root4, shadow ? shd : ext, shadow,
+                              shadow ? so : o, shadow ? sdir : d, shadow ? slim : TH_FLT_MAX, st,
+                              &tt, &ss, lds_nodes, cached);
+    float xt = oswap_f(tt);
+    uint32_t xs = oswap_u(ss), xr = oswap_u((uint32_t)r);
+    *t = shadow ? xt : tt;
+    *slot = shadow ? xs : ss;
+    return shadow ? r : (xr != 0u);
+  }
+
   __device__ __forceinline__ float rnd(uint32_t* s) { return th::rand01(s); }
 
   __device__ v3 random_unit_vector(uint32_t* s) {  // wgsl:202-207
@@ -407,6 +426,152 @@ struct Integrator {
       d = nd;
     }
     return radiance;
+  }
+};
+
+// ray_color (wgsl:805-916) for an octet of lanes.  The shadow rays of a diffuse vertex
+// are traced by the partner quad while the path's quad traces the next bounce, and their
+// terms are added when both are back, before anything the next vertex adds.  The
+// continuation ray does not depend on the shadow rays' outcome, so the draws, terms and
+// additions are those of ray_color in the same order and the result is identical.  The
+// shadow rays of a sample's last vertex go with the next sample's primary ray; a sample's
+// radiance is added to the pixel sum once it is complete, in sample order.
+// ENV false: scenes whose background is not an environment map (no environment shadow
+// rays).
+template <bool ENV, int TMODE, bool COUNT>
+struct OctetPath {
+  Integrator<TMODE, COUNT>& it;
+  bool shadow;  // this lane's quad traces shadow rays
+  // the last diffuse vertex: its throughput, shadow origin, light and environment
+  // shadow rays (pl / pe: ray pending) and their unoccluded terms
+  bool pd = false, pl = false, pe = false;
+  v3 pthr, so, ldir, edir, lterm, eterm;
+  float llim = 0.0f;
+  // a finished sample's radiance waiting for its last vertex's shadow rays
+  bool carry = false;
+  v3 crad;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;  // pixel sums
+
+  __device__ void emit(const v3& L) {
+    cr = cr + L.x;
+    cg = cg + L.y;
+    cb = cb + L.z;
+    ca = ca + 1.0f;
+  }
+
+  // One sample from the camera ray (o, d); `last`: no sample follows in this launch.
+  __device__ void sample(v3 o, v3 d, uint32_t* s, bool last) {
+    const DevScene& S = it.S;
+    const TH_GpuRenderParams& P = it.P;
+    const v3 zero = th::mk(0, 0, 0);
+    v3 throughput = th::mk(1, 1, 1);
+    v3 radiance = zero;
+    bool include_emissive = true, prev_diffuse = false;
+    float prev_pdf = 0.0f;
+    for (uint32_t i = 0;; ++i) {
+      bool ext = i < P.max_bounces;
+      if (!ext && !pd) break;
+      if (!ext && !last) {  // trace this vertex's shadow rays with the next primary ray
+        carry = true;
+        crad = radiance;
+        return;
+      }
+      float t;
+      uint32_t slot;
+      bool occ_l = it.octet_step(shadow, ext, o, d, pl, so, ldir, llim, &t, &slot);
+      bool occ_e = false;
+      if (ENV && pe) {
+        float t2;
+        uint32_t s2;
+        occ_e = it.octet_step(shadow, false, o, d, true, so, edir, 10000.001f, &t2, &s2);
+      }
+      if (pd) {
+        v3 r = carry ? crad : radiance;
+        r = th::add(r, th::mul(pthr, occ_l ? zero : lterm));
+        r = th::add(r, th::mul(pthr, (ENV && !occ_e) ? eterm : zero));
+        if (carry) emit(r);
+        else radiance = r;
+        pd = pl = pe = carry = false;
+      }
+      if (!ext) break;
+      if (slot == THD_NO_SLOT || t > TH_MAX_DISTANCE) {
+        if (P.background_kind == 0u) {
+          float lt = (th::normalize(d).y + 1.0f) * 0.5f;
+          radiance = th::add(radiance, th::mul(throughput, th::mix(th::mk(1.0f, 1.0f, 1.0f),
+                                                                    th::mk(0.35f, 0.65f, 1.0f), lt)));
+        } else if (ENV) {
+          float mis = 1.0f;
+          if (prev_diffuse) mis = it.power_heuristic(prev_pdf, it.env_pdf(d));
+          radiance = th::add(radiance, th::muls(th::mul(throughput, it.env_lookup(d)), mis));
+        }
+        break;
+      }
+      Hit rec;
+      finalize_slot(S, slot, o, d, t, &rec);
+      const TH_GpuMaterial& m = S.materials[rec.mat];
+      v3 att, nd;
+      bool brk = false;
+      if (m.kind == 0u) {
+        v3 albedo = it.base_color(m, rec);
+        pd = true;
+        pthr = throughput;
+        so = th::add(rec.p, th::muls(rec.n, TH_MIN_DISTANCE));
+        lterm = zero;
+        pl = it.light_candidate(rec.p, rec.n, albedo, s, &so, &ldir, &llim, &lterm);
+        eterm = zero;
+        if (ENV) {
+          EnvSample es;
+          float sc;
+          pe = it.env_candidate(rec.n, s, &es, &sc);
+          if (pe) {
+            edir = es.dir;
+            eterm = it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, sc / TH_PI));
+          }
+        }
+        v3 dir = th::add(rec.n, it.random_unit_vector(s));
+        if (th::dot(dir, dir) < TH_EPSILON) dir = rec.n;
+        nd = th::normalize(dir);
+        att = albedo;
+        include_emissive = false;
+        prev_diffuse = true;
+        prev_pdf = th::fmax_(0.0f, th::dot(rec.n, nd)) / TH_PI;
+      } else if (m.kind == 1u) {
+        v3 refl = it.reflect_vec(d, rec.n);
+        float rough = th::clampf(m.params[0], 0.0f, 1.0f);
+        v3 dir = th::add(refl, th::smul(rough, it.random_unit_vector(s)));
+        if (th::dot(dir, rec.n) <= 0.0f) brk = true;
+        nd = th::normalize(dir);
+        att = th::mk(m.albedo[0], m.albedo[1], m.albedo[2]);
+        include_emissive = true;
+        prev_diffuse = false;
+      } else if (m.kind == 2u) {
+        float ratio = rec.front ? 1.0f / m.params[1] : m.params[1];
+        v3 ud = th::normalize(d);
+        float ct = th::fmin_(1.0f, th::dot(th::neg(ud), rec.n));
+        float stt = th::fsqrt(1.0f - ct * ct);
+        bool cannot = (ratio * stt) > 1.0f;
+        v3 dir;
+        if (cannot || it.reflectance(ct, ratio) > it.rnd(s))
+          dir = it.reflect_vec(ud, rec.n);
+        else
+          dir = it.refract_vec(ud, rec.n, ratio);
+        nd = th::normalize(dir);
+        att = th::mk(1, 1, 1);
+        include_emissive = true;
+        prev_diffuse = false;
+      } else {
+        if (m.kind == 3u && include_emissive)
+          radiance = th::add(radiance, th::muls(th::mul(throughput, th::mk(m.albedo[0], m.albedo[1],
+                                                                            m.albedo[2])),
+                                                m.params[2]));
+        brk = true;
+      }
+      if (brk) break;
+      throughput = th::mul(throughput, att);
+      o = rec.p;
+      d = nd;
+    }
+    emit(radiance);
   }
 };
 

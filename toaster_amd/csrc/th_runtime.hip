@@ -272,6 +272,55 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   }
 }
 
+// Reference RNG, eight lanes per pixel: lanes 8k..8k+3 trace the path, lanes 8k+4..8k+7
+// the shadow rays of its previous vertex at the same time (ray_color_octet).  A wave
+// covers a 4x2 pixel tile; the samples of a pixel run in stream order.
+template <bool ENV>
+__global__ void __launch_bounds__(256) k_render_octet(DevScene S, TH_GpuRenderParams P,
+                                                      TH_GpuCamera cam, float4* __restrict__ accum,
+                                                      const float* __restrict__ srgb_lut,
+                                                      uint32_t cached) {
+  extern __shared__ uint2 lds_raw[];
+  Child4* lds_nodes = (Child4*)lds_raw;
+  for (uint32_t i = threadIdx.x; i < 4u * cached; i += blockDim.x)
+    lds_nodes[i] = ((const Child4*)S.nodes4)[i];
+  __syncthreads();
+  uint2* lds_stack = lds_raw + 4u * cached * (sizeof(Child4) / sizeof(uint2));
+  const uint32_t lane = threadIdx.x & 63u, pw = lane >> 3;
+  const bool shadow = ((lane >> 2) & 1u) != 0u;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t tiles_x = (P.width + 3u) / 4u;
+  const uint32_t x = (wave % tiles_x) * 4u + (pw & 3u);
+  const uint32_t y = (wave / tiles_x) * 2u + (pw >> 2);
+  const bool active = x < P.width && y < P.height;
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  LaneCounters<false> lc;
+  Integrator<3, false> it{S, P, srgb_lut, st, lc, lds_nodes, cached};
+  if (!active) return;
+  const uint32_t idx = y * P.width + x;
+  OctetPath<ENV, 3, false> path{it, shadow};
+  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  for (uint32_t smp = 0; smp < P.samples; ++smp) {
+    v3 o, d;
+    camera_ray(cam, P, x, y, &s, &o, &d);
+    path.sample(o, d, &s, smp + 1u == P.samples);
+  }
+  const float cr = path.cr, cg = path.cg, cb = path.cb, ca = path.ca;
+  if ((lane & 7u) == 0u) {
+    float ns = (float)P.samples;
+    float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
+    if (P.accumulated_samples == 0u) {
+      accum[idx] = b;
+    } else {
+      float prev = (float)P.accumulated_samples;
+      float tot = prev + ns;
+      float4 o = accum[idx];
+      accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                               (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+    }
+  }
+}
+
 // Reference RNG with dynamic pixel assignment.  A pixel's samples share one random
 // stream, so its samples run in sequence and a statically assigned wave lasts as long as
 // its slowest pixel.  Here every quad is persistent: when its pixel is done it takes the
@@ -851,7 +900,7 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   th_render_options o;
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
-  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 1u)
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 2u)
     return th_set_error(TH_E_INVALID, "unknown render option");
   if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
     return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
@@ -897,6 +946,14 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_work, 1u);
       hipLaunchKernelGGL(k_render_quad_dyn, dim3(std::max(pblocks, 1u)), blk, lds4, ctx->stream, S, *p, *cam,
                          s->d_accum, ctx->d_srgb_lut, cached, ctx->d_work);
+    } else if (o.rng_mode == TH_RNG_REFERENCE && o.schedule == TH_SCHEDULE_OCTET) {
+      uint32_t owaves = ((p->width + 3u) / 4u) * ((p->height + 1u) / 2u);
+      if (p->background_kind == 2u)
+        hipLaunchKernelGGL(k_render_octet<true>, dim3((owaves + 3u) / 4u), blk, lds4, ctx->stream, S, *p,
+                           *cam, s->d_accum, ctx->d_srgb_lut, cached);
+      else
+        hipLaunchKernelGGL(k_render_octet<false>, dim3((owaves + 3u) / 4u), blk, lds4, ctx->stream, S, *p,
+                           *cam, s->d_accum, ctx->d_srgb_lut, cached);
     } else if (o.rng_mode == TH_RNG_REFERENCE) {
       TH_LAUNCHQ(false, 1);
     } else {

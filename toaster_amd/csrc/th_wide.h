@@ -36,6 +36,16 @@ __device__ __forceinline__ float qbcast_f(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
 }
 
+// Value held by the other quad of this lane's octet (lanes 8k..8k+7; DPP row_half_mirror,
+// lane 8k+i reads lane 8k+7-i).  Meaningful where each quad holds one value in its four
+// lanes and all eight lanes are active.
+__device__ __forceinline__ uint32_t oswap_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float oswap_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
+}
+
 // ------------------------------------------------------------------ ray + box
 struct RayPre {
   v3 o, d, inv;
@@ -96,26 +106,26 @@ __device__ __forceinline__ bool box_bf(float mnx, float mny, float mnz, float mx
 }
 
 // ------------------------------------------------------------------ traversal
-// MODE 1: closest hit (reference tie-breaking).  MODE 2: any hit with t < limit.
-// Every value that steers control flow is identical in the four lanes of the quad.
-// The first `cached` 4-wide nodes (breadth-first order: the top levels every ray
-// visits) may be served from a copy in LDS.
-template <int MODE>
-__device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* __restrict__ nodes4,
-                                              uint32_t root4, const v3& o, const v3& d, float limit,
-                                              Stack& st, float* best_t, uint32_t* best_slot,
-                                              const Child4* lds_nodes = nullptr,
-                                              uint32_t cached = 0) {
+// anyhit false: closest hit (reference tie-breaking).  anyhit true: any hit with
+// t < limit.  Every value that steers control flow is identical in the four lanes of the
+// quad; different quads of a wave may run different modes in the same loop.  A quad
+// with `active` false returns at once (no hit).  The first `cached` 4-wide nodes
+// (breadth-first order: the top levels every ray visits) may be served from a copy in LDS.
+__device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4* __restrict__ nodes4,
+                                                 uint32_t root4, bool active, bool anyhit,
+                                                 const v3& o, const v3& d, float limit, Stack& st,
+                                                 float* best_t, uint32_t* best_slot,
+                                                 const Child4* lds_nodes, uint32_t cached) {
   const uint32_t lq = threadIdx.x & 3u;
   const float INF = __builtin_inff();
-  float best = (MODE == 2) ? limit : TH_FLT_MAX;
+  float best = anyhit ? limit : TH_FLT_MAX;
   uint32_t bslot = THD_NO_SLOT, bleaf = 0;
   RayPre r = make_ray(o, d);
   float te;
   *best_t = TH_FLT_MAX;
   *best_slot = THD_NO_SLOT;
-  if (!box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1],
-              S.root_max[2], r, best, &te))
+  if (!active || !box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0],
+                         S.root_max[1], S.root_max[2], r, best, &te))
     return false;
   st.sp = 0;
   uint32_t e = root4;
@@ -130,7 +140,7 @@ __device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* _
           uint32_t tag;
           t = slot_t(S, first + k, o, d, &tag);
         }
-        if (MODE == 2) {
+        if (anyhit) {
           bool h = t < limit;
           uint32_t any = qbcast_u<0>(h) | qbcast_u<1>(h) | qbcast_u<2>(h) | qbcast_u<3>(h);
           if (any) {
@@ -196,10 +206,21 @@ __device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* _
     }
     if (!found) break;
   }
-  if (MODE == 2) return false;
+  if (anyhit) return false;
   *best_t = best;
   *best_slot = bslot;
   return bslot != THD_NO_SLOT;
+}
+
+// MODE 1: closest hit.  MODE 2: any hit with t < limit.
+template <int MODE>
+__device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* __restrict__ nodes4,
+                                              uint32_t root4, const v3& o, const v3& d, float limit,
+                                              Stack& st, float* best_t, uint32_t* best_slot,
+                                              const Child4* lds_nodes = nullptr,
+                                              uint32_t cached = 0) {
+  return traverse_quad_rt(S, nodes4, root4, true, MODE == 2, o, d, limit, st, best_t, best_slot,
+                          lds_nodes, cached);
 }
 
 }  // namespace thd
