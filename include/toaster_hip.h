@@ -259,11 +259,16 @@ typedef struct th_render_options {
     uint32_t reserved[2];
 } th_render_options;
 
-/* Pixel-to-wave assignment of the 4-lane kernel with the reference RNG.
- * TH_SCHEDULE_STATIC: one 4x4 pixel tile per wave.
- * TH_SCHEDULE_DYNAMIC: persistent quads take the next pixel from a counter when done.
- * TH_SCHEDULE_OCTET: one 4x2 pixel tile per wave, eight lanes per pixel: one quad traces
- *   the path while the other traces the shadow rays of the path's previous vertex. */
+/* Pixel-to-lane assignment of the 4-lane (TH_TRAVERSE_WIDE) megakernel.
+ * TH_SCHEDULE_STATIC: a quad of lanes per path, one 4x4 pixel tile per wave (reference
+ *   RNG), or G quads per pixel (per-sample RNG).
+ * TH_SCHEDULE_DYNAMIC: reference RNG only; persistent quads take the next pixel from a
+ *   counter when done (per-sample RNG: as STATIC).
+ * TH_SCHEDULE_OCTET: eight lanes per path: one quad traces the path while the other
+ *   traces the shadow rays of the path's previous vertex; one 4x2 pixel tile per wave
+ *   (reference RNG), or G = min(samples, 8) octets per pixel (per-sample RNG).
+ * All schedules give the same image for the same RNG mode (per-sample RNG: the same
+ * group count G, i.e. samples < 16). */
 #define TH_SCHEDULE_STATIC 0
 #define TH_SCHEDULE_DYNAMIC 1
 #define TH_SCHEDULE_OCTET 2

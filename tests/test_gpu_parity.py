@@ -98,14 +98,18 @@ def test_octet_is_bit_identical_to_quad(gpu_ctx, name, make):
     assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("traversal", [A.TH_TRAVERSE_ORDERED, A.TH_TRAVERSE_WIDE])
+@pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_ORDERED, A.TH_SCHEDULE_STATIC),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_STATIC),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET)])
 @pytest.mark.parametrize("samples", [1, 8, 16])
-def test_per_sample_rng_matches_oracle(gpu_ctx, traversal, samples):
+def test_per_sample_rng_matches_oracle(gpu_ctx, traversal, schedule, samples):
     sc = scenes.bvh_stress(2048, samples=samples)
-    print(f"case per-sample traversal={traversal} spp={samples}", flush=True)
-    img = _render(gpu_ctx, sc, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=traversal))
-    # lanes/quads per pixel used by the kernel: megakernel min(spp, 8), quad min(spp, 16)
-    cap = 16 if traversal == A.TH_TRAVERSE_WIDE else 8
+    print(f"case per-sample traversal={traversal} schedule={schedule} spp={samples}", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=traversal,
+                                             schedule=schedule))
+    # lanes/quads/octets per pixel used by the kernel: megakernel min(spp, 8), quad
+    # min(spp, 16), octet min(spp, 8)
+    cap = 16 if (traversal == A.TH_TRAVERSE_WIDE and schedule != A.TH_SCHEDULE_OCTET) else 8
     group = 1
     while group * 2 <= min(samples, cap):
         group *= 2

@@ -476,11 +476,15 @@ struct OctetPath {
         crad = radiance;
         return;
       }
+      // the shadow quad takes the light's shadow ray, else the environment's; a second
+      // step traces the environment's when both are pending
       float t;
       uint32_t slot;
-      bool occ_l = it.octet_step(shadow, ext, o, d, pl, so, ldir, llim, &t, &slot);
-      bool occ_e = false;
-      if (ENV && pe) {
+      bool first_env = ENV && !pl;
+      bool occ1 = it.octet_step(shadow, ext, o, d, pl || (ENV && pe), so, first_env ? edir : ldir,
+                                first_env ? 10000.001f : llim, &t, &slot);
+      bool occ_l = !first_env && occ1, occ_e = first_env && occ1;
+      if (ENV && pl && pe) {
         float t2;
         uint32_t s2;
         occ_e = it.octet_step(shadow, false, o, d, true, so, edir, 10000.001f, &t2, &s2);

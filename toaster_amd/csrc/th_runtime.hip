@@ -272,10 +272,12 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   }
 }
 
-// Reference RNG, eight lanes per pixel: lanes 8k..8k+3 trace the path, lanes 8k+4..8k+7
-// the shadow rays of its previous vertex at the same time (ray_color_octet).  A wave
-// covers a 4x2 pixel tile; the samples of a pixel run in stream order.
-template <bool ENV>
+// Eight lanes per path: lanes 8k..8k+3 trace the path, lanes 8k+4..8k+7 the shadow rays
+// of its previous vertex at the same time (OctetPath).  Reference RNG (G = 1): the octet
+// runs the pixel's samples in stream order.  Per-sample RNG: G octets share a pixel
+// (samples j, j+G, ... per octet), combined by the same butterfly as k_render_quad.  A
+// wave covers 8/G pixels.
+template <bool ENV, bool PER_SAMPLE, int G>
 __global__ void __launch_bounds__(256) k_render_octet(DevScene S, TH_GpuRenderParams P,
                                                       TH_GpuCamera cam, float4* __restrict__ accum,
                                                       const float* __restrict__ srgb_lut,
@@ -286,27 +288,41 @@ __global__ void __launch_bounds__(256) k_render_octet(DevScene S, TH_GpuRenderPa
     lds_nodes[i] = ((const Child4*)S.nodes4)[i];
   __syncthreads();
   uint2* lds_stack = lds_raw + 4u * cached * (sizeof(Child4) / sizeof(uint2));
-  const uint32_t lane = threadIdx.x & 63u, pw = lane >> 3;
+  constexpr uint32_t PPW = 8u / G;
+  constexpr uint32_t TW = PPW >= 4u ? 4u : (PPW >= 2u ? 2u : 1u);
+  constexpr uint32_t TH = PPW / TW;
+  const uint32_t lane = threadIdx.x & 63u, ow = lane >> 3;
   const bool shadow = ((lane >> 2) & 1u) != 0u;
   const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t tiles_x = (P.width + 3u) / 4u;
-  const uint32_t x = (wave % tiles_x) * 4u + (pw & 3u);
-  const uint32_t y = (wave / tiles_x) * 2u + (pw >> 2);
+  const uint32_t pw = ow / G, j = ow % G;
+  const uint32_t tiles_x = (P.width + TW - 1u) / TW;
+  const uint32_t x = (wave % tiles_x) * TW + pw % TW;
+  const uint32_t y = (wave / tiles_x) * TH + pw / TW;
   const bool active = x < P.width && y < P.height;
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
   LaneCounters<false> lc;
   Integrator<3, false> it{S, P, srgb_lut, st, lc, lds_nodes, cached};
-  if (!active) return;
   const uint32_t idx = y * P.width + x;
   OctetPath<ENV, 3, false> path{it, shadow};
-  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
-  for (uint32_t smp = 0; smp < P.samples; ++smp) {
-    v3 o, d;
-    camera_ray(cam, P, x, y, &s, &o, &d);
-    path.sample(o, d, &s, smp + 1u == P.samples);
+  if (active) {
+    uint32_t base = idx ^ (P.frame_index * 9781u);
+    uint32_t s = th::pcg_hash(base);
+    for (uint32_t smp = PER_SAMPLE ? j : 0u; smp < P.samples; smp += PER_SAMPLE ? G : 1u) {
+      if (PER_SAMPLE) s = th::pcg_hash(base + smp * 0x9E3779B9u);
+      v3 o, d;
+      camera_ray(cam, P, x, y, &s, &o, &d);
+      path.sample(o, d, &s, smp + (PER_SAMPLE ? G : 1u) >= P.samples);
+    }
   }
-  const float cr = path.cr, cg = path.cg, cb = path.cb, ca = path.ca;
-  if ((lane & 7u) == 0u) {
+  float cr = path.cr, cg = path.cg, cb = path.cb, ca = path.ca;
+#pragma unroll
+  for (int off = 8; off < 8 * G; off <<= 1) {
+    cr = cr + __shfl_xor(cr, off, 64);
+    cg = cg + __shfl_xor(cg, off, 64);
+    cb = cb + __shfl_xor(cb, off, 64);
+    ca = ca + __shfl_xor(ca, off, 64);
+  }
+  if (active && (lane % (8u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
     if (P.accumulated_samples == 0u) {
@@ -946,14 +962,33 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_work, 1u);
       hipLaunchKernelGGL(k_render_quad_dyn, dim3(std::max(pblocks, 1u)), blk, lds4, ctx->stream, S, *p, *cam,
                          s->d_accum, ctx->d_srgb_lut, cached, ctx->d_work);
-    } else if (o.rng_mode == TH_RNG_REFERENCE && o.schedule == TH_SCHEDULE_OCTET) {
-      uint32_t owaves = ((p->width + 3u) / 4u) * ((p->height + 1u) / 2u);
-      if (p->background_kind == 2u)
-        hipLaunchKernelGGL(k_render_octet<true>, dim3((owaves + 3u) / 4u), blk, lds4, ctx->stream, S, *p,
-                           *cam, s->d_accum, ctx->d_srgb_lut, cached);
-      else
-        hipLaunchKernelGGL(k_render_octet<false>, dim3((owaves + 3u) / 4u), blk, lds4, ctx->stream, S, *p,
-                           *cam, s->d_accum, ctx->d_srgb_lut, cached);
+    } else if (o.schedule == TH_SCHEDULE_OCTET) {
+      // octets per pixel: 1 with the reference RNG, else min(samples, 8) as a power of two
+      int GO = o.rng_mode == TH_RNG_REFERENCE ? 1 : std::min(G, 8);
+      uint32_t oppw = 8u / (uint32_t)GO;
+      uint32_t otw = oppw >= 4 ? 4 : (oppw >= 2 ? 2 : 1), oth = oppw / otw;
+      uint32_t owaves = ((p->width + otw - 1) / otw) * ((p->height + oth - 1) / oth);
+      dim3 ogrid((owaves + 3u) / 4u);
+      bool env = p->background_kind == 2u;
+#define TH_LAUNCHO(E, PS, GG)                                                                       \
+  hipLaunchKernelGGL((k_render_octet<E, PS, GG>), ogrid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, \
+                     ctx->d_srgb_lut, cached)
+      if (o.rng_mode == TH_RNG_REFERENCE) {
+        if (env) TH_LAUNCHO(true, false, 1);
+        else TH_LAUNCHO(false, false, 1);
+      } else {
+        switch (GO * (env ? -1 : 1)) {
+          case 8: TH_LAUNCHO(false, true, 8); break;
+          case 4: TH_LAUNCHO(false, true, 4); break;
+          case 2: TH_LAUNCHO(false, true, 2); break;
+          case 1: TH_LAUNCHO(false, true, 1); break;
+          case -8: TH_LAUNCHO(true, true, 8); break;
+          case -4: TH_LAUNCHO(true, true, 4); break;
+          case -2: TH_LAUNCHO(true, true, 2); break;
+          default: TH_LAUNCHO(true, true, 1); break;
+        }
+      }
+#undef TH_LAUNCHO
     } else if (o.rng_mode == TH_RNG_REFERENCE) {
       TH_LAUNCHQ(false, 1);
     } else {
