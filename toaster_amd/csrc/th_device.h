@@ -54,6 +54,7 @@ struct DevScene {
   float root_min[3], root_max[3];
   const float4* nodes4;  // 4-wide nodes (th_wide.h), 8 float4 per node
   uint32_t root4;        // 4-wide traversal entry of the root
+  uint32_t node4_bytes;  // size of the nodes4 array
 };
 
 struct Counters {
@@ -62,6 +63,52 @@ struct Counters {
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+
+// Marks three loaded values as used here (an empty asm statement, no instruction): the
+// loads that produce them are issued with the loads before it instead of inside a later
+// branch, so a record costs one memory round trip.
+__device__ __forceinline__ void th_keep3(float x, float y, float z) {
+  __asm__ volatile("" ::"v"(x), "v"(y), "v"(z));
+}
+__device__ __forceinline__ void th_keep4u(const uint4& v) {
+  __asm__ volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+}
+
+// A whole light record (96 B) in one round trip.
+__device__ __forceinline__ TH_GpuLight load_light(const DevScene& S, uint32_t i) {
+  const uint4* p = (const uint4*)(S.lights + i);
+  TH_GpuLight L;
+  uint4 w[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w[k] = p[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) th_keep4u(w[k]);
+  __builtin_memcpy(&L, w, sizeof L);
+  return L;
+}
+
+// A whole material record (48 B) in one round trip.
+__device__ __forceinline__ TH_GpuMaterial load_material(const DevScene& S, uint32_t i) {
+  const uint4* p = (const uint4*)(S.materials + i);
+  uint4 a = p[0], b = p[1], c = p[2];
+  th_keep4u(a);
+  th_keep4u(b);
+  th_keep4u(c);
+  TH_GpuMaterial m;
+  m.kind = a.x;
+  m.texture_offset = a.y;
+  m.texture_width = a.z;
+  m.texture_height = a.w;
+  m.albedo[0] = __uint_as_float(b.x);
+  m.albedo[1] = __uint_as_float(b.y);
+  m.albedo[2] = __uint_as_float(b.z);
+  m.albedo[3] = __uint_as_float(b.w);
+  m.params[0] = __uint_as_float(c.x);
+  m.params[1] = __uint_as_float(c.y);
+  m.params[2] = __uint_as_float(c.z);
+  m.params[3] = __uint_as_float(c.w);
+  return m;
+}
 
 // ------------------------------------------------------------------ box test
 // wgsl:700-730 with t_max supplied; returns the entry distance in *t_enter.
@@ -124,13 +171,14 @@ __device__ __forceinline__ float sphere_t(const v3& o, const v3& d, const v3& c,
 __device__ __forceinline__ float slot_t(const DevScene& S, uint32_t slot, const v3& o, const v3& d,
                                         uint32_t* tag_out) {
   const float4* p = S.slots + 3u * (size_t)slot;
-  float4 a = p[0], b = p[1];
+  float4 a = p[0], b = p[1], c = p[2];
+  // all 48 bytes in one round trip: without this use the third load would wait for the
+  // tag (the compiler sinks it into the triangle branch)
+  th_keep3(c.x, c.y, c.z);
   uint32_t tag = fbits(b.w);
   *tag_out = tag;
-  if (tag & THD_KIND_TRI) {
-    float4 c = p[2];
+  if (tag & THD_KIND_TRI)
     return tri_t(o, d, th::mk(a.x, a.y, a.z), th::mk(b.x, b.y, b.z), th::mk(c.x, c.y, c.z));
-  }
   return sphere_t(o, d, th::mk(a.x, a.y, a.z), a.w);
 }
 
@@ -139,13 +187,15 @@ struct Stack {
   uint2* base;  // LDS, entry k of this lane at base[k * stride]
   int stride;
   int sp;
+  // entry k (k and stride below 2^23: a full-rate 24-bit multiply)
+  __device__ __forceinline__ uint2& at(int k) { return base[__umul24((uint32_t)k, (uint32_t)stride)]; }
   __device__ __forceinline__ void push(uint32_t e, float t) {
-    base[sp * stride] = make_uint2(e, fbits(t));
+    at(sp) = make_uint2(e, fbits(t));
     ++sp;
   }
   __device__ __forceinline__ uint2 pop() {
     --sp;
-    return base[sp * stride];
+    return at(sp);
   }
 };
 
@@ -381,6 +431,9 @@ __device__ __forceinline__ void finalize_slot(const DevScene& S, uint32_t slot, 
                                               const v3& d, float t, Hit* h) {
   const float4* p = S.slots + 3u * (size_t)slot;
   float4 a = p[0], b = p[1], c = p[2];
+  th_keep3(a.x, a.y, a.z);  // the whole record in one round trip
+  th_keep3(a.w, c.x, c.y);
+  th_keep3(c.z, b.x, b.y);
   uint32_t tag = fbits(b.w);
   uint32_t mat = fbits(c.w);
   if (tag & THD_KIND_TRI) {

@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
           finalize_linear(S, slot, o, d, t, &rec);
         else
           finalize_slot(S, slot, o, d, t, &rec);
-        const TH_GpuMaterial& m = S.materials[rec.mat];
+        const TH_GpuMaterial m = load_material(S, rec.mat);
         v3 att = th::mk(0, 0, 0);
         bool brk = false;
         if (m.kind == 0u) {

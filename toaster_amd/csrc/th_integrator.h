@@ -222,8 +222,10 @@ struct Integrator {
 
   // ---------------------------------------------------------------- lights
   __device__ uint32_t pick_light(uint32_t* s) {  // wgsl:390-403 as a lower-bound search
+    // first light whose cumulative weight reaches the target, else the last one: the
+    // last light's own comparison cannot change the result, so it is not loaded
     float target = rnd(s) * P.total_light_weight;
-    uint32_t n = P.light_count, lo = 0, hi = n;
+    uint32_t n = P.light_count, lo = 0, hi = n - 1u;
     while (lo < hi) {
       uint32_t mid = (lo + hi) >> 1;
       if (S.lights[mid].area_weight_cumulative[2] >= target)
@@ -231,7 +233,7 @@ struct Integrator {
       else
         lo = mid + 1u;
     }
-    uint32_t idx = lo < n ? lo : n - 1u;
+    uint32_t idx = lo;
     if (COUNT) lc.lights_scanned += idx + 1u;
     return idx;
   }
@@ -242,7 +244,7 @@ struct Integrator {
   __device__ bool light_candidate(const v3& hp, const v3& hn, const v3& albedo, uint32_t* s, v3* so,
                                   v3* dir, float* limit, v3* term) {
     if (P.light_count == 0u || P.total_light_weight <= 0.0f) return false;
-    const TH_GpuLight& L = S.lights[pick_light(s)];
+    const TH_GpuLight L = load_light(S, pick_light(s));
     v3 pos, ln;
     if (L.kind == 0u) {
       float u = th::fsqrt(rnd(s));
@@ -258,7 +260,7 @@ struct Integrator {
       pos = th::add(th::mk(L.center_radius[0], L.center_radius[1], L.center_radius[2]),
                     th::smul(L.center_radius[3], ln));
     }
-    const TH_GpuMaterial& em = S.materials[L.material_index];
+    const TH_GpuMaterial em = load_material(S, L.material_index);
     *so = th::add(hp, th::muls(hn, TH_MIN_DISTANCE));
     v3 tl = th::sub(pos, *so);
     float d2 = th::dot(tl, tl);
@@ -374,7 +376,7 @@ struct Integrator {
         }
         break;
       }
-      const TH_GpuMaterial& m = S.materials[rec.mat];
+      const TH_GpuMaterial m = load_material(S, rec.mat);
       v3 att;
       v3 nd;
       bool brk = false;
@@ -512,7 +514,7 @@ struct OctetPath {
       }
       Hit rec;
       finalize_slot(S, slot, o, d, t, &rec);
-      const TH_GpuMaterial& m = S.materials[rec.mat];
+      const TH_GpuMaterial m = load_material(S, rec.mat);
       v3 att, nd;
       bool brk = false;
       if (m.kind == 0u) {

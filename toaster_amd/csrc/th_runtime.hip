@@ -337,6 +337,11 @@ __global__ void __launch_bounds__(256) k_render_octet(DevScene S, TH_GpuRenderPa
   }
 }
 
+// 4-wide nodes the quad and octet kernels copy into LDS: the top four levels.
+#ifndef TH_QUAD_LDS_NODES
+#define TH_QUAD_LDS_NODES 0u
+#endif
+
 // Reference RNG with dynamic pixel assignment.  A pixel's samples share one random
 // stream, so its samples run in sequence and a statically assigned wave lasts as long as
 // its slowest pixel.  Here every quad is persistent: when its pixel is done it takes the
@@ -680,6 +685,7 @@ DevScene dev_scene(const th_scene* s) {
   S.root_entry = s->root_entry;
   S.nodes4 = (const float4*)s->d_nodes4;
   S.root4 = s->root4;
+  S.node4_bytes = (uint32_t)((size_t)s->node4_count * 4u * sizeof(Child4));
   for (int k = 0; k < 3; ++k) {
     S.root_min[k] = s->root_min[k];
     S.root_max[k] = s->root_max[k];
@@ -940,7 +946,7 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   } else if (o.traversal == TH_TRAVERSE_WIDE && s->node_count > 0) {
     const uint32_t qblock = 256;
     uint32_t depth = 3u * s->depth4 + 2u;
-    uint32_t cached = std::min<uint32_t>(s->node4_count, 85u);  // top four 4-wide levels
+    uint32_t cached = std::min<uint32_t>(s->node4_count, TH_QUAD_LDS_NODES);
     size_t lds4 = (size_t)depth * (qblock / 4) * sizeof(uint2) + (size_t)cached * 4 * sizeof(Child4);
     int G = o.rng_mode == TH_RNG_REFERENCE ? 1
             : (p->samples >= 16 ? 16 : p->samples >= 8 ? 8 : p->samples >= 4 ? 4 : p->samples >= 2 ? 2 : 1);

@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
           finalize_linear(S, slot, o, d, t, &rec);
         else
           finalize_slot(S, slot, o, d, t, &rec);
-        const TH_GpuMaterial& m = S.materials[rec.mat];
+        const TH_GpuMaterial m = load_material(S, rec.mat);
         v3 att = th::mk(0, 0, 0), nd = d;
         bool brk = false;
         if (m.kind == 0u) {

@@ -46,11 +46,31 @@ __device__ __forceinline__ float oswap_f(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
 }
 
+// ------------------------------------------------------------------ node fetch
+#ifndef TH_NODE_BUFFER_LOAD
+#define TH_NODE_BUFFER_LOAD 1
+#endif
+// Buffer resource over the node4 array (raw, 32-bit offsets; word 3 = 0x00020000, the
+// gfx9 raw-buffer format).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(const Child4* nodes4, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)nodes4, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ Child4 load_child(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  v4u x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  v4u y = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, 0);
+  Child4 c;
+  c.a = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z), __uint_as_float(x.w));
+  c.b = make_float4(__uint_as_float(y.x), __uint_as_float(y.y), __uint_as_float(y.z), __uint_as_float(y.w));
+  return c;
+}
+
 // ------------------------------------------------------------------ ray + box
 struct RayPre {
   v3 o, d, inv;
   bool px, py, pz;  // |d_axis| < 1e-8: parallel axis (wgsl:712-717)
   bool anypar;
+  float pinf, ninf;  // +inf / -inf, opaque to constant folding (mn2 / mx2)
 };
 
 __device__ __forceinline__ RayPre make_ray(const v3& o, const v3& d) {
@@ -62,6 +82,12 @@ __device__ __forceinline__ RayPre make_ray(const v3& o, const v3& d) {
   r.py = th::fabs_(d.y) < TH_EPSILON;
   r.pz = th::fabs_(d.z) < TH_EPSILON;
   r.anypar = r.px || r.py || r.pz;
+  // an empty asm hides the constants: the compiler would otherwise turn med3 against
+  // +-inf back into fmaxf / fminf with their NaN-quieting instructions
+  r.pinf = __builtin_inff();
+  r.ninf = -__builtin_inff();
+  __asm__("" : "+s"(r.pinf));
+  __asm__("" : "+s"(r.ninf));
   return r;
 }
 
@@ -69,18 +95,24 @@ __device__ __forceinline__ RayPre make_ray(const v3& o, const v3& d) {
 // reference only ever shrink [t_min, t_max], so testing t_min <= t_max once at the end
 // accepts exactly the same boxes; a parallel axis contributes (-inf, inf) when the
 // origin lies in the slab and (inf, -inf) otherwise.
+// min / max as v_med3 against -inf / +inf: exact for non-NaN operands (every slab value
+// of a ray with no parallel axis is finite; a parallel axis's values are replaced below),
+// without the NaN-quieting instructions fminf / fmaxf need in IEEE mode.
+__device__ __forceinline__ float mn2(const RayPre& r, float a, float b) { return __builtin_amdgcn_fmed3f(a, b, r.ninf); }
+__device__ __forceinline__ float mx2(const RayPre& r, float a, float b) { return __builtin_amdgcn_fmed3f(a, b, r.pinf); }
+
 __device__ __forceinline__ bool box_bf(float mnx, float mny, float mnz, float mxx, float mxy,
                                        float mxz, const RayPre& r, float tmax, float* te) {
   const float INF = __builtin_inff();
   float nx = (mnx - r.o.x) * r.inv.x, fx = (mxx - r.o.x) * r.inv.x;
   float ny = (mny - r.o.y) * r.inv.y, fy = (mxy - r.o.y) * r.inv.y;
   float nz = (mnz - r.o.z) * r.inv.z, fz = (mxz - r.o.z) * r.inv.z;
-  float ax = __builtin_fminf(nx, fx), bx = __builtin_fmaxf(nx, fx);
-  float ay = __builtin_fminf(ny, fy), by = __builtin_fmaxf(ny, fy);
-  float az = __builtin_fminf(nz, fz), bz = __builtin_fmaxf(nz, fz);
+  float ax = mn2(r, nx, fx), bx = mx2(r, nx, fx);
+  float ay = mn2(r, ny, fy), by = mx2(r, ny, fy);
+  float az = mn2(r, nz, fz), bz = mx2(r, nz, fz);
   if (__builtin_expect(!r.anypar, 1)) {  // (almost) every ray: no parallel axis
-    float lo = __builtin_fmaxf(__builtin_fmaxf(TH_MIN_DISTANCE, ax), __builtin_fmaxf(ay, az));
-    float hi = __builtin_fminf(__builtin_fminf(tmax, bx), __builtin_fminf(by, bz));
+    float lo = mx2(r, mx2(r, TH_MIN_DISTANCE, ax), mx2(r, ay, az));
+    float hi = mn2(r, mn2(r, tmax, bx), mn2(r, by, bz));
     *te = lo;
     return !(hi < lo);
   }
@@ -124,6 +156,9 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
   float te;
   *best_t = TH_FLT_MAX;
   *best_slot = THD_NO_SLOT;
+#if TH_NODE_BUFFER_LOAD
+  const __amdgpu_buffer_rsrc_t nrs = node_rsrc(nodes4, S.node4_bytes);
+#endif
   if (!active || !box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0],
                          S.root_max[1], S.root_max[2], r, best, &te))
     return false;
@@ -167,29 +202,41 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
         }
       }
     } else {
+#if TH_NODE_BUFFER_LOAD
+      // LDS (top levels) or a buffer load: two distinct load forms, so the compiler
+      // cannot merge them into one flat load that waits on both memory counters
+      Child4 c;
+      if (!(e < cached)) c = load_child(nrs, (4u * e + lq) * (uint32_t)sizeof(Child4));
+      else c = lds_nodes[4u * e + lq];
+#else
       Child4 c = e < cached ? lds_nodes[4u * e + lq] : nodes4[4u * e + lq];
+#endif
+      th_keep3(c.b.x, c.b.y, c.b.z);  // both halves of the child in one round trip
       uint32_t ce = __float_as_uint(c.a.w);
       float tE = INF;
       if (ce != THD_NO_CHILD) {
         float t;
         if (box_bf(c.a.x, c.a.y, c.a.z, c.b.x, c.b.y, c.b.z, r, best, &t)) tE = t;
       }
-      float t0 = qbcast_f<0>(tE), t1 = qbcast_f<1>(tE), t2 = qbcast_f<2>(tE), t3 = qbcast_f<3>(tE);
-      uint32_t e0 = qbcast_u<0>(ce), e1 = qbcast_u<1>(ce), e2 = qbcast_u<2>(ce), e3 = qbcast_u<3>(ce);
-      uint32_t nh = (t0 != INF) + (t1 != INF) + (t2 != INF) + (t3 != INF);
+      // Visit order by key = entry-distance bits with the two low bits replaced by the
+      // lane: unique in the quad, nearest first up to the last two mantissa bits
+      // (distances are positive or +inf, whose bits order as the floats).  The order
+      // only changes the work done: the closest hit and its tie-breaking do not depend
+      // on it.  A missed child keys at or above MISS.
+      const uint32_t MISS = 0x7f800000u;
+      uint32_t key = (__float_as_uint(tE) & ~3u) | lq;
+      uint32_t k0 = qbcast_u<0>(key), k1 = qbcast_u<1>(key), k2 = qbcast_u<2>(key), k3 = qbcast_u<3>(key);
+      uint32_t nh = (k0 < MISS) + (k1 < MISS) + (k2 < MISS) + (k3 < MISS);
       if (nh > 0u) {
-        // rank of this lane's child among hit children by (tE, lane)
-        uint32_t rank = (t0 < tE || (t0 == tE && 0u < lq)) + (t1 < tE || (t1 == tE && 1u < lq)) +
-                        (t2 < tE || (t2 == tE && 2u < lq)) + (t3 < tE || (t3 == tE && 3u < lq));
-        // nearest child (rank 0)
-        uint32_t w = 0;
-        float tw = t0;
-        if (t1 < tw) { tw = t1; w = 1; }
-        if (t2 < tw) { tw = t2; w = 2; }
-        if (t3 < tw) { tw = t3; w = 3; }
-        uint32_t ew = w == 0 ? e0 : (w == 1 ? e1 : (w == 2 ? e2 : e3));
-        if (tE != INF && rank > 0u) st.base[(st.sp + (int)(nh - 1u - rank)) * st.stride] =
-            make_uint2(ce, __float_as_uint(tE));
+        uint32_t rank = (k0 < key) + (k1 < key) + (k2 < key) + (k3 < key);
+        uint32_t w = __builtin_elementwise_min(__builtin_elementwise_min(k0, k1),
+                                               __builtin_elementwise_min(k2, k3)) & 3u;
+        uint32_t e0 = qbcast_u<0>(ce), e1 = qbcast_u<1>(ce), e2 = qbcast_u<2>(ce), e3 = qbcast_u<3>(ce);
+        // entry of lane w by its two bits (selects, no branches)
+        uint32_t e01 = (w & 1u) ? e1 : e0, e23 = (w & 1u) ? e3 : e2;
+        uint32_t ew = (w & 2u) ? e23 : e01;
+        // the other hit children, farthest deepest
+        if (key < MISS && rank > 0u) st.at(st.sp + (int)(nh - 1u - rank)) = make_uint2(ce, __float_as_uint(tE));
         st.sp += (int)nh - 1;
         e = ew;
         continue;
