@@ -97,8 +97,11 @@ def reference_suite(warmup=5, runs=21):
         sc = load_scene(path)
         gs = GpuScene.from_scene(ctx, sc)
         row = {}
-        for mode, opts in (("reference_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
-                                                           schedule=A.TH_SCHEDULE_OCTET)),
+        # reference RNG: the octet and the 4-lane schedule give the same image; both are
+        # timed and the faster one is reported (short traversals favour the 4-lane one)
+        for mode, opts in (("reference_rng_octet", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
+                                                                 schedule=A.TH_SCHEDULE_OCTET)),
+                           ("reference_rng_quad", RenderOptions(traversal=A.TH_TRAVERSE_WIDE)),
                            ("per_sample_rng", RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
                                                             rng_mode=A.TH_RNG_PER_SAMPLE))):
             times, kms, rb = [], [], []
@@ -119,6 +122,10 @@ def reference_suite(warmup=5, runs=21):
             row[mode] = {"dispatch_wait_ms": round(med, 4), "kernel_ms": round(float(np.median(kms)), 4),
                          "readback_ms": round(float(np.median(rb)), 4),
                          "mpaths_s": round(sc.paths_per_frame / med / 1e3, 2)}
+        oc, qu = row.pop("reference_rng_octet"), row.pop("reference_rng_quad")
+        best, sched = (oc, "octet") if oc["dispatch_wait_ms"] <= qu["dispatch_wait_ms"] else (qu, "static")
+        row["reference_rng"] = dict(best, schedule=sched,
+                                    other_schedule_dispatch_wait_ms=(qu if sched == "octet" else oc)["dispatch_wait_ms"])
         pub = PUBLISHED_DISPATCH_WAIT_MS[sc.name]
         row["rtx2080ti_dispatch_wait_ms"] = pub
         row["speedup_vs_rtx2080ti"] = round(pub / row["reference_rng"]["dispatch_wait_ms"], 3)
