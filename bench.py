@@ -222,6 +222,24 @@ def equal_time_guided(reference_spp):
     }
 
 
+def profile_guided(passes):
+    """The guided tracer alone, as the equal-time leg runs it (fresh guide, default
+    settings, one sample per pixel per pass) on the terrain workload."""
+    from toaster_amd import GpuContext, GpuScene, scenes
+    from toaster_amd.guide import GUIDED_DEFAULTS, Guide, render_guided
+    ctx = GpuContext(0)
+    gs = GpuScene.from_scene(ctx, scenes.terrain(1280, 720, samples=16, max_bounces=8))
+    g = Guide(ctx, seed=1)
+    t = time.perf_counter()
+    st = render_guided(gs, g, passes, progressive=False, **GUIDED_DEFAULTS)
+    ctx.synchronize()
+    print(f"guided: {passes} passes in {time.perf_counter() - t:.3f} s, {st['train_steps']} training steps",
+          file=sys.stderr)
+    g.close()
+    gs.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,7 +254,13 @@ def main():
     ap.add_argument("--no-guided", action="store_true")
     ap.add_argument("--no-suite", action="store_true")
     ap.add_argument("--reference-spp", type=int, default=4096)
+    ap.add_argument("--profile-guided", type=int, default=0, metavar="PASSES",
+                    help="only render PASSES guided passes on the terrain workload (for a profiler); "
+                         "prints no JSON line")
     args = ap.parse_args()
+    if args.profile_guided:
+        profile_guided(args.profile_guided)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported",
               file=sys.stderr)

@@ -51,9 +51,9 @@ struct GdBufs {
   float4 *vp, *vn, *vw;  // guide-queue row: (p, albedo.x), (n, albedo.y), (wo, albedo.z)
   float4 *sho, *sh0, *c0, *sh1, *c1;
   uint32_t* vis;
-  uint32_t *q0, *q1, *qs, *qg, *qd;
-  uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [4] records offered,
-                  // [5] diffuse total, [6] done, [8 + c] records of chunk c
+  uint32_t *q0, *q1, *qs, *qg;
+  uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [5] diffuse total,
+                  // [8 + c] records of chunk c
   float* O;       // guide outputs, TG_OUT per guide-queue row
   uint16_t* rX;   // training records: BF16 features, 64 wide (zero beyond 48)
   float *rdir, *rw;
@@ -96,16 +96,32 @@ __device__ __forceinline__ uint32_t gd_push(bool live, uint32_t value, uint32_t*
   return idx;
 }
 
+// Two pushes onto one queue with one atomic: `la` lanes' values a, then `lb` lanes'
+// values b.
+__device__ __forceinline__ void gd_push2(bool la, uint32_t a, bool lb, uint32_t b, uint32_t* queue,
+                                         uint32_t* count) {
+  uint64_t ma = __ballot(la), mb = __ballot(lb);
+  if ((ma | mb) == 0) return;
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t first = (uint32_t)__ffsll((unsigned long long)(ma | mb)) - 1u;
+  uint32_t na = (uint32_t)__popcll(ma);
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(count, na + (uint32_t)__popcll(mb));
+  base = __shfl(base, (int)first, 64);
+  uint64_t lt = (1ull << lane) - 1ull;
+  if (la) queue[base + (uint32_t)__popcll(ma & lt)] = a;
+  if (lb) queue[base + na + (uint32_t)__popcll(mb & lt)] = b;
+}
+
 __global__ void gd_reset(GdBufs B, uint32_t all) {
   const uint32_t t = threadIdx.x;
-  if (t < 5u || t == 6u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[GD_C(t)] = 0u;
+  if (t < 4u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[GD_C(t)] = 0u;
 }
 
 // ---------------------------------------------------------------- generate
 __global__ void __launch_bounds__(256) gd_generate(GdBufs B, GdArgs A, TH_GpuRenderParams P,
                                                    TH_GpuCamera cam) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = false;
   if (k < A.n) {
     float4 a = A.smp == 0u ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : B.acc[k];
     if (P.max_bounces == 0u) {  // ray_color returns zero (wgsl:536)
@@ -119,15 +135,18 @@ __global__ void __launch_bounds__(256) gd_generate(GdBufs B, GdArgs A, TH_GpuRen
       B.thr[k] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(GD_NO_RECORD));
       B.rad[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       B.rng[k] = rng;
-      live = true;
     }
     B.acc[k] = a;
   }
-  gd_push(live, k, B.q0, &B.cnt[GD_C(0)]);
+  // every path is live at the first bounce: that bounce's queue is the identity (path i
+  // at position i), so only its length is stored
+  if (k == 0u) B.cnt[GD_C(0)] = P.max_bounces == 0u ? 0u : A.n;
 }
 
 // ---------------------------------------------------------------- extend / shadow
-__global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRenderParams P, uint32_t li) {
+// `first`: the first bounce, whose queue is the identity.
+__global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRenderParams P, uint32_t li,
+                                                uint32_t first) {
   extern __shared__ uint2 lds_stack[];
   const uint32_t* q = li ? B.q1 : B.q0;
   const uint32_t n = B.cnt[GD_C(li)];
@@ -142,7 +161,7 @@ __global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRend
     bool act = i < n;  // uniform per quad
     if (!__any(act)) break;
     if (act) {
-      uint32_t k = q[i];
+      uint32_t k = first ? i : q[i];
       float4 o4 = B.o[k], d4 = B.d[k];
       v3 o = th::mk(o4.x, o4.y, o4.z), d = th::mk(d4.x, d4.y, d4.z);
       float t;
@@ -273,6 +292,7 @@ __device__ __forceinline__ void apply_pending(const GdBufs& B, uint32_t k, uint3
 // ---------------------------------------------------------------- shade A
 __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
                                                  const float* __restrict__ srgb_lut, uint32_t li) {
+  const bool first = A.bounce == 0u;  // identity queue
   const uint32_t* q = li ? B.q1 : B.q0;
   const uint32_t n = B.cnt[GD_C(li)];
   Stack st{nullptr, 0, 0};
@@ -282,11 +302,11 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
-    bool sh_light = false, sh_env = false, park = false, live = false, fin = false;
+    bool sh_light = false, sh_env = false, park = false, live = false;
     uint32_t k = 0;
     float4 pk_p, pk_n, pk_w;  // the parked vertex
     if (act) {
-      k = q[i];
+      k = first ? i : q[i];
       float4 o4 = B.o[k], d4 = B.d[k], t4 = B.thr[k], r4 = B.rad[k];
       apply_pending(B, k, __float_as_uint(d4.w), r4, t4);
       uint32_t rng = B.rng[k];
@@ -383,14 +403,11 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
         B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, t4.w);
         B.rng[k] = rng;
         live = !done;
-        fin = done;
       }
       B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
     }
-    gd_push(sh_light, 2u * k, B.qs, &B.cnt[GD_C(2)]);
-    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
+    gd_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
     gd_push(live, k, li ? B.q0 : B.q1, &B.cnt[GD_C(li ^ 1u)]);
-    gd_push(fin, k, B.qd, &B.cnt[GD_C(6)]);
     const uint32_t gi = gd_push(park, k, B.qg, &B.cnt[GD_C(3)]);
     if (park) {  // vertex data in queue order: contiguous rows for inference and shade B
       B.vp[gi] = pk_p;
@@ -413,7 +430,7 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
-    bool sh_light = false, sh_env = false, rec_ok = false, live = false, fin = false;
+    bool sh_light = false, sh_env = false, rec_ok = false, live = false;
     uint32_t k = 0;
     v3 p, nrm, wo, nd, beta;
     if (act) {
@@ -456,13 +473,9 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, __uint_as_float(GD_NO_RECORD));
       B.rng[k] = rng;
       live = !done;
-      fin = done;
     }
-    gd_push(sh_light, 2u * k, B.qs, &B.cnt[GD_C(2)]);
-    gd_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
+    gd_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
     gd_push(live, k, li ? B.q0 : B.q1, &B.cnt[GD_C(li ^ 1u)]);
-    gd_push(fin, k, B.qd, &B.cnt[GD_C(6)]);
-    gd_push(rec_ok, 0u, nullptr, &B.cnt[GD_C(4)]);  // records offered
     // chunk drawn at random so every training step sees an unbiased subset; slots are
     // handed out with one atomic per (wave, chunk)
     uint32_t r = 0xffffffffu;
@@ -513,10 +526,10 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
 // ---------------------------------------------------------------- retire
 // Paths finished during the pass: their last deferred direct terms, then
 // color += vec4(ray_color, 1) (wgsl:536).
-__global__ void __launch_bounds__(256) gd_retire(GdBufs B) {
-  const uint32_t n = B.cnt[GD_C(6)];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t k = B.qd[i];
+// Every path of a pass finishes exactly once (n = 0 when there are no bounces: the
+// generate kernel has counted those samples).
+__global__ void __launch_bounds__(256) gd_retire(GdBufs B, uint32_t n) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     float4 r4 = B.rad[k], t4 = B.thr[k];
     apply_pending(B, k, __float_as_uint(B.d[k].w), r4, t4);
     B.rad[k] = r4;
@@ -604,7 +617,7 @@ static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
       (rc = gd_alloc(w, (void**)&b.sh1, f4)) || (rc = gd_alloc(w, (void**)&b.c1, f4)) ||
       (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, u)) ||
       (rc = gd_alloc(w, (void**)&b.q1, u)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * u)) ||
-      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.qd, u)) || (rc = gd_alloc(w, (void**)&b.cnt, GD_C(32) * sizeof(uint32_t))) ||
+      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.cnt, GD_C(32) * sizeof(uint32_t))) ||
       (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.rX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
       (rc = gd_alloc(w, (void**)&b.rdir, (size_t)rcap * 3 * sizeof(float))) ||
@@ -700,7 +713,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     for (uint32_t b = 0; b < p->max_bounces; ++b) {
       const uint32_t li = b & 1u;
       A.bounce = b;
-      hipLaunchKernelGGL(gd_extend, dim3(grid_trace), dim3(block), lds, sm, B, S, *p, li);
+      hipLaunchKernelGGL(gd_extend, dim3(grid_trace), dim3(block), lds, sm, B, S, *p, li, b == 0u ? 1u : 0u);
       hipLaunchKernelGGL(gd_shade_a, dim3(grid_scalar), dim3(block), 0, sm, B, A, S, *p,
                          (const float*)ctx->d_srgb_lut, li);
       if (A.defer) {
@@ -714,7 +727,8 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
       }
       hipLaunchKernelGGL(gd_shadow, dim3(grid_shadow), dim3(block), lds, sm, B, S, *p);
     }
-    hipLaunchKernelGGL(gd_retire, dim3(std::min<uint32_t>((npix + 255) / 256, 2048u)), dim3(256), 0, sm, B);
+    hipLaunchKernelGGL(gd_retire, dim3(std::min<uint32_t>((npix + 255) / 256, 2048u)), dim3(256), 0, sm, B,
+                       p->max_bounces == 0u ? 0u : npix);
     if (train_pass) {
       hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((pass_steps * chunk_cap + 255) / 256, 4096u)),
                          dim3(256), 0, sm, B, pass_steps, chunk_cap);
