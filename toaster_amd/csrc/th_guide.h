@@ -127,22 +127,47 @@ TH_HD float flog(float x) {
 #endif
 }
 
+// Reciprocal, reciprocal square root and square root for the guided estimator.  On the
+// device these are the hardware approximations (about 1 ulp): the estimator only needs
+// the sampling and the pdf to use one decoded mixture, not correctly rounded values.
+TH_HD float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+TH_HD float frsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+TH_HD float fsqrt_fast(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 TH_HD void decode(const float* o, Mixture* m) {
   float mx = -3.0e38f;
   for (int k = 0; k < TG_LOBES; ++k) mx = th::fmax_(mx, o[32 + k]);
   float sum = 0.0f;
   for (int k = 0; k < TG_LOBES; ++k) {
     v3 r = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
-    float len = th::fsqrt(th::dot(r, r) + 1e-8f);
-    m->mu[k] = th::divs(r, len);
+    m->mu[k] = th::smul(frsq(th::dot(r, r) + 1e-8f), r);
     m->kappa[k] = th::fmin_(softplus(o[24 + k]) + 0.01f, 1000.0f);
     m->w[k] = fexp(o[32 + k] - mx);
     sum += m->w[k];
   }
   const float INV_2PI = 0.15915494f;
+  const float rs = frcp(sum);
   for (int k = 0; k < TG_LOBES; ++k) {
-    m->w[k] = m->w[k] / sum;
-    m->wn[k] = m->w[k] * (m->kappa[k] * INV_2PI / (1.0f - fexp(-2.0f * m->kappa[k])));
+    m->w[k] = m->w[k] * rs;
+    m->wn[k] = m->w[k] * (m->kappa[k] * INV_2PI * frcp(1.0f - fexp(-2.0f * m->kappa[k])));
   }
 }
 
@@ -161,16 +186,16 @@ TH_HD float mixture_pdf(const Mixture& m, v3 w) {
 // Orthonormal basis around a unit vector (Duff et al. 2017, branchless).
 TH_HD void onb(v3 n, v3* t, v3* b) {
   float s = n.z >= 0.0f ? 1.0f : -1.0f;
-  float a = -1.0f / (s + n.z);
+  float a = -frcp(s + n.z);
   float bb = n.x * n.y * a;
   *t = th::mk(1.0f + s * n.x * n.x * a, s * bb, -s * n.x);
   *b = th::mk(bb, s + n.y * n.y * a, -n.y);
 }
 
 TH_HD v3 vmf_sample(v3 mu, float kappa, float u1, float u2) {
-  float w = 1.0f + flog(u1 + (1.0f - u1) * fexp(-2.0f * kappa)) / kappa;
+  float w = 1.0f + flog(u1 + (1.0f - u1) * fexp(-2.0f * kappa)) * frcp(kappa);
   w = th::clampf(w, -1.0f, 1.0f);
-  float r = th::fsqrt(th::fmax_(0.0f, 1.0f - w * w));
+  float r = fsqrt_fast(th::fmax_(0.0f, 1.0f - w * w));
   float s, c;
   th::sincos_(6.2831855f * u2, &s, &c);
   v3 t, b;

@@ -480,6 +480,10 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
 
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
+#ifndef TG_INFER_WAVES_PER_CU
+#define TG_INFER_WAVES_PER_CU 8u
+#endif
+
 }  // namespace
 
 // ====================================================================== guide object
@@ -581,7 +585,9 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
 int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const float4* vp,
                     const float4* vn, const float4* vw, const float* bmin, const float* bmax, float* O,
                     hipStream_t st) {
-  uint32_t grid = std::min<uint32_t>((max_rows + 15u) / 16u, 8192u);
+  // persistent: as many one-wave blocks as stay resident (2 per SIMD at this kernel's
+  // register count), so each wave loads the weights once and walks many tiles
+  uint32_t grid = std::min<uint32_t>((max_rows + 15u) / 16u, TG_INFER_WAVES_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1));
   if (grid == 0) return TH_OK;
   float4 lo = make_float4(bmin[0], bmin[1], bmin[2], 0.0f), sc;
   float e[3];

@@ -41,7 +41,7 @@ TH_HD float ffloor(float x) {
   return floorf(x);
 #endif
 }
-TH_HD float fabs_(float x) { return x < 0.0f ? -x : (x == 0.0f ? 0.0f : x); }
+TH_HD float fabs_(float x) { return __builtin_fabsf(x); }
 TH_HD float fmin_(float a, float b) { return b < a ? b : a; }
 TH_HD float fmax_(float a, float b) { return a < b ? b : a; }
 TH_HD float clampf(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }

@@ -276,9 +276,10 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
 // of its previous vertex at the same time (OctetPath).  Reference RNG (G = 1): the octet
 // runs the pixel's samples in stream order.  Per-sample RNG: G octets share a pixel
 // (samples j, j+G, ... per octet), combined by the same butterfly as k_render_quad.  A
-// wave covers 8/G pixels.
+// wave covers 8/G pixels.  The reference-RNG variant without an environment is held to
+// 96 VGPRs (5 waves per SIMD), which it fits without spilling.
 template <bool ENV, bool PER_SAMPLE, int G>
-__global__ void __launch_bounds__(256) k_render_octet(DevScene S, TH_GpuRenderParams P,
+__global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_octet(DevScene S, TH_GpuRenderParams P,
                                                       TH_GpuCamera cam, float4* __restrict__ accum,
                                                       const float* __restrict__ srgb_lut,
                                                       uint32_t cached) {
