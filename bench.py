@@ -20,6 +20,10 @@ and sample streams.  From the two plain points, relRMSE^2 = C / spp + E (E = the
 reference's own noise); the plain time to reach a guided run's error is
 T_plain(256) * (C / (relRMSE_guided^2 - E)) / 256, and time_ratio is the guided time over
 that, averaged over the five runs.
+
+reference_suite times the reference's own benchmark scenes (scenes/benchmarks/001-007)
+as its benchmark mode does; batched renders 16 scenes x 4 frames in one persistent
+launch beside one launch per frame.
 """
 import argparse
 import json
@@ -222,6 +226,48 @@ def equal_time_guided(reference_spp):
     }
 
 
+def batched_leg(runs=3):
+    """The batched workload: 16 scenes (128 - 262,144 triangles, 0 - 256 spheres, three
+    resolutions) x 4 rigid-motion frames, 16 spp, 6 bounces, per-sample RNG, rendered by
+    one persistent launch (th_render_batch), beside the same 64 frames rendered one
+    th_render each."""
+    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
+    from toaster_amd.batch import BatchRenderer
+    frames = [f for per_scene in scenes.batched_workload() for f in per_scene]
+    ctx = GpuContext(0)
+    br = BatchRenderer(ctx)
+    for sc in frames:
+        br.add(sc)
+    o = RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=A.TH_TRAVERSE_WIDE)
+    br.render(o, use_tuning=False)
+    ctx.synchronize()
+    ctx.reset_stats()
+    t = time.perf_counter()
+    for _ in range(runs):
+        br.render(o, use_tuning=False)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t) / runs
+    kms = ctx.stats()["total_kernel_ms"] / runs
+    # the same frames, one launch each
+    for gs in br.scenes:
+        gs.render(options=o, frame_index=0, accumulated_samples=0)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for gs in br.scenes:
+        gs.render(options=o, frame_index=0, accumulated_samples=0)
+    ctx.synchronize()
+    dseq = time.perf_counter() - t
+    paths = br.paths
+    br.close()
+    ctx.close()
+    return {"workload": "16 scenes x 4 rigid-motion frames, 128 - 262,144 triangles, 0 - 256 spheres, "
+                        "320x180 / 640x360 / 1280x720, 16 spp, 6 bounces, per-sample RNG",
+            "frames": len(frames), "paths": paths, "launch_wall_ms": round(dt * 1e3, 2),
+            "launch_kernel_ms": round(kms, 2), "gpaths_s": round(paths / dt / 1e9, 3),
+            "one_launch_per_frame_wall_ms": round(dseq * 1e3, 2),
+            "one_launch_per_frame_gpaths_s": round(paths / dseq / 1e9, 3)}
+
+
 def profile_guided(passes):
     """The guided tracer alone, as the equal-time leg runs it (fresh guide, default
     settings, one sample per pixel per pass) on the terrain workload."""
@@ -253,6 +299,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true")
     ap.add_argument("--no-suite", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--reference-spp", type=int, default=4096)
     ap.add_argument("--profile-guided", type=int, default=0, metavar="PASSES",
                     help="only render PASSES guided passes on the terrain workload (for a profiler); "
@@ -337,6 +384,7 @@ def main():
 
     equal_time = None if args.no_guided else equal_time_guided(args.reference_spp)
     suite = None if args.no_suite else reference_suite()
+    batched = None if args.no_batch else batched_leg()
     cpu, counters = (None, None)
     if not args.no_cpu:
         cpu, counters = cpu_baseline(sc)
@@ -372,6 +420,7 @@ def main():
         "variants": variants,
         "equal_time": equal_time,
         "reference_suite": suite,
+        "batched": batched,
         "cpu_baseline": cpu,
         "host": platform.node(),
     }
