@@ -85,11 +85,13 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const uint16_t* __restrict__ 
 }
 
 // Loss gradient of one batch: L = -sum_b w_b log q_b(dir_b) / sum_b w_b.  The weight sum
-// is reduced in a fixed order (256 block partials, then one block), so gradients are
-// reproducible call to call.
+// is reduced in a fixed order (256 block partials here, then the same 256-to-1 tree in
+// every block of the training kernel), so gradients are reproducible call to call.
+// Block 0 also clears the step's scalars (sums[1]: the loss accumulator).
 __global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict__ w, uint32_t n,
-                                                        float* __restrict__ part) {
+                                                        float* __restrict__ part, float* __restrict__ sums) {
   __shared__ float red[256];
+  if (blockIdx.x == 0 && threadIdx.x < 16u) sums[threadIdx.x] = 0.0f;
   float s = 0.0f;
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) s += w[i];
   red[threadIdx.x] = s;
@@ -101,19 +103,18 @@ __global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void __launch_bounds__(256) k_weight_total(const float* __restrict__ part, float* __restrict__ sums) {
-  __shared__ float red[256];
+// The batch's total weight from the 256 partials (fixed-order tree; `red` holds 256
+// floats of LDS).  Every thread of the block gets it.
+__device__ __forceinline__ float weight_total(const float* __restrict__ part, float* red) {
   red[threadIdx.x] = part[threadIdx.x];
   __syncthreads();
   for (uint32_t off = 128; off > 0; off >>= 1) {
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) sums[0] = red[0];
-}
-
-__global__ void k_clear(float* p, uint32_t n) {
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0f;
+  const float t = red[0];
+  __syncthreads();
+  return t;
 }
 
 // Fused training step.  One block (4 waves) walks `rows_per` consecutive samples in
@@ -138,7 +139,7 @@ __device__ __forceinline__ float quad_max(float v) {
 __global__ void __launch_bounds__(256) k_train_fused(const uint16_t* __restrict__ X,
                                                      const float* __restrict__ dirs,
                                                      const float* __restrict__ w, uint32_t count,
-                                                     uint32_t rows_per, const float* __restrict__ sums,
+                                                     uint32_t rows_per, const float* __restrict__ wpart,
                                                      const uint16_t* __restrict__ woi,
                                                      const uint16_t* __restrict__ wio,
                                                      const float* __restrict__ bias,
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(256) k_train_fused(const uint16_t* __restrict_
   __shared__ __align__(16) float tO[64 * TG_OUT];
   const uint32_t tid = threadIdx.x, wv = tid >> 6, l = tid & 63u, r = l & 15u, g = l >> 4;
   const uint32_t r0 = blockIdx.x * rows_per, r1 = min(r0 + rows_per, count);
-  const float S = sums[0];
+  const float S = weight_total(wpart, (float*)tDa);  // tDa is free until the backward pass
   const uint16_t* hin[4] = {tX, tH1, tH2, tH3};
   tg_f32x4 acc[4][4];
 #pragma unroll
@@ -555,14 +556,12 @@ static int32_t th_guide_forward_rows(th_guide* g, const uint16_t* X, const uint3
 static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, const float* w,
                           uint32_t count, float lr) {
   hipStream_t st = g->ctx->stream;
-  hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, g->scal, 16u);
-  hipLaunchKernelGGL(k_weight_partial, dim3(256), dim3(256), 0, st, w, count, g->wpart);
-  hipLaunchKernelGGL(k_weight_total, dim3(1), dim3(256), 0, st, g->wpart, g->scal);
+  hipLaunchKernelGGL(k_weight_partial, dim3(256), dim3(256), 0, st, w, count, g->wpart, g->scal);
   // 64-sample chunks, at most kWgradBlocks blocks (one partial slot each)
   const uint32_t rows_per = round_up((count + kWgradBlocks - 1) / kWgradBlocks, 64);
   const uint32_t nb = (count + rows_per - 1) / rows_per;
   float* P = g->partials;
-  hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(256), 0, st, X, dirs, w, count, rows_per, g->scal, g->d_woi,
+  hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(256), 0, st, X, dirs, w, count, rows_per, g->wpart, g->d_woi,
                      g->d_wio, g->d_bias, P, g->scal + 1);
   TH_CHECK(hipGetLastError());
   float bc1 = 1.0f, bc2 = 1.0f;
@@ -600,12 +599,15 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, con
   return TH_OK;
 }
 
-int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count,
-                    float* loss_out) {
+int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count) {
   int32_t rc = guide_reserve(g, count);
   if (rc) return rc;
-  if ((rc = guide_step(g, X, dirs, w, count, g->lr))) return rc;
-  if (loss_out) TH_CHECK(hipMemcpyAsync(loss_out, g->scal + 1, 4, hipMemcpyDeviceToHost, g->ctx->stream));
+  return guide_step(g, X, dirs, w, count, g->lr);
+}
+
+// The last step's loss, copied once (a copy per step would stall the stream each time).
+int32_t guide_last_loss(th_guide* g, float* out) {
+  TH_CHECK(hipMemcpyAsync(out, g->scal + 1, 4, hipMemcpyDeviceToHost, g->ctx->stream));
   return TH_OK;
 }
 

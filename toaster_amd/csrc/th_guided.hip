@@ -742,7 +742,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
         st.records += cnt;
         if (cnt < 1024u) continue;  // too few samples for a step
         const size_t off = (size_t)j * chunk_cap;
-        if ((rc = guide_train(g, B.rX + off * 64u, B.rdir + off * 3u, B.rw + off, cnt, &st.last_loss))) {
+        if ((rc = guide_train(g, B.rX + off * 64u, B.rdir + off * 3u, B.rw + off, cnt))) {
           (void)end_timed(ctx, evp);
           return rc;
         }
@@ -756,6 +756,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if (le != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("guided launch: ") + hipGetErrorString(le));
   ctx->counters_pending = false;
   ctx->stats.paths = npix64 * p->samples;
+  if (st.train_steps > 0 && (rc = guide_last_loss(g, &st.last_loss))) return rc;
   if (train) TH_CHECK(hipStreamSynchronize(sm));  // the last loss has landed in `st`
   if (stats) {
     TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));

@@ -84,7 +84,7 @@ void th_guided_free(th_guided_ws* w);
 int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const float4* vp,
                     const float4* vn, const float4* vw, const float* bmin, const float* bmax, float* O,
                     hipStream_t st);
-int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count,
-                    float* loss_out);
+int32_t guide_last_loss(th_guide* g, float* out);
+int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count);
 uint32_t guide_steps(const th_guide* g);
 th_context* guide_context(const th_guide* g);
