@@ -86,16 +86,16 @@ def mfma_selftest(ctx: GpuContext, a, b):
 
 
 # Defaults measured on the indirect-lit terrain (1280x720, 262,144 triangles): guide
-# probability 0.8, environment light by the continuation alone at guided vertices, a
-# warm-up of 768 Adam steps at 16 steps per pass (48 passes) with alpha ramped over the
-# first 512, then one step on every 3rd pass.
-GUIDED_DEFAULTS = dict(alpha=0.8, env_nee=False, warmup_steps=768, warmup_rate=16, alpha_ramp=512,
-                       train_every=3)
+# probability 0.8, environment light by the continuation alone at guided vertices, Adam
+# steps of 2^15 samples, a warm-up of 768 steps at 16 steps per pass (48 passes) with
+# alpha ramped over the first 512, then one step every pass.
+GUIDED_DEFAULTS = dict(alpha=0.8, env_nee=False, batch=1 << 15, warmup_steps=768, warmup_rate=16,
+                       alpha_ramp=512, train_every=1)
 
 
 def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.8, train=True, steps_per_pass=1,
-                  batch=1 << 18, progressive=True, env_nee=False, warmup_steps=768, warmup_rate=16,
-                  alpha_ramp=512, train_every=3):
+                  batch=1 << 15, progressive=True, env_nee=False, warmup_steps=768, warmup_rate=16,
+                  alpha_ramp=512, train_every=1):
     """`samples` passes of one sample per pixel with the guide (see th_render_guided);
     with progressive=True the running average and frame seed continue like
     GpuScene.render_progressive.  Returns the call's statistics as a dict."""
