@@ -47,9 +47,6 @@ __device__ __forceinline__ float oswap_f(float v) {
 }
 
 // ------------------------------------------------------------------ node fetch
-#ifndef TH_NODE_BUFFER_LOAD
-#define TH_NODE_BUFFER_LOAD 1
-#endif
 // Buffer resource over the node4 array (raw, 32-bit offsets; word 3 = 0x00020000, the
 // gfx9 raw-buffer format).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(const Child4* nodes4, uint32_t bytes) {
@@ -156,9 +153,7 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
   float te;
   *best_t = TH_FLT_MAX;
   *best_slot = THD_NO_SLOT;
-#if TH_NODE_BUFFER_LOAD
   const __amdgpu_buffer_rsrc_t nrs = node_rsrc(nodes4, S.node4_bytes);
-#endif
   if (!active || !box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0],
                          S.root_max[1], S.root_max[2], r, best, &te))
     return false;
@@ -202,15 +197,11 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
         }
       }
     } else {
-#if TH_NODE_BUFFER_LOAD
       // LDS (top levels) or a buffer load: two distinct load forms, so the compiler
       // cannot merge them into one flat load that waits on both memory counters
       Child4 c;
       if (!(e < cached)) c = load_child(nrs, (4u * e + lq) * (uint32_t)sizeof(Child4));
       else c = lds_nodes[4u * e + lq];
-#else
-      Child4 c = e < cached ? lds_nodes[4u * e + lq] : nodes4[4u * e + lq];
-#endif
       th_keep3(c.b.x, c.b.y, c.b.z);  // both halves of the child in one round trip
       uint32_t ce = __float_as_uint(c.a.w);
       float tE = INF;
