@@ -293,6 +293,11 @@ typedef struct th_stats {
 } th_stats;
 
 int32_t th_device_count(int32_t* count);
+/* Identity of a device for benchmark reports: "<name> (<architecture>)" and the HIP
+ * runtime and driver versions, each NUL-terminated and truncated to its capacity.
+ * Stands in for the adapter information the reference reports (wgpu AdapterInfo,
+ * crates/toaster-cli/src/benchmark.rs:64-76: name, backend, device type, driver). */
+int32_t th_device_info(int32_t device, char* name, size_t name_cap, char* driver, size_t driver_cap);
 int32_t th_context_create(int32_t device, th_context** out);
 /* Scenes and guides of a context must be destroyed before the context. */
 void th_context_destroy(th_context* ctx);

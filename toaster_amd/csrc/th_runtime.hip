@@ -755,6 +755,18 @@ int32_t th_device_count(int32_t* count) {
   return TH_OK;
 }
 
+int32_t th_device_info(int32_t device, char* name, size_t name_cap, char* driver, size_t driver_cap) {
+  if (!name || !driver || name_cap == 0 || driver_cap == 0) return th_set_error(TH_E_INVALID, "null argument");
+  hipDeviceProp_t prop;
+  TH_CHECK(hipGetDeviceProperties(&prop, device));
+  int rt = 0, drv = 0;
+  TH_CHECK(hipRuntimeGetVersion(&rt));
+  TH_CHECK(hipDriverGetVersion(&drv));
+  snprintf(name, name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+  snprintf(driver, driver_cap, "HIP runtime %d, driver %d", rt, drv);
+  return TH_OK;
+}
+
 int32_t th_context_create(int32_t device, th_context** out) {
   if (!out) return th_set_error(TH_E_INVALID, "null argument");
   *out = nullptr;
