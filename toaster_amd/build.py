@@ -13,6 +13,9 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtoaster_hip.so")
+ROOT = os.path.dirname(HERE)
+EXAMPLE_SRC = os.path.join(HERE, "examples", "cornell_c.cpp")
+EXAMPLE_BIN = os.path.join(HERE, "examples", "cornell_c")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 HIP_FLAGS = [
@@ -60,6 +63,9 @@ def build(verbose=False, defines=(), lib_path=LIB, jobs=None):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-lpthread"])
     os.replace(tmp, lib_path)
+    if lib_path == LIB:  # the C caller example links the in-tree library
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), EXAMPLE_SRC,
+              "-L", HERE, "-ltoaster_hip", "-Wl,-rpath,$ORIGIN/..", "-o", EXAMPLE_BIN])
     if verbose:
         print("built", lib_path)
     return lib_path
