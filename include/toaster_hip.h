@@ -267,11 +267,17 @@ typedef struct th_render_options {
  * TH_SCHEDULE_OCTET: eight lanes per path: one quad traces the path while the other
  *   traces the shadow rays of the path's previous vertex; one 4x2 pixel tile per wave
  *   (reference RNG), or G = min(samples, 8) octets per pixel (per-sample RNG).
+ * TH_SCHEDULE_SPECULATIVE: reference RNG: two octets per pixel; one traces the pixel's
+ *   next sample, the other the sample after it from the start of the pixel's random
+ *   stream that follows a full-length path; kept when it equals the first sample's end
+ *   state, restarted otherwise (per-sample RNG: as OCTET).  Faster where a frame leaves
+ *   lanes idle (long per-pixel chains), slower where every pixel is busy.
  * All schedules give the same image for the same RNG mode (per-sample RNG: the same
  * group count G, i.e. samples < 16). */
 #define TH_SCHEDULE_STATIC 0
 #define TH_SCHEDULE_DYNAMIC 1
 #define TH_SCHEDULE_OCTET 2
+#define TH_SCHEDULE_SPECULATIVE 3
 
 typedef struct th_stats {
     double last_kernel_ms;    /* device time of the last th_render (hipEvents)    */

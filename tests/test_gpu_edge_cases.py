@@ -2,8 +2,8 @@
 and one-row images, leaves beyond the 62 primitives a packed entry counts (coincident
 centroids; the root itself a leaf), exact ties between primitives, a textured material
 (bilinear sRGB texels, texture.rs:53-92 / wgsl:233-270).  Each runs through the scalar
-megakernel (reference and ordered traversal), the 4-lane and octet megakernels and the
-wavefront.  Tolerance as tests/test_gpu_parity.py."""
+megakernel (reference and ordered traversal), the 4-lane, octet and speculative
+megakernels and the wavefront.  Tolerance as tests/test_gpu_parity.py."""
 import numpy as np
 import pytest
 
@@ -20,6 +20,7 @@ KERNELS = {
     "scalar": RenderOptions(traversal=A.TH_TRAVERSE_ORDERED),
     "quad": RenderOptions(traversal=A.TH_TRAVERSE_WIDE),
     "octet": RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_OCTET),
+    "speculative": RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_SPECULATIVE),
     "wavefront": RenderOptions(traversal=A.TH_TRAVERSE_WIDE, kernel=A.TH_KERNEL_WAVEFRONT),
 }
 

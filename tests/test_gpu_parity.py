@@ -52,7 +52,7 @@ def test_image_matches_oracle(gpu_ctx, name, make, traversal):
     assert frac >= TOL_FRAC
 
 
-@pytest.mark.parametrize("schedule", [A.TH_SCHEDULE_DYNAMIC, A.TH_SCHEDULE_OCTET])
+@pytest.mark.parametrize("schedule", [A.TH_SCHEDULE_DYNAMIC, A.TH_SCHEDULE_OCTET, A.TH_SCHEDULE_SPECULATIVE])
 @pytest.mark.parametrize("name,make", [
     ("cornell", lambda: scenes.cornell()),
     ("stress2048", lambda: scenes.bvh_stress(2048)),
@@ -61,8 +61,9 @@ def test_image_matches_oracle(gpu_ctx, name, make, traversal):
 ])
 def test_schedule_matches_oracle(gpu_ctx, name, make, schedule):
     """The 4-lane kernel with other pixel-to-lane assignments (the default is one static
-    tile per wave): persistent quads taking pixels from a counter, or eight lanes per
-    pixel with the shadow rays traced beside the path.  Same per-pixel work, same image."""
+    tile per wave): persistent quads taking pixels from a counter, eight lanes per pixel
+    with the shadow rays traced beside the path, or a pixel's samples traced at once from
+    guessed starts of its random stream.  Same per-pixel results, same image."""
     sc = make()
     print(f"case {name} wide schedule={schedule}", flush=True)
     img = _render(gpu_ctx, sc, RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=schedule))
@@ -79,23 +80,32 @@ def test_schedule_matches_oracle(gpu_ctx, name, make, schedule):
     ("stress8192_odd_size", lambda: scenes.bvh_stress(8192, width=123, height=77, samples=3)),
     ("mixed_env", lambda: scenes.mixed_environment(width=160, height=90, samples=5, max_bounces=8,
                                                    env_size=(256, 128))),
+    ("mixed_env_16spp", lambda: scenes.mixed_environment(width=64, height=48, samples=16, max_bounces=8,
+                                                         env_size=(256, 128))),
+    ("stress2048_9spp", lambda: scenes.bvh_stress(2048, width=96, height=64, samples=9)),
+    ("stress2048_2spp_0_bounces", lambda: scenes.bvh_stress(2048, width=64, height=48, samples=2,
+                                                            max_bounces=0)),
 ])
 def test_octet_is_bit_identical_to_quad(gpu_ctx, name, make):
     """The octet kernel traces a vertex's shadow rays beside the next bounce (and a
-    sample's last shadow rays beside the next sample's primary ray) but adds every term in
-    the reference's order: its image equals the 4-lane kernel's bit for bit, also when it
-    folds into an accumulated image."""
+    sample's last shadow rays beside the next sample's primary ray), the speculative
+    kernel traces a pixel's samples at once from guessed starts of its random stream and
+    restarts the mispredicted ones; both add every term in the reference's order, so their
+    images equal the 4-lane kernel's bit for bit, also when folded into an accumulated
+    image.  The cases cover guesses that hold (diffuse paths) and guesses that fail
+    (metal, glass, escaped and zero-bounce paths), and 1 to 16 samples."""
     sc = make()
-    print(f"case octet vs quad {name}", flush=True)
+    print(f"case octet/speculative vs quad {name}", flush=True)
     gs = GpuScene.from_scene(gpu_ctx, sc)
     out = []
-    for sched in (A.TH_SCHEDULE_STATIC, A.TH_SCHEDULE_OCTET):
+    for sched in (A.TH_SCHEDULE_STATIC, A.TH_SCHEDULE_OCTET, A.TH_SCHEDULE_SPECULATIVE):
         o = RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=sched)
         gs.render(options=o, frame_index=0, accumulated_samples=0)
         gs.render(options=o, frame_index=1, accumulated_samples=sc.samples)
         out.append(gs.readback().copy())
     gs.close()
     assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+    assert np.array_equal(out[0].view(np.uint32), out[2].view(np.uint32))
 
 
 @pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_ORDERED, A.TH_SCHEDULE_STATIC),

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_REFERENCE, A.TH_SCHEDULE_STATIC),
                                                 (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_STATIC),
-                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET)])
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_SPECULATIVE)])
 @pytest.mark.parametrize("index", range(7))
 def test_suite_scene_matches_oracle(gpu_ctx, index, traversal, schedule):
     sc = load_scene(suite_paths()[index])

@@ -461,22 +461,50 @@ struct OctetPath {
     ca = ca + 1.0f;
   }
 
+  // the current sample: ray, throughput, radiance so far, bounce index
+  v3 o, d, throughput, radiance;
+  bool include_emissive = true, prev_diffuse = false;
+  float prev_pdf = 0.0f;
+  uint32_t i = 0;
+
+  // Starts a sample from the camera ray (o, d).
+  __device__ __forceinline__ void begin(const v3& o0, const v3& d0) {
+    o = o0;
+    d = d0;
+    throughput = th::mk(1, 1, 1);
+    radiance = th::mk(0, 0, 0);
+    include_emissive = true;
+    prev_diffuse = false;
+    prev_pdf = 0.0f;
+    i = 0;
+  }
+
+  // Drops the current sample and any pending shadow rays (the pixel sums are kept).
+  __device__ void abandon() { pd = pl = pe = carry = false; }
+
   // One sample from the camera ray (o, d); `last`: no sample follows in this launch.
-  __device__ void sample(v3 o, v3 d, uint32_t* s, bool last) {
+  __device__ __forceinline__ void sample(const v3& o0, const v3& d0, uint32_t* s, bool last) {
+    begin(o0, d0);
+    while (step(s, last)) {
+    }
+  }
+
+  // One vertex of the current sample (the extension ray and the previous vertex's
+  // shadow rays in one traversal, then shading); false once the sample has ended (its
+  // radiance emitted, or carried to the next sample when `last` is false).
+  __device__ __forceinline__ bool step(uint32_t* s, bool last) {
     const DevScene& S = it.S;
     const TH_GpuRenderParams& P = it.P;
     const v3 zero = th::mk(0, 0, 0);
-    v3 throughput = th::mk(1, 1, 1);
-    v3 radiance = zero;
-    bool include_emissive = true, prev_diffuse = false;
-    float prev_pdf = 0.0f;
-    for (uint32_t i = 0;; ++i) {
+    bool more = false, emits = true;
+    do {  // one vertex; `break` ends the sample
       bool ext = i < P.max_bounces;
       if (!ext && !pd) break;
       if (!ext && !last) {  // trace this vertex's shadow rays with the next primary ray
         carry = true;
         crad = radiance;
-        return;
+        emits = false;
+        break;
       }
       // the shadow quad takes the light's shadow ray, else the environment's; a second
       // step traces the environment's when both are pending
@@ -576,8 +604,11 @@ struct OctetPath {
       throughput = th::mul(throughput, att);
       o = rec.p;
       d = nd;
-    }
-    emit(radiance);
+      ++i;
+      more = true;
+    } while (false);
+    if (!more && emits) emit(radiance);
+    return more;
   }
 };
 

@@ -338,6 +338,156 @@ __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_oct
   }
 }
 
+// Reference RNG, speculative: G octets per pixel.  A pixel's samples share one random
+// stream (each draw replaces the state by pcg_hash of it), so sample k starts in the state
+// where sample k-1 ended, and a sample is a function of its start state alone.  One octet
+// (the lead) traces the pixel's next sample from its exact start; the others trace the
+// sample after it from candidate starts: the state `2 + per_vertex * b` draws after the
+// lead's start, i.e. the start that follows a lead whose path ends after b diffuse
+// vertices, for b = max_bounces, max_bounces - 1, ...  When the lead ends, its radiance is
+// added to the pixel (samples in order, so the image equals the sequential kernels' bit for
+// bit); a candidate whose start equals the lead's end state is exactly the reference's
+// next sample and becomes the lead, and every other octet restarts as a candidate for the
+// sample after that.  With no match the next sample restarts from the known state, no
+// later than the sequential order would start it.  The dispatch runs G = 2: a single
+// candidate for a full-length lead; more candidates multiply the work past what the
+// frame's idle lanes absorb.
+#define TH_SPEC_LEAD 0u
+#define TH_SPEC_CAND 1u
+#define TH_SPEC_IDLE 2u
+
+template <bool ENV, int G>
+__global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRenderParams P, TH_GpuCamera cam,
+                                                        float4* __restrict__ accum,
+                                                        const float* __restrict__ srgb_lut, uint32_t cached,
+                                                        uint32_t per_vertex) {
+  extern __shared__ uint2 lds_raw[];
+  Child4* lds_nodes = (Child4*)lds_raw;
+  for (uint32_t i = threadIdx.x; i < 4u * cached; i += blockDim.x)
+    lds_nodes[i] = ((const Child4*)S.nodes4)[i];
+  __syncthreads();
+  uint2* lds_stack = lds_raw + 4u * cached * (sizeof(Child4) / sizeof(uint2));
+  constexpr uint32_t PPW = 8u / G;
+  constexpr uint32_t TW = PPW >= 4u ? 4u : (PPW >= 2u ? 2u : 1u);
+  constexpr uint32_t TH = PPW / TW;
+  const uint32_t lane = threadIdx.x & 63u, ow = lane >> 3;
+  const bool shadow = ((lane >> 2) & 1u) != 0u;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t pw = ow / G, j = ow % G;
+  const uint32_t gl = pw * 8u * G;  // first lane of this pixel's octets
+  const uint32_t tiles_x = (P.width + TW - 1u) / TW;
+  const uint32_t x = (wave % tiles_x) * TW + pw % TW;
+  const uint32_t y = (wave / tiles_x) * TH + pw / TW;
+  const bool active = x < P.width && y < P.height;
+  Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
+  LaneCounters<false> lc;
+  Integrator<3, false> it{S, P, srgb_lut, st, lc, lds_nodes, cached};
+  const uint32_t idx = y * P.width + x;
+  OctetPath<ENV, 3, false> path{it, shadow};
+  const uint32_t n = P.samples, B = P.max_bounces;
+  float pr = 0.0f, pg = 0.0f, pb = 0.0f, pa = 0.0f;  // pixel sums in sample order
+  if (active && n > 0u) {
+    uint32_t role = TH_SPEC_IDLE, start = 0u, s = 0u;
+    bool begun = false, fin = false;
+    uint32_t next = 0u, lq = 0u;  // the lead's sample and octet
+    // (re)assigns the octets for lead octet lq starting sample `next` at state e: the lead
+    // restarts unless `keep`; the others become candidates for sample next + 1
+    auto assign = [&](uint32_t e, bool keep) {
+      uint32_t cand[G > 1 ? G - 1 : 1];
+      {
+        // one walk of the stream from e records the candidate starts: rank r takes
+        // b = B - r diffuse vertices (ranks with b < 0 stay idle)
+        uint32_t h = e;
+        const uint32_t len = 2u + per_vertex * B;
+#pragma unroll
+        for (int r = 0; r < G - 1; ++r) cand[r] = 0u;
+        for (uint32_t t = 1u; t <= len; ++t) {
+          h = th::pcg_hash(h);
+#pragma unroll
+          for (int r = 0; r < G - 1; ++r)
+            if ((uint32_t)r <= B && t == 2u + per_vertex * (B - (uint32_t)r)) cand[r] = h;
+        }
+      }
+      if (j == lq) {
+        role = TH_SPEC_LEAD;
+        if (!keep) {
+          start = s = e;
+          begun = fin = false;
+          path.abandon();
+        }
+      } else {
+        const uint32_t r = j < lq ? j : j - 1u;  // rank among the other octets
+        bool use = false;
+        uint32_t c = 0u;
+#pragma unroll
+        for (int q = 0; q < G - 1; ++q)
+          if ((uint32_t)q == r) {
+            use = (uint32_t)q <= B && next + 1u < n;
+            c = cand[q];
+          }
+        role = use ? TH_SPEC_CAND : TH_SPEC_IDLE;
+        start = s = c;
+        begun = fin = false;
+        path.abandon();
+      }
+    };
+    assign(th::pcg_hash(idx ^ (P.frame_index * 9781u)), false);
+    for (;;) {
+      if (role != TH_SPEC_IDLE && !fin) {
+        if (!begun) {
+          v3 o, d;
+          camera_ray(cam, P, x, y, &s, &o, &d);
+          path.begin(o, d);
+          path.cr = path.cg = path.cb = path.ca = 0.0f;
+          begun = true;
+        }
+        if (!path.step(&s, true)) fin = true;
+      }
+      bool done = false;
+      // the lead has ended: add its radiance, hand the lead to the matching candidate
+      // (which may have ended too: then add it at once, and so on)
+      while (__shfl((int)fin, (int)(gl + 8u * lq), 64) != 0) {
+        const int src = (int)(gl + 8u * lq);
+        const uint32_t e = (uint32_t)__shfl((int)s, src, 64);
+        const float lr = __shfl(path.cr, src, 64), lg = __shfl(path.cg, src, 64), lb = __shfl(path.cb, src, 64);
+        pr = pr + lr;
+        pg = pg + lg;
+        pb = pb + lb;
+        pa = pa + 1.0f;
+        if (++next >= n) {
+          done = true;
+          break;
+        }
+        const bool mine = role == TH_SPEC_CAND && start == e;
+        uint32_t nl = lq;
+        bool found = false;
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (!found && __shfl((int)mine, (int)(gl + 8u * (uint32_t)q), 64) != 0) {
+            found = true;
+            nl = (uint32_t)q;
+          }
+        lq = nl;
+        assign(e, found);
+      }
+      if (done) break;
+    }
+  }
+  if (active && (lane % (8u * G)) == 0u) {
+    float ns = (float)P.samples;
+    float4 b = make_float4(pr / ns, pg / ns, pb / ns, pa / ns);
+    if (P.accumulated_samples == 0u) {
+      accum[idx] = b;
+    } else {
+      float prev = (float)P.accumulated_samples;
+      float tot = prev + ns;
+      float4 o = accum[idx];
+      accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
+                               (o.z * prev + b.z * ns) / tot, (o.w * prev + b.w * ns) / tot);
+    }
+  }
+}
+
 // 4-wide nodes the quad and octet kernels copy into LDS: the top four levels.
 #ifndef TH_QUAD_LDS_NODES
 #define TH_QUAD_LDS_NODES 0u
@@ -935,7 +1085,7 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   th_render_options o;
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
-  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 2u)
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 3u)
     return th_set_error(TH_E_INVALID, "unknown render option");
   if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
     return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
@@ -981,7 +1131,29 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_work, 1u);
       hipLaunchKernelGGL(k_render_quad_dyn, dim3(std::max(pblocks, 1u)), blk, lds4, ctx->stream, S, *p, *cam,
                          s->d_accum, ctx->d_srgb_lut, cached, ctx->d_work);
-    } else if (o.schedule == TH_SCHEDULE_OCTET) {
+    } else if (o.rng_mode == TH_RNG_REFERENCE && o.schedule == TH_SCHEDULE_SPECULATIVE) {
+      // two octets per pixel (one with a single sample); the draws per diffuse vertex:
+      // 2 (BSDF), 3 (light pick and point) with lights, 3 (environment pick and jitter)
+      // with an environment
+      const int GS = p->samples >= 2 ? 2 : 1;
+      const bool env = p->background_kind == 2u;
+      const uint32_t per_vertex = 2u + ((p->light_count > 0u && p->total_light_weight > 0.0f) ? 3u : 0u) +
+                                  (env && p->environment_width * p->environment_height > 0u ? 3u : 0u);
+      uint32_t sppw = 8u / (uint32_t)GS;
+      uint32_t stw = sppw >= 4 ? 4 : (sppw >= 2 ? 2 : 1), sth = sppw / stw;
+      uint32_t swaves = ((p->width + stw - 1) / stw) * ((p->height + sth - 1) / sth);
+      dim3 sgrid((swaves + 3u) / 4u);
+#define TH_LAUNCHS(E, GG)                                                                               \
+  hipLaunchKernelGGL((k_render_spec<E, GG>), sgrid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, \
+                     ctx->d_srgb_lut, cached, per_vertex)
+      switch (GS * (env ? -1 : 1)) {
+        case 2: TH_LAUNCHS(false, 2); break;
+        case 1: TH_LAUNCHS(false, 1); break;
+        case -2: TH_LAUNCHS(true, 2); break;
+        default: TH_LAUNCHS(true, 1); break;
+      }
+#undef TH_LAUNCHS
+    } else if (o.schedule == TH_SCHEDULE_OCTET || o.schedule == TH_SCHEDULE_SPECULATIVE) {
       // octets per pixel: 1 with the reference RNG, else min(samples, 8) as a power of two
       int GO = o.rng_mode == TH_RNG_REFERENCE ? 1 : std::min(G, 8);
       uint32_t oppw = 8u / (uint32_t)GO;
