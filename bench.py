@@ -26,7 +26,8 @@ that, averaged over the five runs.
 
 reference_suite times the reference's own benchmark scenes (scenes/benchmarks/001-007)
 as its benchmark mode does; batched renders 16 scenes x 4 frames in one persistent
-launch beside one launch per frame.
+launch beside one launch per frame; scene_update times a frame of moved geometry on the
+headline scene, rebuilt and uploaded as the reference does against refitted on the GPU.
 """
 import argparse
 import json
@@ -87,6 +88,60 @@ def cpu_baseline(sc, budget_s=8.0):
         "single_thread_mpaths_s": round(sc.paths_per_frame / dt1 / 1e6, 4),
         "cpu_model": model, "nproc": os.cpu_count(),
     }, counters
+
+
+def scene_update_leg(ctx, sc, frames=10):
+    """Moved geometry per frame on the headline scene (every other triangle rotated about
+    the cloud's centre, the light shifted): the reference's way (host repack with a new
+    median-split BVH + upload of every array, path_tracer.rs:639-720) against a device
+    refit of the uploaded tree (host repack without BVH + refit), median ms over
+    `frames` frames, each followed by one render of the moved frame."""
+    import dataclasses
+    import numpy as np
+    from toaster_amd import GpuScene, RenderOptions, _abi as A
+    from toaster_amd.scene import pack
+    opts = RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_SPECULATIVE)
+    v0 = sc.triangles["v"].astype(np.float64)
+    c = v0.reshape(-1, 3).mean(axis=0)
+    sel = np.arange(len(v0)) % 2 == 0
+
+    def frame(k):
+        a = 0.02 * k
+        R = np.array([[np.cos(a), 0.0, np.sin(a)], [0.0, 1.0, 0.0], [-np.sin(a), 0.0, np.cos(a)]])
+        v = v0.copy()
+        v[sel] = (v[sel] - c) @ R.T + c
+        tris = sc.triangles.copy()
+        tris["v"] = v.astype(np.float32)
+        sph = sc.spheres.copy()
+        sph["center"] = sph["center"] + np.float32(0.01 * k)
+        return dataclasses.replace(sc, triangles=tris, spheres=sph)
+
+    moved = [frame(k) for k in range(1, frames + 1)]
+    out = {}
+    everything = (A.TH_CHANGE_CAMERA | A.TH_CHANGE_SPHERES | A.TH_CHANGE_TRIANGLES | A.TH_CHANGE_LIGHTS
+                  | A.TH_CHANGE_BVH | A.TH_CHANGE_MATERIALS)
+    for mode in ("rebuild", "refit"):
+        gs = GpuScene.from_scene(ctx, sc)
+        upd, ren = [], []
+        for m in moved:
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            if mode == "rebuild":
+                gs.update(pack(m), everything)
+            else:
+                gs.refit(m)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            gs.render(options=opts, frame_index=0, accumulated_samples=0)
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            upd.append((t1 - t0) * 1e3)
+            ren.append((t2 - t1) * 1e3)
+        out[mode] = {"update_ms": round(float(np.median(upd)), 3), "render_ms": round(float(np.median(ren)), 4)}
+        gs.close()
+    out["workload"] = (f"{len(v0)} triangles, every other one rotated per frame, {frames} frames, "
+                       "render with the speculative kernel")
+    return out
 
 
 def reference_suite(warmup=5, runs=21):
@@ -385,6 +440,7 @@ def main():
         kms = vs["total_kernel_ms"] / max(1, vs["launches"])
         variants[name] = {"kernel_ms": round(kms, 4), "mpaths_s": round(sc.paths_per_frame / kms / 1e3, 2)}
     gs.close()
+    scene_update = scene_update_leg(ctx, sc)
     ctx.close()
     dt = t1 - t0
     paths = sc.paths_per_frame
@@ -427,6 +483,7 @@ def main():
         "kernel_ms_mean": round(kernel_ms, 5),
         "roofline": roofline,
         "variants": variants,
+        "scene_update": scene_update,
         "equal_time": equal_time,
         "reference_suite": suite,
         "batched": batched,

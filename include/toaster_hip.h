@@ -209,7 +209,12 @@ typedef struct th_packed_scene th_packed_scene; /* opaque, owns the arrays */
  * triangles, attributes, texture atlas, materials (371-404), power-weighted light
  * list (412-475), median-split BVH (296-340 + crates/toaster-bvh) and the
  * environment importance CDF (environment.rs:245-278 via 231-246).
- * include_static_pixels = 0 mirrors scene_to_gpu_frame (texture/env omitted). */
+ * include_static_pixels: bit 0 set packs texels and environment pixels (0 mirrors
+ * scene_to_gpu_frame: texture/env omitted); TH_PACK_NO_BVH leaves the BVH arrays empty,
+ * for geometry that moved in a scene refitted on the device (TH_CHANGE_REFIT), which the
+ * reference does not have (it rebuilds; README.md:159-166 names refitting as its next
+ * optimisation). */
+#define TH_PACK_NO_BVH 2
 int32_t th_pack_scene(const th_scene_input* scene, int32_t include_static_pixels,
                       th_packed_scene** out);
 int32_t th_packed_view(const th_packed_scene* packed, th_scene_desc* out);
@@ -321,6 +326,16 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* desc, uint32_t nod
 #define TH_CHANGE_LIGHTS 8u
 #define TH_CHANGE_BVH 16u
 #define TH_CHANGE_MATERIALS 32u
+/* With TH_CHANGE_SPHERES and/or TH_CHANGE_TRIANGLES: keep the uploaded tree's primitive
+ * order and leaf ranges and recompute every node box on the device from the new
+ * primitives (the packer's bounds: triangle vertex min/max -/+ 1e-5, sphere center -/+
+ * radius; unions in the packer's order, so unmoved geometry refits to the same boxes bit
+ * for bit).  The descriptor's BVH arrays are not read (pack with TH_PACK_NO_BVH).  No
+ * counterpart in the reference, which rebuilds the tree whenever geometry moves
+ * (docs/architecture.md:118-121); exclusive with TH_CHANGE_BVH.  Closest hits equal a
+ * rebuilt tree's except where two primitives tie exactly in distance (the reference's
+ * tie order follows the tree). */
+#define TH_CHANGE_REFIT 64u
 int32_t th_scene_update(th_scene* scene, uint32_t change_mask, const th_scene_desc* desc);
 void th_scene_destroy(th_scene* scene);
 

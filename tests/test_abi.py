@@ -91,6 +91,20 @@ def test_packing_rejects_invalid_scenes():  # scene_upload.rs:84-92, 226-228
         pack(scene([diffuse((1, 1, 1))], sph=[((0, 0, -2), 1.0, 0)], background=A.TH_BG_ENVIRONMENT))
 
 
+def test_packing_without_bvh_for_a_refit():
+    """TH_PACK_NO_BVH: the same records and light list, no tree (for a device refit)."""
+    sc = scenes.bvh_stress(512, width=32, height=16)
+    full, bare = pack(sc), pack(sc, bvh=False)
+    assert len(full.nodes) > 0 and len(bare.nodes) == 0 and len(bare.prims) == 0
+    assert bare.params.bvh_node_count == 0 and bare.params.bvh_primitive_count == 0
+    for a, b in ((full.triangles, bare.triangles), (full.spheres, bare.spheres), (full.lights, bare.lights),
+                 (full.materials, bare.materials)):
+        assert a.tobytes() == b.tobytes()
+    inp, keep = sc.input()
+    h = C.c_void_p()
+    assert A.lib().th_pack_scene(C.byref(inp), 4, C.byref(h)) == A.TH_E_INVALID  # unknown flag
+
+
 def test_null_arguments_fail_before_device_work():
     L = A.lib()
     assert L.th_pack_scene(None, 1, None) == A.TH_E_INVALID

@@ -35,6 +35,8 @@ TH_CHANGE_TRIANGLES = 4
 TH_CHANGE_LIGHTS = 8
 TH_CHANGE_BVH = 16
 TH_CHANGE_MATERIALS = 32
+TH_CHANGE_REFIT = 64
+TH_PACK_NO_BVH = 2
 
 u32, f32 = C.c_uint32, C.c_float
 

@@ -61,6 +61,11 @@ struct th_scene {
   uint32_t root_entry = 0;
   float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
   uint32_t max_depth = 0;
+  // refit: reference node indices deepest level first (refit_level_end[k] ends level k of
+  // that order) and, per 4-wide child slot, its reference node (0xffffffff: none)
+  uint32_t* d_refit_order = nullptr;
+  std::vector<uint32_t> refit_level_end;
+  uint32_t* d_wide_src = nullptr;
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
   th_guided_ws* gw = nullptr;  // allocated by the first guided render
   th_tuning tuning = {0, 0, 0.0f, 0};

@@ -345,9 +345,11 @@ int32_t th_build_bvh(const TH_GpuSphere* spheres, uint32_t ns, const TH_GpuTrian
   return TH_OK;
 }
 
-int32_t th_pack_scene(const th_scene_input* in, int32_t include_static, th_packed_scene** out) {
+int32_t th_pack_scene(const th_scene_input* in, int32_t flags, th_packed_scene** out) {
   if (!in || !out) return th_set_error(TH_E_INVALID, "null argument");
   *out = nullptr;
+  if (flags & ~(1 | TH_PACK_NO_BVH)) return th_set_error(TH_E_INVALID, "unknown packing flags");
+  const bool include_static = (flags & 1) != 0, with_bvh = (flags & TH_PACK_NO_BVH) == 0;
   if (in->sphere_count == 0 && in->triangle_count == 0)
     return th_set_error(TH_E_INVALID, "GPU renderer requires at least one object");
   if (in->material_count == 0 || !in->materials)
@@ -504,7 +506,7 @@ int32_t th_pack_scene(const th_scene_input* in, int32_t include_static, th_packe
     L.area_weight_cumulative[2] = cumulative;
     s->lights.push_back(L);
   }
-  build_bvh(s->spheres.data(), ns, s->triangles.data(), nt, s->nodes, s->prims);
+  if (with_bvh) build_bvh(s->spheres.data(), ns, s->triangles.data(), nt, s->nodes, s->prims);
   // environment (scene_upload.rs:214-246, environment.rs:42-88)
   uint32_t ew = 0, eh = 0;
   float eint = 0.0f, erot = 0.0f;

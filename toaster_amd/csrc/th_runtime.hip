@@ -68,6 +68,71 @@ __global__ void k_prepare_slots(const TH_GpuPrimitiveRef* __restrict__ prims, ui
   slots[3u * i + 2u] = c;
 }
 
+// Refit, one level of the reference tree (deepest first): a leaf's box from its
+// primitives, an interior node's from its two children, as the packer computes them
+// (th_pack.cpp collect_prims / bounds_of): triangle vertex min/max -/+ 1e-5, sphere
+// center -/+ radius, folded with "strictly smaller / larger replaces" in primitive
+// order; left child before right keeps that order.  Topology fields are not touched.
+__device__ __forceinline__ float refit_min(float acc, float x) { return x < acc ? x : acc; }
+__device__ __forceinline__ float refit_max(float acc, float x) { return acc < x ? x : acc; }
+
+__global__ void k_refit_level(TH_GpuBvhNode* __restrict__ nodes, const uint32_t* __restrict__ order,
+                              uint32_t count, const TH_GpuPrimitiveRef* __restrict__ prims,
+                              const TH_GpuSphere* __restrict__ spheres, const TH_GpuTriangle* __restrict__ tris) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t ni = order[i];
+  TH_GpuBvhNode& n = nodes[ni];
+  const float INF = __builtin_inff();
+  float mn[3] = {INF, INF, INF}, mx[3] = {-INF, -INF, -INF};
+  if (n.primitive_count > 0u) {
+    for (uint32_t k = 0; k < n.primitive_count; ++k) {
+      const TH_GpuPrimitiveRef r = prims[n.first_or_right + k];
+      if (r.kind == 1u) {
+        const TH_GpuTriangle& t = tris[r.index];
+        for (int a = 0; a < 3; ++a) {
+          const float lo = refit_min(refit_min(t.v0[a], t.v1[a]), t.v2[a]) - 1e-5f;
+          const float hi = refit_max(refit_max(t.v0[a], t.v1[a]), t.v2[a]) + 1e-5f;
+          mn[a] = refit_min(mn[a], lo);
+          mx[a] = refit_max(mx[a], hi);
+        }
+      } else {
+        const float* cr = spheres[r.index].center_radius;
+        for (int a = 0; a < 3; ++a) {
+          mn[a] = refit_min(mn[a], cr[a] - cr[3]);
+          mx[a] = refit_max(mx[a], cr[a] + cr[3]);
+        }
+      }
+    }
+  } else {
+    const TH_GpuBvhNode& L = nodes[ni + 1u];
+    const TH_GpuBvhNode& R = nodes[n.first_or_right];
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = refit_min(L.min[a], R.min[a]);
+      mx[a] = refit_max(L.max[a], R.max[a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    n.min[a] = mn[a];
+    n.max[a] = mx[a];
+  }
+}
+
+// Refit of the 4-wide records: each child slot takes the box of its reference node and
+// keeps its entry.
+__global__ void k_refit_wide(Child4* __restrict__ nodes4, const uint32_t* __restrict__ src, uint32_t count,
+                             const TH_GpuBvhNode* __restrict__ nodes) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t b = src[i];
+  if (b == THD_NO_CHILD) return;
+  const TH_GpuBvhNode& g = nodes[b];
+  Child4 c = nodes4[i];
+  c.a = make_float4(g.min[0], g.min[1], g.min[2], c.a.w);
+  c.b = make_float4(g.max[0], g.max[1], g.max[2], c.b.w);
+  nodes4[i] = c;
+}
+
 // Pixel of a lane: each 64-lane wave covers one 8x8 tile (coherent primary rays).
 __device__ __forceinline__ bool tile_pixel(uint32_t W, uint32_t H, uint32_t* x, uint32_t* y) {
   uint32_t lane = threadIdx.x & 63u;
@@ -616,7 +681,7 @@ int32_t upload(th_context* ctx, T* dst, const T* src, size_t count) {
 void free_scene_buffers(th_scene* s) {
   void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
                   s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum,
-                  s->d_nodes4};
+                  s->d_nodes4, s->d_refit_order, s->d_wide_src};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -723,8 +788,9 @@ float host_bits_to_float(uint32_t u) {
 }
 
 void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out, uint32_t* root4,
-                uint32_t* depth4) {
+                uint32_t* depth4, std::vector<uint32_t>* src) {
   out.clear();
+  src->clear();
   *depth4 = 0;
   if (n == 0) {
     *root4 = 0;
@@ -762,9 +828,11 @@ void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out
         }
         ch.a = make_float4(g.min[0], g.min[1], g.min[2], host_bits_to_float(entry));
         ch.b = make_float4(g.max[0], g.max[1], g.max[2], 0.0f);
+        src->push_back(kids[k]);
       } else {
         ch.a = make_float4(0, 0, 0, host_bits_to_float(THD_NO_CHILD));
         ch.b = make_float4(0, 0, 0, 0);
+        src->push_back(THD_NO_CHILD);
       }
       out.push_back(ch);
     }
@@ -785,10 +853,36 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
     }
     s->max_depth = bvh_depth(d->bvh_nodes, d->bvh_node_count);
     std::vector<Child4> w;
-    build_wide(d->bvh_nodes, d->bvh_node_count, w, &s->root4, &s->depth4);
+    std::vector<uint32_t> wsrc;
+    build_wide(d->bvh_nodes, d->bvh_node_count, w, &s->root4, &s->depth4, &wsrc);
     s->node4_count = (uint32_t)(w.size() / 4);
     if (!w.empty()) {
       TH_CHECK(hipMemcpy(s->d_nodes4, w.data(), w.size() * sizeof(Child4), hipMemcpyHostToDevice));
+      TH_CHECK(hipMemcpy(s->d_wide_src, wsrc.data(), wsrc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    // refit order: node indices grouped by level, deepest level first (pre-order: a
+    // parent precedes its children, so one pass assigns every level)
+    {
+      const uint32_t nn = d->bvh_node_count;
+      std::vector<uint32_t> level(nn, 0u);
+      for (uint32_t i = 0; i < nn; ++i)
+        if (d->bvh_nodes[i].primitive_count == 0u) {
+          level[i + 1u] = level[i] + 1u;
+          level[d->bvh_nodes[i].first_or_right] = level[i] + 1u;
+        }
+      const uint32_t levels = s->max_depth;
+      std::vector<uint32_t> cnt(levels + 1u, 0u), order(nn);
+      for (uint32_t i = 0; i < nn; ++i) cnt[levels - 1u - level[i]]++;
+      s->refit_level_end.assign(levels, 0u);
+      uint32_t acc = 0;
+      for (uint32_t k = 0; k < levels; ++k) {
+        uint32_t c = cnt[k];
+        cnt[k] = acc;
+        acc += c;
+        s->refit_level_end[k] = acc;
+      }
+      for (uint32_t i = 0; i < nn; ++i) order[cnt[levels - 1u - level[i]]++] = i;
+      TH_CHECK(hipMemcpy(s->d_refit_order, order.data(), nn * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     hipLaunchKernelGGL(k_prepare_cnodes, dim3((s->node_count + 255) / 256), dim3(256), 0, ctx->stream,
                        s->d_nodes, s->node_count, s->d_cnodes);
@@ -798,6 +892,42 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
     hipLaunchKernelGGL(k_prepare_slots, dim3((s->prim_count + 255) / 256), dim3(256), 0, ctx->stream,
                        s->d_prims, s->prim_count, s->d_spheres, s->d_tris, s->d_attrs, s->d_slots);
     TH_CHECK(hipGetLastError());
+  }
+  return TH_OK;
+}
+
+// Device refit of the uploaded tree from the scene's current spheres and triangles:
+// slots, reference nodes level by level, child-pair records, 4-wide records, and the
+// root box the launches take by value (read back: one 24-byte copy).
+int32_t refit_bvh(th_scene* s) {
+  th_context* ctx = s->ctx;
+  hipLaunchKernelGGL(k_prepare_slots, dim3((s->prim_count + 255) / 256), dim3(256), 0, ctx->stream,
+                     s->d_prims, s->prim_count, s->d_spheres, s->d_tris, s->d_attrs, s->d_slots);
+  TH_CHECK(hipGetLastError());
+  uint32_t begin = 0;
+  for (uint32_t end : s->refit_level_end) {
+    if (end > begin)
+      hipLaunchKernelGGL(k_refit_level, dim3((end - begin + 255) / 256), dim3(256), 0, ctx->stream,
+                         s->d_nodes, s->d_refit_order + begin, end - begin, s->d_prims, s->d_spheres,
+                         s->d_tris);
+    begin = end;
+  }
+  TH_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_prepare_cnodes, dim3((s->node_count + 255) / 256), dim3(256), 0, ctx->stream,
+                     s->d_nodes, s->node_count, s->d_cnodes);
+  TH_CHECK(hipGetLastError());
+  if (s->node4_count) {
+    const uint32_t nc = 4u * s->node4_count;
+    hipLaunchKernelGGL(k_refit_wide, dim3((nc + 255) / 256), dim3(256), 0, ctx->stream, s->d_nodes4,
+                       s->d_wide_src, nc, s->d_nodes);
+    TH_CHECK(hipGetLastError());
+  }
+  TH_GpuBvhNode root;
+  TH_CHECK(hipMemcpyAsync(&root, s->d_nodes, sizeof root, hipMemcpyDeviceToHost, ctx->stream));
+  TH_CHECK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; ++k) {
+    s->root_min[k] = root.min[k];
+    s->root_max[k] = root.max[k];
   }
   return TH_OK;
 }
@@ -993,7 +1123,8 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_c
       (rc = dalloc(&s->d_lights, d->light_count)) || (rc = dalloc(&s->d_env, d->environment_pixel_count)) ||
       (rc = dalloc(&s->d_tex, d->texture_pixel_count)) || (rc = dalloc(&s->d_cnodes, cap)) ||
       (rc = dalloc(&s->d_slots, 3 * (size_t)nprim)) || (rc = dalloc(&s->d_accum, npix)) ||
-      (rc = dalloc(&s->d_nodes4, 4 * (size_t)cap))) {
+      (rc = dalloc(&s->d_nodes4, 4 * (size_t)cap)) || (rc = dalloc(&s->d_refit_order, cap)) ||
+      (rc = dalloc(&s->d_wide_src, 4 * (size_t)cap))) {
     free_scene_buffers(s);
     delete s;
     return rc;
@@ -1026,10 +1157,17 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_c
 int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
   if (!s || !d) return th_set_error(TH_E_INVALID, "null argument");
   th_context* ctx = s->ctx;
+  const bool refit = (mask & TH_CHANGE_REFIT) != 0u;
+  if (refit && (mask & TH_CHANGE_BVH))
+    return th_set_error(TH_E_INVALID, "a refit and a new bvh are exclusive");
+  if (refit && !(mask & (TH_CHANGE_SPHERES | TH_CHANGE_TRIANGLES)))
+    return th_set_error(TH_E_INVALID, "a refit needs new spheres or triangles");
+  if (refit && s->node_count == 0)
+    return th_set_error(TH_E_INVALID, "the scene has no bvh to refit");
   // path_tracer.rs:639-653: counts are fixed after setup; nodes within capacity.
   if (d->sphere_count != s->params.sphere_count || d->triangle_count != s->params.triangle_count ||
       d->material_count != s->params.material_count || d->light_count != s->params.light_count ||
-      d->bvh_primitive_count != s->prim_count || d->params.width != s->width ||
+      (!refit && d->bvh_primitive_count != s->prim_count) || d->params.width != s->width ||
       d->params.height != s->height)
     return th_set_error(TH_E_MISMATCH, "scene counts changed after setup");
   if (d->bvh_node_count > s->node_capacity)
@@ -1059,7 +1197,11 @@ int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
     geometry = true;
   }
   s->params = d->params;
-  if (geometry) {
+  if (refit) {
+    s->params.bvh_node_count = s->node_count;
+    s->params.bvh_primitive_count = s->prim_count;
+    if ((rc = refit_bvh(s))) return rc;
+  } else if (geometry) {
     if ((rc = prepare_bvh(s, d))) return rc;
   }
   TH_CHECK(hipStreamSynchronize(ctx->stream));

@@ -125,7 +125,18 @@ class GpuScene:
     def update(self, packed: PackedScene, change_mask):
         d = packed.desc()
         A.check(self._lib.th_scene_update(self._h, int(change_mask), C.byref(d)))
+        if change_mask & A.TH_CHANGE_REFIT:  # the uploaded tree stays: keep its counts
+            packed.params.bvh_node_count = self.packed.params.bvh_node_count
+            packed.params.bvh_primitive_count = self.packed.params.bvh_primitive_count
         self.packed = packed
+
+    def refit(self, scene: Scene):
+        """Moved geometry (same primitives, materials and lights count) without a host
+        BVH rebuild: new records and light list, node boxes recomputed on the device over
+        the uploaded tree.  The reference rebuilds instead (docs/architecture.md:118-121)."""
+        self.update(pack(scene, bvh=False),
+                    A.TH_CHANGE_SPHERES | A.TH_CHANGE_TRIANGLES | A.TH_CHANGE_LIGHTS | A.TH_CHANGE_CAMERA
+                    | A.TH_CHANGE_REFIT)
 
     def params(self, samples=None, frame_index=None, accumulated_samples=None, max_bounces=None):
         p = A.GpuRenderParams.from_buffer_copy(self.packed.params)

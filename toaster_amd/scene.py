@@ -165,12 +165,14 @@ class PackedScene:
         return d
 
 
-def pack(scene: Scene, include_static_pixels=True) -> PackedScene:
-    """scene_to_gpu: validation, camera, records, lights, BVH and environment CDF."""
+def pack(scene: Scene, include_static_pixels=True, bvh=True) -> PackedScene:
+    """scene_to_gpu: validation, camera, records, lights, BVH and environment CDF
+    (bvh=False: no BVH arrays, for a device refit of moved geometry)."""
     L = A.lib()
     inp, keep = scene.input()
     h = C.c_void_p()
-    A.check(L.th_pack_scene(C.byref(inp), 1 if include_static_pixels else 0, C.byref(h)))
+    flags = (1 if include_static_pixels else 0) | (0 if bvh else A.TH_PACK_NO_BVH)
+    A.check(L.th_pack_scene(C.byref(inp), flags, C.byref(h)))
     try:
         d = A.SceneDesc()
         A.check(L.th_packed_view(h, C.byref(d)))
