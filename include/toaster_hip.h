@@ -277,12 +277,17 @@ typedef struct th_render_options {
  *   stream that follows a full-length path; kept when it equals the first sample's end
  *   state, restarted otherwise (per-sample RNG: as OCTET).  Faster where a frame leaves
  *   lanes idle (long per-pixel chains), slower where every pixel is busy.
+ * TH_SCHEDULE_AUTO: reference RNG: the first render of a scene with given samples,
+ *   bounces and background times STATIC, OCTET and SPECULATIVE on that frame (9 extra
+ *   launches; an accumulation being folded into is kept) and uses the fastest until the
+ *   scene's geometry changes (per-sample RNG or other traversals: as STATIC).
  * All schedules give the same image for the same RNG mode (per-sample RNG: the same
  * group count G, i.e. samples < 16). */
 #define TH_SCHEDULE_STATIC 0
 #define TH_SCHEDULE_DYNAMIC 1
 #define TH_SCHEDULE_OCTET 2
 #define TH_SCHEDULE_SPECULATIVE 3
+#define TH_SCHEDULE_AUTO 4
 
 typedef struct th_stats {
     double last_kernel_ms;    /* device time of the last th_render (hipEvents)    */

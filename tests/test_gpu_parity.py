@@ -52,7 +52,8 @@ def test_image_matches_oracle(gpu_ctx, name, make, traversal):
     assert frac >= TOL_FRAC
 
 
-@pytest.mark.parametrize("schedule", [A.TH_SCHEDULE_DYNAMIC, A.TH_SCHEDULE_OCTET, A.TH_SCHEDULE_SPECULATIVE])
+@pytest.mark.parametrize("schedule", [A.TH_SCHEDULE_DYNAMIC, A.TH_SCHEDULE_OCTET, A.TH_SCHEDULE_SPECULATIVE,
+                                      A.TH_SCHEDULE_AUTO])
 @pytest.mark.parametrize("name,make", [
     ("cornell", lambda: scenes.cornell()),
     ("stress2048", lambda: scenes.bvh_stress(2048)),
@@ -106,6 +107,38 @@ def test_octet_is_bit_identical_to_quad(gpu_ctx, name, make):
     gs.close()
     assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
     assert np.array_equal(out[0].view(np.uint32), out[2].view(np.uint32))
+
+
+@pytest.mark.parametrize("name,make", [
+    ("cornell", lambda: scenes.cornell(width=96, height=64)),
+    ("stress2048", lambda: scenes.bvh_stress(2048, width=96, height=64, samples=3)),
+])
+def test_auto_schedule_keeps_the_accumulation(gpu_ctx, name, make):
+    """TH_SCHEDULE_AUTO times the schedules on its first render of the scene (overwriting
+    the accumulation while it does) and must leave exactly what a fixed schedule leaves,
+    also when that first render folds into an accumulated frame; later renders reuse the
+    choice."""
+    sc = make()
+    print(f"case auto schedule {name}", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    static = RenderOptions(traversal=A.TH_TRAVERSE_WIDE)
+    auto = RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_AUTO)
+    out = []
+    for second in (static, auto):
+        gs.render(options=static, frame_index=0, accumulated_samples=0)
+        gs.render(options=second, frame_index=1, accumulated_samples=sc.samples)  # auto: tunes here
+        gs.render(options=second, frame_index=2, accumulated_samples=2 * sc.samples)
+        out.append(gs.readback().copy())
+    gs.close()
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+    # the timed trials stay out of the statistics: one launch per render
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    gpu_ctx.synchronize()
+    gpu_ctx.reset_stats()
+    gs.render(options=auto, frame_index=0, accumulated_samples=0)
+    gs.render(options=auto, frame_index=1, accumulated_samples=sc.samples)
+    assert gpu_ctx.stats()["launches"] == 2
+    gs.close()
 
 
 @pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_ORDERED, A.TH_SCHEDULE_STATIC),

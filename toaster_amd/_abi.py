@@ -24,7 +24,7 @@ TH_BG_SKY, TH_BG_BLACK, TH_BG_ENVIRONMENT = 0, 1, 2
 TH_RNG_REFERENCE, TH_RNG_PER_SAMPLE = 0, 1
 TH_KERNEL_MEGA, TH_KERNEL_WAVEFRONT = 0, 1
 TH_TRAVERSE_REFERENCE, TH_TRAVERSE_ORDERED, TH_TRAVERSE_WIDE = 0, 1, 2
-TH_SCHEDULE_STATIC, TH_SCHEDULE_DYNAMIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULATIVE = 0, 1, 2, 3
+TH_SCHEDULE_STATIC, TH_SCHEDULE_DYNAMIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULATIVE, TH_SCHEDULE_AUTO = 0, 1, 2, 3, 4
 
 TH_GUIDE_FEATURES, TH_GUIDE_OUTPUTS, TH_GUIDE_PARAMS = 48, 40, 14056
 TH_GUIDED_NO_ENV_NEE = 1

@@ -66,6 +66,11 @@ struct th_scene {
   uint32_t* d_refit_order = nullptr;
   std::vector<uint32_t> refit_level_end;
   uint32_t* d_wide_src = nullptr;
+  // TH_SCHEDULE_AUTO: the schedule chosen for (samples, bounces, background), and a copy
+  // of the accumulation kept while the candidates are timed
+  bool auto_valid = false;
+  uint32_t auto_samples = 0, auto_bounces = 0, auto_background = 0, auto_choice = 0;
+  float4* d_accum_save = nullptr;
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
   th_guided_ws* gw = nullptr;  // allocated by the first guided render
   th_tuning tuning = {0, 0, 0.0f, 0};

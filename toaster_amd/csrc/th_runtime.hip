@@ -681,7 +681,7 @@ int32_t upload(th_context* ctx, T* dst, const T* src, size_t count) {
 void free_scene_buffers(th_scene* s) {
   void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
                   s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum,
-                  s->d_nodes4, s->d_refit_order, s->d_wide_src};
+                  s->d_nodes4, s->d_refit_order, s->d_wide_src, s->d_accum_save};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1197,6 +1197,7 @@ int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
     geometry = true;
   }
   s->params = d->params;
+  if (geometry || refit) s->auto_valid = false;  // the fastest schedule may change with the tree
   if (refit) {
     s->params.bvh_node_count = s->node_count;
     s->params.bvh_primitive_count = s->prim_count;
@@ -1218,6 +1219,62 @@ void th_scene_destroy(th_scene* s) {
   delete s;
 }
 
+// TH_SCHEDULE_AUTO (reference RNG, 4-wide megakernel): the first render of a scene with
+// these samples / bounces / background times the static, octet and speculative schedules
+// (same image) on this frame -- one warm-up and two timed launches each, at seed 0 of the
+// accumulation -- and keeps the fastest until the scene's geometry changes.  The trials
+// overwrite the accumulation, so a render that folds into it gets it back first; their
+// launches are left out of th_get_stats.
+static int32_t auto_schedule(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                             th_render_options* o) {
+  if (s->auto_valid && s->auto_samples == p->samples && s->auto_bounces == p->max_bounces &&
+      s->auto_background == p->background_kind) {
+    o->schedule = s->auto_choice;
+    return TH_OK;
+  }
+  int32_t rc = fold_timing(ctx);  // the caller's earlier renders keep their timings
+  if (rc) return rc;
+  const size_t bytes = (size_t)s->width * s->height * sizeof(float4);
+  const bool keep = p->accumulated_samples > 0u;
+  if (keep) {
+    if (!s->d_accum_save) TH_CHECK(hipMalloc((void**)&s->d_accum_save, bytes));
+    TH_CHECK(hipMemcpyAsync(s->d_accum_save, s->d_accum, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  TH_GpuRenderParams tp = *p;
+  tp.accumulated_samples = 0u;
+  const uint32_t cands[3] = {TH_SCHEDULE_STATIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULATIVE};
+  float best = 0.0f;
+  uint32_t choice = TH_SCHEDULE_STATIC;
+  for (int c = 0; c < 3; ++c) {
+    th_render_options t = *o;
+    t.schedule = cands[c];
+    for (int r = 0; r < 3; ++r)
+      if ((rc = th_render(ctx, s, &tp, cam, &t))) return rc;
+    float total = 0.0f;
+    for (size_t i = 0; i < ctx->ev_pending.size(); ++i) {
+      auto& pr = ctx->ev_pending[i];
+      TH_CHECK(hipEventSynchronize(pr.second));
+      float ms = 0.0f;
+      TH_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+      if (i > 0) total += ms;  // the first launch warms up
+      ctx->ev_pool.push_back(pr);
+    }
+    ctx->ev_pending.clear();
+    if (c == 0 || total < best) {
+      best = total;
+      choice = cands[c];
+    }
+  }
+  if (keep) TH_CHECK(hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  s->auto_valid = true;
+  s->auto_samples = p->samples;
+  s->auto_bounces = p->max_bounces;
+  s->auto_background = p->background_kind;
+  s->auto_choice = choice;
+  o->schedule = choice;
+  return TH_OK;
+}
+
 int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
                   const th_render_options* opt) {
   if (!ctx || !s || !cam) return th_set_error(TH_E_INVALID, "null argument");
@@ -1227,8 +1284,16 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   th_render_options o;
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
-  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 3u)
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 4u)
     return th_set_error(TH_E_INVALID, "unknown render option");
+  if (o.schedule == TH_SCHEDULE_AUTO) {
+    if (o.rng_mode == TH_RNG_REFERENCE && o.traversal == TH_TRAVERSE_WIDE && o.kernel == TH_KERNEL_MEGA &&
+        !o.collect_stats && s->node_count > 0) {
+      if ((rc = auto_schedule(ctx, s, p, cam, &o))) return rc;
+    } else {
+      o.schedule = TH_SCHEDULE_STATIC;
+    }
+  }
   if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
     return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
   if (o.kernel == TH_KERNEL_WAVEFRONT && o.collect_stats)
