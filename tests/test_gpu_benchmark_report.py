@@ -38,3 +38,18 @@ def test_suite_report_and_comparison(gpu_ctx):
     assert not cmp["matching_gpu"]
     print("dispatch_wait change vs RTX 2080 Ti: %.1f %%" % cmp["deltas"]["dispatch_wait_percent"])
     assert cmp["deltas"]["dispatch_wait_percent"] < 0.0  # faster than the published run
+
+
+def test_refit_update_gives_the_same_report_image():
+    """update="refit": the per-frame scene update refits the uploaded tree instead of
+    rebuilding it; for the static suite scene the boxes are the built ones, so the image
+    digest equals the rebuild report's, and the update stage is cheaper."""
+    path = suite_paths()[5]  # 006_mixed_2048t_128s: spheres and triangles
+    print("case benchmark report refit 006", flush=True)
+    rebuild = B.benchmark_scene(path, warmup_frames=1, measured_frames=3)
+    refit = B.benchmark_scene(path, warmup_frames=1, measured_frames=3, update="refit")
+    assert refit["image"]["rgba_sha256"] == rebuild["image"]["rgba_sha256"]
+    med = lambda r: r["summary"]["scene_update_upload_ms"]["median"]
+    print("scene update ms: rebuild %.3f, refit %.3f" % (med(rebuild), med(refit)))
+    with pytest.raises(ValueError):
+        B.benchmark_scene(path, update="rotate")

@@ -7,7 +7,9 @@ frame (crates/toaster-gpu/src/path_tracer.rs:463-515), frame seed 0, no accumula
 - scene_update_upload_ms: the scene packed again (emitter list, median-split BVH rebuilt
   and flattened, records) and every array written to the device (th_scene_update with
   all change flags) -- the reference rebuilds and uploads these every frame, static
-  scenes included;
+  scenes included; with update="refit" the scene is packed without a tree and the
+  uploaded tree is refitted on the device instead (TH_CHANGE_REFIT; for a static scene
+  the refitted boxes equal the built ones, so the image is the same);
 - dispatch_wait_ms: th_render and the wait for it;
 - readback_ms: th_readback of the linear-HDR float4 image;
 - conversion_ms: exposure / ACES / sRGB to RGBA8, here on the GPU
@@ -77,12 +79,19 @@ def _emissive_objects(scene):
     return n + sum(1 for m in scene.triangles["material_index"] if int(m) in emissive)
 
 
+REFIT_CHANGES = (A.TH_CHANGE_CAMERA | A.TH_CHANGE_SPHERES | A.TH_CHANGE_TRIANGLES | A.TH_CHANGE_LIGHTS
+                 | A.TH_CHANGE_MATERIALS | A.TH_CHANGE_REFIT)
+
+
 def benchmark_scene(path, warmup_frames=2, measured_frames=5, options: RenderOptions = None, device=0,
-                    exposure_stops=0.0):
+                    exposure_stops=0.0, update="rebuild"):
     """Schema-v1 report of `path` (a JSON scene file) rendered `warmup_frames` +
-    `measured_frames` times."""
+    `measured_frames` times; `update` is "rebuild" (the reference's per-frame work) or
+    "refit"."""
     if measured_frames < 1:
         raise ValueError("benchmark needs at least one measured frame")
+    if update not in ("rebuild", "refit"):
+        raise ValueError("update must be 'rebuild' or 'refit'")
     opts = options or RenderOptions(traversal=A.TH_TRAVERSE_WIDE)
     t_setup = time.perf_counter()
     scene = load_scene(path)
@@ -94,7 +103,10 @@ def benchmark_scene(path, warmup_frames=2, measured_frames=5, options: RenderOpt
     try:
         for i in range(warmup_frames + measured_frames):
             t0 = time.perf_counter()
-            gs.update(pack(scene), ALL_CHANGES)
+            if update == "refit":
+                gs.update(pack(scene, bvh=False), REFIT_CHANGES)
+            else:
+                gs.update(pack(scene), ALL_CHANGES)
             ctx.synchronize()
             t1 = time.perf_counter()
             gs.render(frame_index=0, accumulated_samples=0, options=opts)
