@@ -143,16 +143,21 @@ def test_auto_schedule_keeps_the_accumulation(gpu_ctx, name, make):
 
 @pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_ORDERED, A.TH_SCHEDULE_STATIC),
                                                 (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_STATIC),
-                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET)])
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_SPECULATIVE),
+                                                (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_AUTO)])
 @pytest.mark.parametrize("samples", [1, 8, 16])
 def test_per_sample_rng_matches_oracle(gpu_ctx, traversal, schedule, samples):
+    """Per-sample RNG; the speculative schedule runs the octet kernel there (no stream to
+    foresee) and the automatic one the 4-lane kernel."""
     sc = scenes.bvh_stress(2048, samples=samples)
     print(f"case per-sample traversal={traversal} schedule={schedule} spp={samples}", flush=True)
     img = _render(gpu_ctx, sc, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=traversal,
                                              schedule=schedule))
     # lanes/quads/octets per pixel used by the kernel: megakernel min(spp, 8), quad
     # min(spp, 16), octet min(spp, 8)
-    cap = 16 if (traversal == A.TH_TRAVERSE_WIDE and schedule != A.TH_SCHEDULE_OCTET) else 8
+    octets = schedule in (A.TH_SCHEDULE_OCTET, A.TH_SCHEDULE_SPECULATIVE)
+    cap = 16 if (traversal == A.TH_TRAVERSE_WIDE and not octets) else 8
     group = 1
     while group * 2 <= min(samples, cap):
         group *= 2
