@@ -106,3 +106,41 @@ def test_random_scene_matches_oracle(gpu_ctx, seed):
         print(f"  random {seed} {name}: within tol {frac:.5f}")
         assert frac >= TOL_FRAC, (name, frac)
     gs.close()
+
+
+PROGRESSIVE = {
+    "speculative": (A.TH_RNG_REFERENCE, RenderOptions(traversal=A.TH_TRAVERSE_WIDE,
+                                                      schedule=A.TH_SCHEDULE_SPECULATIVE)),
+    "octet": (A.TH_RNG_REFERENCE, RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_OCTET)),
+    "per_sample_quad": (A.TH_RNG_PER_SAMPLE, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
+                                                           traversal=A.TH_TRAVERSE_WIDE)),
+}
+
+
+@pytest.mark.parametrize("seed", SEEDS[:5])
+def test_random_scene_two_frames_accumulated(gpu_ctx, seed):
+    """Two progressive frames (seed 0, then seed 1 folded into the average) against the
+    oracle run the same way; per-sample RNG with the 4-lane kernel's group count."""
+    sc = random_scene(seed)
+    print(f"case random progressive {seed}", flush=True)
+    o = oracle.OracleScene(sc)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    for name, (rng, opts) in PROGRESSIVE.items():
+        group = 1
+        if rng == A.TH_RNG_PER_SAMPLE:
+            while group * 2 <= min(sc.samples, 16):
+                group *= 2
+        p0 = type(o.params).from_buffer_copy(o.params)
+        p0.frame_index, p0.accumulated_samples = 0, 0
+        ref, _ = o.render(params=p0, threads=8, rng_mode=rng, group=group)
+        p1 = type(o.params).from_buffer_copy(o.params)
+        p1.frame_index, p1.accumulated_samples = 1, sc.samples
+        ref, _ = o.render(params=p1, accum=ref, threads=8, rng_mode=rng, group=group)
+        gs.render(options=opts, frame_index=0, accumulated_samples=0)
+        gs.render(options=opts, frame_index=1, accumulated_samples=sc.samples)
+        img = gs.readback()
+        close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
+        frac = float(np.mean(np.all(close, axis=-1)))
+        print(f"  random {seed} {name} two frames: within tol {frac:.5f}")
+        assert frac >= TOL_FRAC, (name, frac)
+    gs.close()
