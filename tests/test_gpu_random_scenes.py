@@ -144,3 +144,27 @@ def test_random_scene_two_frames_accumulated(gpu_ctx, seed):
         print(f"  random {seed} {name} two frames: within tol {frac:.5f}")
         assert frac >= TOL_FRAC, (name, frac)
     gs.close()
+
+
+@pytest.mark.parametrize("rng", [A.TH_RNG_REFERENCE, A.TH_RNG_PER_SAMPLE])
+def test_random_scenes_in_one_batch(gpu_ctx, rng):
+    """All random scenes in one persistent batched launch: each equals its oracle image."""
+    from toaster_amd.batch import BatchRenderer
+    items = [random_scene(s) for s in SEEDS]
+    print(f"case random batch rng={rng}", flush=True)
+    br = BatchRenderer(gpu_ctx)
+    for sc in items:
+        br.add(sc)
+    br.render(RenderOptions(rng_mode=rng), use_tuning=False)
+    imgs = [gs.readback() for gs in br.scenes]
+    for sc, img in zip(items, imgs):
+        group = 1
+        if rng == A.TH_RNG_PER_SAMPLE:
+            while group * 2 <= min(sc.samples, 16):
+                group *= 2
+        ref, _ = oracle.OracleScene(sc).render(threads=8, rng_mode=rng, group=group)
+        close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
+        frac = float(np.mean(np.all(close, axis=-1)))
+        print(f"  {sc.name} batched: within tol {frac:.5f}")
+        assert frac >= TOL_FRAC, (sc.name, frac)
+    br.close()
