@@ -168,3 +168,23 @@ def test_random_scenes_in_one_batch(gpu_ctx, rng):
         print(f"  {sc.name} batched: within tol {frac:.5f}")
         assert frac >= TOL_FRAC, (sc.name, frac)
     br.close()
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_scene_guided_alpha_zero(gpu_ctx, seed):
+    """The guided wavefront at alpha = 0 (its own shading kernels, training records on)
+    is the reference estimator with per-sample streams: equal to the oracle's image."""
+    from toaster_amd.guide import Guide, render_guided
+    sc = random_scene(seed)
+    print(f"case random guided alpha=0 {seed}", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    g = Guide(gpu_ctx, seed=3)
+    render_guided(gs, g, sc.samples, alpha=0.0, train=True, batch=1024, progressive=False, warmup_steps=0)
+    img = gs.readback()
+    ref, _ = oracle.OracleScene(sc).render(threads=8, rng_mode=A.TH_RNG_PER_SAMPLE, group=1)
+    close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
+    frac = float(np.mean(np.all(close, axis=-1)))
+    print(f"  random {seed} guided alpha=0: within tol {frac:.5f}")
+    assert frac >= TOL_FRAC
+    g.close()
+    gs.close()
