@@ -188,3 +188,35 @@ def test_random_scene_guided_alpha_zero(gpu_ctx, seed):
     assert frac >= TOL_FRAC
     g.close()
     gs.close()
+
+
+@pytest.mark.parametrize("seed", SEEDS[:6])
+def test_random_scene_refit_after_motion(gpu_ctx, seed):
+    """The random scene uploaded once, then moved (every other triangle rotated about the
+    y axis, spheres shifted) and refitted on the device: equal to the oracle's image of the
+    moved scene."""
+    import dataclasses
+    sc = random_scene(seed)
+    print(f"case random refit {seed}", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    tris = sc.triangles.copy()
+    if len(tris):
+        v = tris["v"].astype(np.float64)
+        c = v.reshape(-1, 3).mean(axis=0)
+        R = np.array([[np.cos(0.3), 0.0, np.sin(0.3)], [0.0, 1.0, 0.0], [-np.sin(0.3), 0.0, np.cos(0.3)]])
+        sel = np.arange(len(tris)) % 2 == 0
+        v[sel] = (v[sel] - c) @ R.T + c
+        tris["v"] = v.astype(np.float32)
+    sph = sc.spheres.copy()
+    sph["center"] = sph["center"] + np.float32(0.2)
+    moved = dataclasses.replace(sc, triangles=tris, spheres=sph)
+    gs.refit(moved)
+    ref, _ = oracle.OracleScene(moved).render(threads=8)
+    for name in ("reference", "speculative", "wavefront"):
+        gs.render(options=KERNELS[name], frame_index=0, accumulated_samples=0)
+        img = gs.readback()
+        close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
+        frac = float(np.mean(np.all(close, axis=-1)))
+        print(f"  random {seed} refit {name}: within tol {frac:.5f}")
+        assert frac >= TOL_FRAC, (name, frac)
+    gs.close()
