@@ -1249,7 +1249,13 @@ static int32_t auto_schedule(th_context* ctx, th_scene* s, const TH_GpuRenderPar
     th_render_options t = *o;
     t.schedule = cands[c];
     for (int r = 0; r < 3; ++r)
-      if ((rc = th_render(ctx, s, &tp, cam, &t))) return rc;
+      if ((rc = th_render(ctx, s, &tp, cam, &t))) {
+        // a failed trial: the caller's accumulation comes back, the trials leave no timings
+        for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
+        ctx->ev_pending.clear();
+        if (keep) (void)hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        return rc;
+      }
     float total = 0.0f;
     for (size_t i = 0; i < ctx->ev_pending.size(); ++i) {
       auto& pr = ctx->ev_pending[i];
