@@ -3,8 +3,11 @@
 // read back and converted to RGBA8, written as a binary PPM.  What a host program
 // (or a Rust crate binding the same entry points) does in place of toaster-gpu's
 // renderer: scene_to_gpu, buffer creation, dispatch, readback, pixels_to_rgba_image.
+// With MOVED.ppm it then moves the spheres, packs that frame without a tree, refits the
+// uploaded tree on the device (TH_CHANGE_REFIT) and renders it with the automatically
+// chosen schedule.
 //
-//   cornell_c OUT.ppm [width height samples]
+//   cornell_c OUT.ppm [width height samples] [MOVED.ppm]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +20,23 @@ namespace {
 int fail(const char* what) {
   std::fprintf(stderr, "cornell_c: %s: %s\n", what, th_last_error());
   return 1;
+}
+
+// linear HDR of the scene's accumulation -> RGBA8 -> binary PPM
+int write_ppm(th_context* ctx, th_scene* scene, uint32_t width, uint32_t height, const char* path) {
+  std::vector<float> hdr((size_t)width * height * 4);
+  if (th_readback(ctx, scene, hdr.data(), hdr.size() * sizeof(float))) return fail("readback");
+  std::vector<uint8_t> rgba((size_t)width * height * 4);
+  if (th_display_rgba8(hdr.data(), width, height, 0.0f, rgba.data(), rgba.size())) return fail("display");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    std::fprintf(stderr, "cornell_c: cannot write %s\n", path);
+    return 1;
+  }
+  std::fprintf(f, "P6\n%u %u\n255\n", width, height);
+  for (size_t i = 0; i < (size_t)width * height; ++i) std::fwrite(&rgba[4 * i], 1, 3, f);
+  std::fclose(f);
+  return 0;
 }
 
 th_triangle tri(const float a[3], const float b[3], const float c[3], uint32_t m) {
@@ -45,6 +65,7 @@ int main(int argc, char** argv) {
   const uint32_t width = argc > 4 ? (uint32_t)std::atoi(argv[2]) : 320u;
   const uint32_t height = argc > 4 ? (uint32_t)std::atoi(argv[3]) : 180u;
   const uint32_t samples = argc > 4 ? (uint32_t)std::atoi(argv[4]) : 4u;
+  const char* moved_path = argc == 3 ? argv[2] : (argc > 5 ? argv[5] : nullptr);
 
   const th_material mats[] = {
       {TH_MAT_DIFFUSE, {0.73f, 0.73f, 0.73f}, 0.0f, 0.0f, 0.0f, 0},
@@ -120,23 +141,33 @@ int main(int argc, char** argv) {
   params.frame_index = 0;
   params.accumulated_samples = 0;
   if (th_render(ctx, scene, &params, &desc.camera, &opt)) return fail("render");
-  std::vector<float> hdr((size_t)width * height * 4);
-  if (th_readback(ctx, scene, hdr.data(), hdr.size() * sizeof(float))) return fail("readback");
-  std::vector<uint8_t> rgba((size_t)width * height * 4);
-  if (th_display_rgba8(hdr.data(), width, height, 0.0f, rgba.data(), rgba.size())) return fail("display");
+  if (write_ppm(ctx, scene, width, height, argv[1])) return 1;
   th_stats st;
   if (th_get_stats(ctx, &st)) return fail("stats");
-
-  FILE* f = std::fopen(argv[1], "wb");
-  if (!f) {
-    std::fprintf(stderr, "cornell_c: cannot write %s\n", argv[1]);
-    return 1;
-  }
-  std::fprintf(f, "P6\n%u %u\n255\n", width, height);
-  for (size_t i = 0; i < (size_t)width * height; ++i) std::fwrite(&rgba[4 * i], 1, 3, f);
-  std::fclose(f);
   std::printf("cornell %ux%u, %u spp: %.4f ms, %llu paths\n", width, height, samples, st.last_kernel_ms,
               (unsigned long long)st.paths);
+
+  if (moved_path) {
+    // the next frame of an animation: both spheres moved, the tree refitted on the device
+    th_sphere moved[2] = {spheres[0], spheres[1]};
+    moved[0].center[0] = spheres[0].center[0] + 0.1f;
+    moved[1].center[1] = spheres[1].center[1] + 0.1f;
+    th_scene_input in2 = in;
+    in2.spheres = moved;
+    th_packed_scene* packed2 = nullptr;
+    if (th_pack_scene(&in2, 1 | TH_PACK_NO_BVH, &packed2)) return fail("pack moved frame");
+    th_scene_desc desc2;
+    if (th_packed_view(packed2, &desc2)) return fail("packed view");
+    if (th_scene_update(scene, TH_CHANGE_SPHERES | TH_CHANGE_LIGHTS | TH_CHANGE_REFIT, &desc2))
+      return fail("refit");
+    th_render_options auto_opt = opt;
+    auto_opt.schedule = TH_SCHEDULE_AUTO;
+    // the render parameters keep the uploaded tree's counts (the moved frame has no tree)
+    if (th_render(ctx, scene, &params, &desc2.camera, &auto_opt)) return fail("render moved frame");
+    if (write_ppm(ctx, scene, width, height, moved_path)) return 1;
+    std::printf("moved frame refitted and rendered\n");
+    th_packed_free(packed2);
+  }
 
   th_scene_destroy(scene);
   th_context_destroy(ctx);
