@@ -142,6 +142,8 @@ def scene_update_leg(ctx, sc, frames=10):
         out[mode] = {"update_ms": round(float(np.median(upd)), 3),
                      "upload_and_device_ms": round(float(np.median(dev)), 3),
                      "render_ms": round(float(np.median(ren)), 4)}
+        if mode == "refit":  # summed node-box area of the refitted tree over the built one's
+            out[mode]["area_ratio_after"] = round(gs.refit_quality(), 4)
         gs.close()
     out["workload"] = (f"{len(v0)} triangles, every other one rotated per frame, {frames} frames, "
                        "render with the speculative kernel")

@@ -342,6 +342,12 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* desc, uint32_t nod
  * tie order follows the tree). */
 #define TH_CHANGE_REFIT 64u
 int32_t th_scene_update(th_scene* scene, uint32_t change_mask, const th_scene_desc* desc);
+/* How far refits have loosened the tree: the summed surface area of all node boxes (the
+ * weight of a SAH cost) after the last refit over that of the tree as last built or
+ * uploaded (1 = as built; the same device reduction computes both).  No counterpart in the
+ * reference, which rebuilds every frame; a caller rebuilds (TH_CHANGE_BVH) once it passes
+ * a bound of its choice. */
+int32_t th_scene_refit_quality(const th_scene* scene, float* ratio);
 void th_scene_destroy(th_scene* scene);
 
 /* One dispatch: params.samples samples per pixel folded into the progressive

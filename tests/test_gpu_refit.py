@@ -68,7 +68,9 @@ def test_refit_of_unmoved_geometry_changes_nothing(gpu_ctx, name):
     print(f"case refit identity {name}", flush=True)
     gs = GpuScene.from_scene(gpu_ctx, sc)
     before = {k: _render(gs, o) for k, o in KERNELS.items()}
+    assert gs.refit_quality() == 1.0
     gs.refit(sc)
+    assert gs.refit_quality() == 1.0  # the same boxes, summed in the same order
     for k, o in KERNELS.items():
         after = _render(gs, o)
         assert np.array_equal(before[k].view(np.uint32), after.view(np.uint32)), k
@@ -110,3 +112,27 @@ def test_refit_flags_are_validated(gpu_ctx):
     ref, _ = oracle.OracleScene(sc).render(threads=8)
     assert _close(img, ref) >= TOL_FRAC
     gs.close()
+
+
+def test_refit_quality_and_rebuild_policy(gpu_ctx):
+    """Motion loosens a refitted tree (summed box area grows with it); refit(...,
+    rebuild_above=r) rebuilds once the ratio passes r, after which the tree is as built
+    and the image is a freshly uploaded scene's, bit for bit."""
+    sc = SCENES["stress2048"]()
+    print("case refit quality", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    ratios = []
+    for step in (1, 2, 3):
+        assert gs.refit(_moved(sc, step)) is False
+        ratios.append(gs.refit_quality())
+    print("  summed box area after 1, 2, 3 steps:", [round(r, 4) for r in ratios])
+    assert 1.0 < ratios[0] < ratios[1] < ratios[2]
+    moved = _moved(sc, 3)
+    assert gs.refit(moved, rebuild_above=1.0) is True
+    assert gs.refit_quality() == 1.0
+    img = _render(gs, KERNELS["speculative"])
+    fresh = GpuScene.from_scene(gpu_ctx, moved)
+    ref = _render(fresh, KERNELS["speculative"])
+    fresh.close()
+    gs.close()
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

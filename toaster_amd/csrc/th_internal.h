@@ -68,6 +68,9 @@ struct th_scene {
   uint32_t* d_wide_src = nullptr;
   // TH_SCHEDULE_AUTO: the schedule chosen for (samples, bounces, background), and a copy
   // of the accumulation kept while the candidates are timed
+  // summed surface area of all node boxes: as built, and after the last refit
+  float area_build = 0.0f, area_now = 0.0f;
+  float* d_area = nullptr;
   bool auto_valid = false;
   uint32_t auto_samples = 0, auto_bounces = 0, auto_background = 0, auto_choice = 0;
   float4* d_accum_save = nullptr;

@@ -133,6 +133,25 @@ __global__ void k_refit_wide(Child4* __restrict__ nodes4, const uint32_t* __rest
   nodes4[i] = c;
 }
 
+// Summed surface area of all node boxes (the weight of a SAH cost), one block, fixed
+// order: per-thread strided sums, then a tree reduction in LDS.
+__global__ void k_area_sum(const TH_GpuBvhNode* __restrict__ nodes, uint32_t count, float* __restrict__ out) {
+  __shared__ float part[256];
+  float acc = 0.0f;
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+    const TH_GpuBvhNode& n = nodes[i];
+    const float dx = n.max[0] - n.min[0], dy = n.max[1] - n.min[1], dz = n.max[2] - n.min[2];
+    acc = acc + 2.0f * (dx * dy + dy * dz + dz * dx);
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t w = blockDim.x / 2u; w > 0u; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
 // Pixel of a lane: each 64-lane wave covers one 8x8 tile (coherent primary rays).
 __device__ __forceinline__ bool tile_pixel(uint32_t W, uint32_t H, uint32_t* x, uint32_t* y) {
   uint32_t lane = threadIdx.x & 63u;
@@ -681,7 +700,7 @@ int32_t upload(th_context* ctx, T* dst, const T* src, size_t count) {
 void free_scene_buffers(th_scene* s) {
   void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
                   s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum,
-                  s->d_nodes4, s->d_refit_order, s->d_wide_src, s->d_accum_save};
+                  s->d_nodes4, s->d_refit_order, s->d_wide_src, s->d_accum_save, s->d_area};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -887,6 +906,11 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
     hipLaunchKernelGGL(k_prepare_cnodes, dim3((s->node_count + 255) / 256), dim3(256), 0, ctx->stream,
                        s->d_nodes, s->node_count, s->d_cnodes);
     TH_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_area_sum, dim3(1), dim3(256), 0, ctx->stream, s->d_nodes, s->node_count, s->d_area);
+    TH_CHECK(hipGetLastError());
+    TH_CHECK(hipMemcpyAsync(&s->area_build, s->d_area, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    TH_CHECK(hipStreamSynchronize(ctx->stream));
+    s->area_now = s->area_build;
   }
   if (s->prim_count) {
     hipLaunchKernelGGL(k_prepare_slots, dim3((s->prim_count + 255) / 256), dim3(256), 0, ctx->stream,
@@ -922,8 +946,11 @@ int32_t refit_bvh(th_scene* s) {
                        s->d_wide_src, nc, s->d_nodes);
     TH_CHECK(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_area_sum, dim3(1), dim3(256), 0, ctx->stream, s->d_nodes, s->node_count, s->d_area);
+  TH_CHECK(hipGetLastError());
   TH_GpuBvhNode root;
   TH_CHECK(hipMemcpyAsync(&root, s->d_nodes, sizeof root, hipMemcpyDeviceToHost, ctx->stream));
+  TH_CHECK(hipMemcpyAsync(&s->area_now, s->d_area, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   TH_CHECK(hipStreamSynchronize(ctx->stream));
   for (int k = 0; k < 3; ++k) {
     s->root_min[k] = root.min[k];
@@ -1124,6 +1151,7 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_c
       (rc = dalloc(&s->d_tex, d->texture_pixel_count)) || (rc = dalloc(&s->d_cnodes, cap)) ||
       (rc = dalloc(&s->d_slots, 3 * (size_t)nprim)) || (rc = dalloc(&s->d_accum, npix)) ||
       (rc = dalloc(&s->d_nodes4, 4 * (size_t)cap)) || (rc = dalloc(&s->d_refit_order, cap)) ||
+      (rc = dalloc(&s->d_area, 1)) ||
       (rc = dalloc(&s->d_wide_src, 4 * (size_t)cap))) {
     free_scene_buffers(s);
     delete s;
@@ -1206,6 +1234,12 @@ int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
     if ((rc = prepare_bvh(s, d))) return rc;
   }
   TH_CHECK(hipStreamSynchronize(ctx->stream));
+  return TH_OK;
+}
+
+int32_t th_scene_refit_quality(const th_scene* s, float* ratio) {
+  if (!s || !ratio) return th_set_error(TH_E_INVALID, "null argument");
+  *ratio = s->area_build > 0.0f ? s->area_now / s->area_build : 1.0f;
   return TH_OK;
 }
 

@@ -130,13 +130,25 @@ class GpuScene:
             packed.params.bvh_primitive_count = self.packed.params.bvh_primitive_count
         self.packed = packed
 
-    def refit(self, scene: Scene):
+    def refit(self, scene: Scene, rebuild_above=None):
         """Moved geometry (same primitives, materials and lights count) without a host
         BVH rebuild: new records and light list, node boxes recomputed on the device over
-        the uploaded tree.  The reference rebuilds instead (docs/architecture.md:118-121)."""
+        the uploaded tree.  The reference rebuilds instead (docs/architecture.md:118-121).
+        With `rebuild_above`, a refit that loosens the tree past that ratio of summed box
+        area (refit_quality) is followed by a rebuild; returns True when it rebuilt."""
         self.update(pack(scene, bvh=False),
                     A.TH_CHANGE_SPHERES | A.TH_CHANGE_TRIANGLES | A.TH_CHANGE_LIGHTS | A.TH_CHANGE_CAMERA
                     | A.TH_CHANGE_REFIT)
+        if rebuild_above is not None and self.refit_quality() > rebuild_above:
+            self.update(pack(scene), A.TH_CHANGE_SPHERES | A.TH_CHANGE_TRIANGLES | A.TH_CHANGE_BVH)
+            return True
+        return False
+
+    def refit_quality(self):
+        """Summed node-box area after the last refit over that of the tree as built."""
+        r = C.c_float()
+        A.check(self._lib.th_scene_refit_quality(self._h, C.byref(r)))
+        return float(r.value)
 
     def params(self, samples=None, frame_index=None, accumulated_samples=None, max_bounces=None):
         p = A.GpuRenderParams.from_buffer_copy(self.packed.params)
