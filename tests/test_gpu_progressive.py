@@ -52,3 +52,41 @@ def test_sink_stops_and_sets_batches(gpu_ctx):
     sink = Recorder(batches=[1, 3])
     out = render_progressive_with_sink(gpu_ctx, sc, 6, sink)
     assert [(n, acc) for _, n, acc, _, _ in sink.frames] == [(1, 1), (3, 4), (2, 6)]
+
+
+def _moving_cornell(frame):
+    from toaster_amd.progressive import EvaluatedFrame
+    sc = scenes.cornell(width=96, height=64, samples=2)
+    sc.spheres["center"][0, 0] = sc.spheres["center"][0, 0] + np.float32(0.05 * frame)
+    return EvaluatedFrame(scene=sc, camera=False, spheres=True, triangles=False, emissive_geometry=False,
+                          time_seconds=frame / 30.0)
+
+
+@pytest.mark.parametrize("update", ["rebuild", "refit"])
+def test_animation_frames_equal_single_renders(gpu_ctx, update):
+    """An evaluated animation through a frame sink: each delivered frame is that frame's
+    scene rendered on its own with frame seed = frame index, byte for byte, whether the
+    moved geometry was rebuilt or refitted."""
+    from toaster_amd.progressive import render_animation_with_sink
+    opts = RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_SPECULATIVE)
+    print(f"case animation sink update={update}", flush=True)
+    sink = Recorder()
+    out = render_animation_with_sink(gpu_ctx, _moving_cornell, 3, sink, options=opts, update=update)
+    assert out["frames"] == 3 and [f[0] for f in sink.frames] == [0, 1, 2]
+    for index, _, _, image, _ in sink.frames:
+        gs = GpuScene.from_scene(gpu_ctx, _moving_cornell(index).scene)
+        gs.render(frame_index=index, accumulated_samples=0, options=opts)
+        assert np.array_equal(gs.display_rgba8(), image)
+        gs.close()
+    assert not np.array_equal(sink.frames[0][3], sink.frames[2][3])
+
+
+def test_static_independent_frames_repeat(gpu_ctx):
+    """static_independent (the benchmark mode): time zero and seed zero every frame, so
+    every frame is the same image; the sink can stop the loop early."""
+    from toaster_amd.progressive import render_animation_with_sink
+    sink = Recorder(stop_after=2)
+    print("case animation static independent", flush=True)
+    out = render_animation_with_sink(gpu_ctx, _moving_cornell, 5, sink, frame_mode="static_independent")
+    assert out["frames"] == 2
+    assert np.array_equal(sink.frames[0][3], sink.frames[1][3])
