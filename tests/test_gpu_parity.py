@@ -141,6 +141,25 @@ def test_auto_schedule_keeps_the_accumulation(gpu_ctx, name, make):
     gs.close()
 
 
+def test_launch_timing_stays_bounded(gpu_ctx):
+    """Every launch is timed by an event pair that th_get_stats folds into the totals; a
+    caller that renders 5,000 frames without reading the statistics keeps the pending
+    list bounded (its oldest 2,048 are folded at 4,096) and still has every launch
+    counted, with the last one's time in last_kernel_ms."""
+    sc = scenes.cornell(width=16, height=16, samples=1, max_bounces=1)
+    print("case 5000 small renders without stats reads", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    gpu_ctx.synchronize()
+    gpu_ctx.reset_stats()
+    n = 5000
+    for i in range(n):
+        gs.render(frame_index=i, accumulated_samples=0)
+    st = gpu_ctx.stats()
+    gs.close()
+    assert st["launches"] == n
+    assert 0.0 < st["last_kernel_ms"] <= st["total_kernel_ms"]
+
+
 @pytest.mark.parametrize("traversal,schedule", [(A.TH_TRAVERSE_ORDERED, A.TH_SCHEDULE_STATIC),
                                                 (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_STATIC),
                                                 (A.TH_TRAVERSE_WIDE, A.TH_SCHEDULE_OCTET),

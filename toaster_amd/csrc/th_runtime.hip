@@ -1001,10 +1001,39 @@ DevScene dev_scene(const th_scene* s) {
   return S;
 }
 
+// Folds the `n` oldest pending launches into the statistics, in launch order, and
+// returns their event pairs to the pool.  On an event error the pairs already folded
+// leave the pending list, so no pair is ever both pooled and pending.
+static int32_t fold_oldest(th_context* ctx, size_t n) {
+  auto& pend = ctx->ev_pending;
+  n = std::min(n, pend.size());
+  size_t done = 0;
+  int32_t rc = TH_OK;
+  for (; done < n; ++done) {
+    const auto pr = pend[done];
+    hipError_t e = hipEventSynchronize(pr.second);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
+    if (e != hipSuccess) {
+      rc = th_set_error(TH_E_DEVICE, std::string("launch timing: ") + hipGetErrorString(e));
+      break;
+    }
+    ctx->stats.last_kernel_ms = ms;
+    ctx->stats.total_kernel_ms += ms;
+    ctx->stats.launches += 1;
+    ctx->ev_pool.push_back(pr);
+  }
+  pend.erase(pend.begin(), pend.begin() + (std::ptrdiff_t)done);
+  return rc;
+}
+
 // Per-launch event pair from a pool (every launch is timed; th_get_stats sums them).
+// A caller that never reads the statistics still keeps the pending list bounded: at
+// 4,096 pending launches the oldest 2,048 (long finished) are folded.
 int32_t begin_timed(th_context* ctx, std::pair<hipEvent_t, hipEvent_t>* pr) {
-  if (ctx->ev_pending.size() >= 4096) {  // bound the pending list: fold the oldest
-    TH_CHECK(hipEventSynchronize(ctx->ev_pending.front().second));
+  if (ctx->ev_pending.size() >= 4096) {
+    int32_t rc = fold_oldest(ctx, 2048);
+    if (rc) return rc;
   }
   if (ctx->ev_pool.empty()) {
     hipEvent_t a, b;
@@ -1028,19 +1057,7 @@ int32_t end_timed(th_context* ctx, std::pair<hipEvent_t, hipEvent_t> pr) {
   return TH_OK;
 }
 
-int32_t fold_timing(th_context* ctx) {
-  for (auto& pr : ctx->ev_pending) {
-    TH_CHECK(hipEventSynchronize(pr.second));
-    float ms = 0.0f;
-    TH_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
-    ctx->stats.last_kernel_ms = ms;
-    ctx->stats.total_kernel_ms += ms;
-    ctx->stats.launches += 1;
-    ctx->ev_pool.push_back(pr);
-  }
-  ctx->ev_pending.clear();
-  return TH_OK;
-}
+int32_t fold_timing(th_context* ctx) { return fold_oldest(ctx, ctx->ev_pending.size()); }
 
 
 int32_t th_check_render_params(const th_scene* s, const TH_GpuRenderParams* p) {
@@ -1282,23 +1299,26 @@ static int32_t auto_schedule(th_context* ctx, th_scene* s, const TH_GpuRenderPar
   for (int c = 0; c < 3; ++c) {
     th_render_options t = *o;
     t.schedule = cands[c];
+    // a failed trial: the caller's accumulation comes back, the trials leave no timings
+    auto fail = [&](int32_t code) {
+      for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
+      ctx->ev_pending.clear();
+      if (keep) (void)hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+      return code;
+    };
     for (int r = 0; r < 3; ++r)
-      if ((rc = th_render(ctx, s, &tp, cam, &t))) {
-        // a failed trial: the caller's accumulation comes back, the trials leave no timings
-        for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
-        ctx->ev_pending.clear();
-        if (keep) (void)hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream);
-        return rc;
-      }
+      if ((rc = th_render(ctx, s, &tp, cam, &t))) return fail(rc);
     float total = 0.0f;
     for (size_t i = 0; i < ctx->ev_pending.size(); ++i) {
-      auto& pr = ctx->ev_pending[i];
-      TH_CHECK(hipEventSynchronize(pr.second));
+      const auto& pr = ctx->ev_pending[i];
       float ms = 0.0f;
-      TH_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+      hipError_t e = hipEventSynchronize(pr.second);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
+      if (e != hipSuccess)
+        return fail(th_set_error(TH_E_DEVICE, std::string("schedule trial timing: ") + hipGetErrorString(e)));
       if (i > 0) total += ms;  // the first launch warms up
-      ctx->ev_pool.push_back(pr);
     }
+    for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
     ctx->ev_pending.clear();
     if (c == 0 || total < best) {
       best = total;
