@@ -324,7 +324,8 @@ const char* th_last_error(void);
  * the caller keeps ownership.  node_capacity = 0 selects 2N-1 (buffers.rs:205-211). */
 int32_t th_scene_create(th_context* ctx, const th_scene_desc* desc, uint32_t node_capacity,
                         th_scene** out);
-/* Per-frame update of the arrays whose bit is set (path_tracer.rs:676-720). */
+/* Per-frame update of the arrays whose bit is set (path_tracer.rs:676-720); a bit not
+ * defined below is rejected (TH_E_INVALID) before anything is uploaded. */
 #define TH_CHANGE_CAMERA 1u
 #define TH_CHANGE_SPHERES 2u
 #define TH_CHANGE_TRIANGLES 4u

@@ -247,9 +247,14 @@ def test_display_device_matches_host(gpu_ctx):
     gs.render()
     img = gs.readback()
     dev = gs.display_rgba8(0.5)
+    # the staging buffer is kept with the scene: a second conversion after another
+    # render and at another exposure must not see the first one's bytes
+    gs.render(frame_index=1, accumulated_samples=sc.samples)
+    img2 = gs.readback()
+    dev2 = gs.display_rgba8(-1.0)
     gs.close()
-    host = display_rgba8(img, 0.5)
-    assert np.array_equal(dev, host)
+    assert np.array_equal(dev, display_rgba8(img, 0.5))
+    assert np.array_equal(dev2, display_rgba8(img2, -1.0))
 
 
 def test_converged_per_sample_matches_oracle_reference_stream(gpu_ctx):

@@ -108,6 +108,8 @@ def test_refit_flags_are_validated(gpu_ctx):
         gs.update(pack(sc), A.TH_CHANGE_TRIANGLES | A.TH_CHANGE_BVH | A.TH_CHANGE_REFIT)
     with pytest.raises(A.ToasterError):  # refit without new geometry
         gs.update(pack(sc, bvh=False), A.TH_CHANGE_MATERIALS | A.TH_CHANGE_REFIT)
+    with pytest.raises(A.ToasterError):  # a change bit the library does not define
+        gs.update(pack(sc), A.TH_CHANGE_TRIANGLES | 128)
     img = _render(gs, KERNELS["quad"])  # the scene is unchanged by the refused updates
     ref, _ = oracle.OracleScene(sc).render(threads=8)
     assert _close(img, ref) >= TOL_FRAC

@@ -766,10 +766,10 @@ int32_t th_guide_mfma_selftest(th_context* ctx, const float* A, const float* B, 
   if (!ctx || !A || !B || !C) return th_set_error(TH_E_INVALID, "null argument");
   TH_CHECK(hipSetDevice(ctx->device));
   float *dA = nullptr, *dB = nullptr, *dC = nullptr;
-  TH_CHECK(hipMalloc((void**)&dA, 16 * 32 * 4));
-  TH_CHECK(hipMalloc((void**)&dB, 32 * 16 * 4));
-  TH_CHECK(hipMalloc((void**)&dC, 16 * 16 * 4));
-  hipError_t e = hipMemcpy(dA, A, 16 * 32 * 4, hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc((void**)&dA, 16 * 32 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&dB, 32 * 16 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&dC, 16 * 16 * 4);
+  if (e == hipSuccess) e = hipMemcpy(dA, A, 16 * 32 * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dB, B, 32 * 16 * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);

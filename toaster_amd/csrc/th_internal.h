@@ -74,6 +74,7 @@ struct th_scene {
   bool auto_valid = false;
   uint32_t auto_samples = 0, auto_bounces = 0, auto_background = 0, auto_choice = 0;
   float4* d_accum_save = nullptr;
+  uchar4* d_rgba8 = nullptr;  // display staging (width*height), allocated on first use
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
   th_guided_ws* gw = nullptr;  // allocated by the first guided render
   th_tuning tuning = {0, 0, 0.0f, 0};

@@ -700,7 +700,8 @@ int32_t upload(th_context* ctx, T* dst, const T* src, size_t count) {
 void free_scene_buffers(th_scene* s) {
   void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
                   s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum,
-                  s->d_nodes4, s->d_refit_order, s->d_wide_src, s->d_accum_save, s->d_area};
+                  s->d_nodes4, s->d_refit_order, s->d_wide_src, s->d_accum_save, s->d_area,
+                  s->d_rgba8};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1202,6 +1203,9 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_c
 int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
   if (!s || !d) return th_set_error(TH_E_INVALID, "null argument");
   th_context* ctx = s->ctx;
+  const uint32_t known = TH_CHANGE_CAMERA | TH_CHANGE_SPHERES | TH_CHANGE_TRIANGLES | TH_CHANGE_LIGHTS |
+                         TH_CHANGE_BVH | TH_CHANGE_MATERIALS | TH_CHANGE_REFIT;
+  if (mask & ~known) return th_set_error(TH_E_INVALID, "unknown change bit");
   const bool refit = (mask & TH_CHANGE_REFIT) != 0u;
   if (refit && (mask & TH_CHANGE_BVH))
     return th_set_error(TH_E_INVALID, "a refit and a new bvh are exclusive");
@@ -1364,7 +1368,11 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
   const uint32_t block = 64;
   size_t lds = (size_t)depth * block * sizeof(uint2);
   if (lds > 64 * 1024) return th_set_error(TH_E_UNSUPPORTED, "bvh too deep for the traversal stack");
-  if (o.collect_stats) TH_CHECK(hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
+  if (o.collect_stats) {  // a fill kernel, not a memset node, so a caller may capture the render
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->d_counters,
+                       (uint32_t)(16 * sizeof(unsigned long long) / sizeof(uint32_t)));
+    TH_CHECK(hipGetLastError());
+  }
   unsigned long long* cnt = ctx->d_counters;
   std::pair<hipEvent_t, hipEvent_t> evp;
   if ((rc = begin_timed(ctx, &evp))) return rc;
@@ -1612,15 +1620,13 @@ int32_t th_display_rgba8_device(th_context* ctx, th_scene* s, float exposure_sto
   size_t npix = (size_t)s->width * s->height;
   if (out_bytes != npix * 4) return th_set_error(TH_E_INVALID, "output size must be width*height*4");
   TH_CHECK(hipSetDevice(ctx->device));
-  uchar4* d_out = nullptr;
-  TH_CHECK(hipMalloc((void**)&d_out, npix * 4));
+  if (!s->d_rgba8) TH_CHECK(hipMalloc((void**)&s->d_rgba8, npix * 4));  // kept with the scene
   float scale = exp2f(exposure_stops);
   hipLaunchKernelGGL(k_display, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, s->d_accum,
-                     (uint32_t)npix, scale, ctx->d_srgb_lut + 256, d_out);
+                     (uint32_t)npix, scale, ctx->d_srgb_lut + 256, s->d_rgba8);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, npix * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, s->d_rgba8, npix * 4, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(d_out);
   if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("display: ") + hipGetErrorString(e));
   return TH_OK;
 }
