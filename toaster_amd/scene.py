@@ -127,7 +127,7 @@ def _copy(ptr, count, dtype):
     if count == 0 or not ptr:
         return np.zeros(0, dtype)
     buf = (C.c_char * (count * dtype.itemsize)).from_address(ptr)
-    return np.frombuffer(bytes(buf), dtype=dtype).copy()
+    return np.frombuffer(buf, dtype=dtype).copy()  # one copy out of the packer's memory
 
 
 class PackedScene:
