@@ -2,7 +2,10 @@
 and triangles (with and without shading normals and texture coordinates), all material
 kinds (diffuse, metal, dielectric, emissive, textured diffuse), sky, black and
 environment-map backgrounds, 1-6 samples and 0-6 bounces.  Image tolerance of
-tests/test_gpu_parity.py."""
+tests/test_gpu_parity.py.  TOASTER_EXTRA_RANDOM_SEEDS="lo:hi" adds seeds lo..hi-1 to the
+oracle comparison (a wider sweep than the default ten)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +27,8 @@ KERNELS = {
     "wavefront": RenderOptions(traversal=A.TH_TRAVERSE_WIDE, kernel=A.TH_KERNEL_WAVEFRONT),
 }
 SEEDS = list(range(10))
+_extra = os.environ.get("TOASTER_EXTRA_RANDOM_SEEDS", "")
+EXTRA_SEEDS = list(range(*map(int, _extra.split(":")))) if _extra else []
 
 
 def random_scene(seed):
@@ -88,7 +93,7 @@ def random_scene(seed):
                  textures=textures, environment=env, name=f"random{seed}")
 
 
-@pytest.mark.parametrize("seed", SEEDS)
+@pytest.mark.parametrize("seed", SEEDS + EXTRA_SEEDS)
 def test_random_scene_matches_oracle(gpu_ctx, seed):
     sc = random_scene(seed)
     print(f"case random scene {seed}: {len(sc.spheres)} spheres, {len(sc.triangles)} triangles, "
@@ -103,7 +108,8 @@ def test_random_scene_matches_oracle(gpu_ctx, seed):
         assert np.isfinite(img).all()
         close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
         frac = float(np.mean(np.all(close, axis=-1)))
-        print(f"  random {seed} {name}: within tol {frac:.5f}")
+        exact = float(np.mean(np.all(img == ref, axis=-1)))
+        print(f"  random {seed} {name}: within tol {frac:.5f} bit-exact {exact:.5f}")
         assert frac >= TOL_FRAC, (name, frac)
     gs.close()
 
