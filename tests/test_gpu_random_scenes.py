@@ -3,7 +3,7 @@ and triangles (with and without shading normals and texture coordinates), all ma
 kinds (diffuse, metal, dielectric, emissive, textured diffuse), sky, black and
 environment-map backgrounds, 1-6 samples and 0-6 bounces.  Image tolerance of
 tests/test_gpu_parity.py.  TOASTER_EXTRA_RANDOM_SEEDS="lo:hi" adds seeds lo..hi-1 to the
-oracle comparison (a wider sweep than the default ten)."""
+single-frame, two-frame and guided comparisons (a wider sweep than the default seeds)."""
 import os
 
 import numpy as np
@@ -123,7 +123,7 @@ PROGRESSIVE = {
 }
 
 
-@pytest.mark.parametrize("seed", SEEDS[:5])
+@pytest.mark.parametrize("seed", SEEDS[:5] + EXTRA_SEEDS)
 def test_random_scene_two_frames_accumulated(gpu_ctx, seed):
     """Two progressive frames (seed 0, then seed 1 folded into the average) against the
     oracle run the same way; per-sample RNG with the 4-lane kernel's group count."""
@@ -176,7 +176,7 @@ def test_random_scenes_in_one_batch(gpu_ctx, rng):
     br.close()
 
 
-@pytest.mark.parametrize("seed", SEEDS)
+@pytest.mark.parametrize("seed", SEEDS + EXTRA_SEEDS)
 def test_random_scene_guided_alpha_zero(gpu_ctx, seed):
     """The guided wavefront at alpha = 0 (its own shading kernels, training records on)
     is the reference estimator with per-sample streams: equal to the oracle's image."""
