@@ -3,7 +3,7 @@ and triangles (with and without shading normals and texture coordinates), all ma
 kinds (diffuse, metal, dielectric, emissive, textured diffuse), sky, black and
 environment-map backgrounds, 1-6 samples and 0-6 bounces.  Image tolerance of
 tests/test_gpu_parity.py.  TOASTER_EXTRA_RANDOM_SEEDS="lo:hi" adds seeds lo..hi-1 to the
-single-frame, two-frame and guided comparisons (a wider sweep than the default seeds)."""
+single-frame, two-frame, guided and refit comparisons (a wider sweep than the default seeds)."""
 import os
 
 import numpy as np
@@ -196,7 +196,7 @@ def test_random_scene_guided_alpha_zero(gpu_ctx, seed):
     gs.close()
 
 
-@pytest.mark.parametrize("seed", SEEDS[:6])
+@pytest.mark.parametrize("seed", SEEDS[:6] + EXTRA_SEEDS)
 def test_random_scene_refit_after_motion(gpu_ctx, seed):
     """The random scene uploaded once, then moved (every other triangle rotated about the
     y axis, spheres shifted) and refitted on the device: equal to the oracle's image of the
