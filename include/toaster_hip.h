@@ -360,7 +360,10 @@ int32_t th_readback(th_context* ctx, th_scene* scene, float* rgba, size_t bytes)
 /* Overwrites the accumulation buffer (progressive resume from a checkpoint). */
 int32_t th_upload_accumulation(th_context* ctx, th_scene* scene, const float* rgba, size_t bytes);
 int32_t th_synchronize(th_context* ctx);
-/* Waits for pending renders, then reports timing and (if collected) counters. */
+/* Waits for pending renders, then reports timing and (if collected) counters.  Every
+ * launch is timed by an event pair folded in launch order; a caller that never reads the
+ * statistics still holds at most 4,096 unfolded pairs (the oldest 2,048 are folded, after
+ * waiting for them, when the bound is reached). */
 int32_t th_get_stats(const th_context* ctx, th_stats* out);
 int32_t th_reset_stats(th_context* ctx);
 
