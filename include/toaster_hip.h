@@ -325,7 +325,14 @@ const char* th_last_error(void);
 int32_t th_scene_create(th_context* ctx, const th_scene_desc* desc, uint32_t node_capacity,
                         th_scene** out);
 /* Per-frame update of the arrays whose bit is set (path_tracer.rs:676-720); a bit not
- * defined below is rejected (TH_E_INVALID) before anything is uploaded. */
+ * defined below is rejected (TH_E_INVALID) before anything is uploaded.  Counts are fixed
+ * after create (path_tracer.rs:639-653): texels and environment pixels are never
+ * replaced, so a descriptor carrying either must carry the uploaded count (TH_E_MISMATCH
+ * otherwise; a per-frame pack without them carries 0) and material texture ranges are
+ * checked against the uploaded texels.  Moved spheres or triangles upload the
+ * descriptor's tree with them, as path_tracer.rs:706-719 does (TH_CHANGE_BVH implied); a
+ * descriptor without a tree for a scene that has one is refused (TH_E_INVALID) unless
+ * the update is a refit. */
 #define TH_CHANGE_CAMERA 1u
 #define TH_CHANGE_SPHERES 2u
 #define TH_CHANGE_TRIANGLES 4u
@@ -451,8 +458,10 @@ typedef struct th_guided_options {
                               /*   of the call only (0 or 1 = every pass)              */
     uint32_t reserved[3];
 } th_guided_options;
-/* At guided vertices, estimate environment light by the continuation alone (no
- * environment sample and shadow ray; an escaping continuation counts with weight 1). */
+/* At every diffuse vertex (also while alpha is 0, so the estimator with and without the
+ * guide can be compared), estimate environment light by the continuation alone (no
+ * environment sample and shadow ray; an escaping continuation counts with weight 1).
+ * Without this flag the reference's environment sample and MIS are kept. */
 #define TH_GUIDED_NO_ENV_NEE 1u
 
 typedef struct th_guided_stats {

@@ -1,0 +1,216 @@
+"""The benchmark workloads at their full shapes against the oracle on the GPU.
+
+- Workload 2 at 65,536 triangles: the frame bench.py times (320x180, 4 spp, 4 bounces,
+  reference streams, frame seed 0) through every kernel and schedule, plus per-sample
+  streams and the primary hits.
+- Workload 3 with its 1024x512 sky + sun HDR (sun disk of 0.5 degrees at radiance 5e4,
+  sampled through the environment CDF): 320x180 at 4 spp and 8 bounces through every
+  kernel, and the full 1280x720 frame at 64 spp.
+- Workload 5, the 262,144-triangle terrain at 1280x720: the plain kernels and the guided
+  wavefront at alpha = 0 (with and without training records) against the oracle, and the
+  guided estimator (alpha = 0.8, trained online) unbiased against a plain render.
+
+Tolerance (binary32 image, per channel, as tests/test_gpu_parity.py): |gpu - oracle| <=
+1e-4 * (1 + |oracle|) on at least 99.5 % of pixels.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from toaster_amd import GpuScene, RenderOptions, _abi as A, scenes
+from toaster_amd.guide import Guide, render_guided
+
+pytestmark = pytest.mark.gpu
+
+TOL_FRAC = 0.995
+ORACLE_THREADS = 16
+
+WIDE = A.TH_TRAVERSE_WIDE
+KERNELS = {
+    "reference": RenderOptions(),
+    "ordered": RenderOptions(traversal=A.TH_TRAVERSE_ORDERED),
+    "quad": RenderOptions(traversal=WIDE),
+    "dynamic": RenderOptions(traversal=WIDE, schedule=A.TH_SCHEDULE_DYNAMIC),
+    "octet": RenderOptions(traversal=WIDE, schedule=A.TH_SCHEDULE_OCTET),
+    "speculative": RenderOptions(traversal=WIDE, schedule=A.TH_SCHEDULE_SPECULATIVE),
+    "wavefront": RenderOptions(traversal=WIDE, kernel=A.TH_KERNEL_WAVEFRONT),
+}
+
+_cache = {}
+
+
+def _scene(key):
+    if key not in _cache:
+        _cache[key] = {
+            "stress65536": lambda: scenes.bvh_stress(65536),
+            "mixed_sun_320": lambda: scenes.mixed_environment(320, 180, samples=4, max_bounces=8),
+            "mixed_sun_1280": lambda: scenes.mixed_environment(1280, 720, samples=64, max_bounces=8),
+            "terrain": lambda: scenes.terrain(1280, 720, samples=2, max_bounces=8),
+        }[key]()
+    return _cache[key]
+
+
+def _oracle(key, rng_mode=A.TH_RNG_REFERENCE, group=1):
+    k = ("oracle", key, rng_mode, group)
+    if k not in _cache:
+        print(f"  oracle {key} rng={rng_mode} group={group}", flush=True)
+        _cache[k] = oracle.OracleScene(_scene(key)).render(threads=ORACLE_THREADS, rng_mode=rng_mode,
+                                                           group=group)[0]
+    return _cache[k]
+
+
+def _compare(img, ref):
+    close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
+    frac = float(np.mean(np.all(close, axis=-1)))
+    exact = float(np.mean(np.all(img == ref, axis=-1)))
+    return frac, exact
+
+
+def _render(ctx, sc, opts):
+    gs = GpuScene.from_scene(ctx, sc)
+    gs.render(options=opts, frame_index=0, accumulated_samples=0)
+    img = gs.readback().copy()
+    gs.close()
+    return img
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+def test_stress65536_headline_frame(gpu_ctx, kernel):
+    sc = _scene("stress65536")
+    print(f"case stress65536 320x180x4spp kernel={kernel}", flush=True)
+    img = _render(gpu_ctx, sc, KERNELS[kernel])
+    frac, exact = _compare(img, _oracle("stress65536"))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert np.isfinite(img).all()
+    assert frac >= TOL_FRAC
+
+
+@pytest.mark.parametrize("traversal,schedule,group", [(WIDE, A.TH_SCHEDULE_STATIC, 4),
+                                                      (WIDE, A.TH_SCHEDULE_OCTET, 4),
+                                                      (A.TH_TRAVERSE_ORDERED, A.TH_SCHEDULE_STATIC, 4)])
+def test_stress65536_per_sample_streams(gpu_ctx, traversal, schedule, group):
+    sc = _scene("stress65536")
+    print(f"case stress65536 per-sample traversal={traversal} schedule={schedule}", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=traversal,
+                                             schedule=schedule))
+    frac, exact = _compare(img, _oracle("stress65536", A.TH_RNG_PER_SAMPLE, group))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert frac >= TOL_FRAC
+
+
+def test_stress65536_primary_hits(gpu_ctx):
+    sc = _scene("stress65536")
+    print("case stress65536 primary hits", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    ref_slot, ref_t = oracle.OracleScene(sc).primary_hits()
+    for trav in (A.TH_TRAVERSE_REFERENCE, A.TH_TRAVERSE_ORDERED, WIDE):
+        slot, t = gs.primary_hits(traversal=trav)
+        same = slot == ref_slot
+        print(f"  traversal={trav}: identical {same.mean():.6f}")
+        assert same.mean() >= 0.9999
+        assert np.array_equal(t[same], ref_t[same])
+    gs.close()
+
+
+def test_sun_environment_has_its_disk():
+    """The workload-3 map holds the 0.5-degree sun at 5e4 (8 texels near elevation 45,
+    azimuth 30), and its CDF gives those few texels over 40 % of the selection
+    probability (the other 524,280 sky texels share the rest)."""
+    sc = _scene("mixed_sun_320")
+    rgb = sc.environment.rgb
+    assert rgb.shape == (512, 1024, 3)
+    sun = rgb[..., 0] >= 5.0e4
+    assert 1 <= sun.sum() <= 32
+    from toaster_amd.scene import pack
+    env = pack(sc).env
+    cdf = env[:, 3].astype(np.float64)
+    p = np.diff(np.concatenate([[0.0], cdf]))
+    assert p[sun.ravel()].sum() > 0.4
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+def test_mixed_sun_320(gpu_ctx, kernel):
+    sc = _scene("mixed_sun_320")
+    print(f"case mixed+sun 320x180x4spp x8 kernel={kernel}", flush=True)
+    img = _render(gpu_ctx, sc, KERNELS[kernel])
+    frac, exact = _compare(img, _oracle("mixed_sun_320"))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f} max {img.max():.1f}")
+    assert np.isfinite(img).all()
+    assert frac >= TOL_FRAC
+
+
+def test_mixed_sun_320_per_sample_streams(gpu_ctx):
+    sc = _scene("mixed_sun_320")
+    print("case mixed+sun 320x180 per-sample quad", flush=True)
+    img = _render(gpu_ctx, sc, RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=WIDE))
+    frac, exact = _compare(img, _oracle("mixed_sun_320", A.TH_RNG_PER_SAMPLE, 4))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert frac >= TOL_FRAC
+
+
+@pytest.mark.parametrize("kernel", ["quad", "speculative"])
+def test_mixed_sun_full_frame(gpu_ctx, kernel):
+    """Workload 3 exactly: 1280x720, 64 spp, 8 bounces, sky + sun HDR."""
+    sc = _scene("mixed_sun_1280")
+    print(f"case mixed+sun 1280x720x64spp kernel={kernel}", flush=True)
+    img = _render(gpu_ctx, sc, KERNELS[kernel])
+    frac, exact = _compare(img, _oracle("mixed_sun_1280"))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert np.isfinite(img).all()
+    assert frac >= TOL_FRAC
+
+
+@pytest.mark.parametrize("kernel", ["quad", "wavefront"])
+def test_terrain_plain(gpu_ctx, kernel):
+    sc = _scene("terrain")
+    print(f"case terrain 262144 1280x720x2spp per-sample kernel={kernel}", flush=True)
+    opts = RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=WIDE,
+                         kernel=A.TH_KERNEL_WAVEFRONT if kernel == "wavefront" else A.TH_KERNEL_MEGA)
+    img = _render(gpu_ctx, sc, opts)
+    frac, exact = _compare(img, _oracle("terrain", A.TH_RNG_PER_SAMPLE, 2))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f}")
+    assert frac >= TOL_FRAC
+
+
+@pytest.mark.parametrize("train", [False, True])
+def test_terrain_guided_alpha_zero(gpu_ctx, train):
+    sc = _scene("terrain")
+    print(f"case terrain 262144 guided alpha=0 train={train}", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    g = Guide(gpu_ctx, seed=3)
+    st = render_guided(gs, g, sc.samples, alpha=0.0, train=train, batch=1 << 15, progressive=False,
+                       warmup_steps=0)
+    img = gs.readback().copy()
+    gs.close()
+    g.close()
+    frac, exact = _compare(img, _oracle("terrain", A.TH_RNG_PER_SAMPLE, 1))
+    print(f"  within tol {frac:.5f} bit-exact {exact:.5f} stats {st}")
+    assert frac >= TOL_FRAC
+    if train:
+        assert st["records"] > 0 and st["train_steps"] > 0
+
+
+def test_terrain_guided_is_unbiased(gpu_ctx):
+    """At full size: the mean of a guided render (alpha 0.8, trained online from scratch,
+    environment light by the continuation alone) agrees with a 1,024-spp plain render's
+    within 4 standard errors of 16 independent chunk means."""
+    sc = _scene("terrain")
+    print("case terrain 262144 guided unbiased", flush=True)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    gs.render(samples=1024, frame_index=77, accumulated_samples=0,
+              options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=WIDE))
+    ref = float(gs.readback()[..., :3].mean())
+    g = Guide(gpu_ctx, seed=8)
+    kw = dict(alpha=0.8, env_nee=False, batch=1 << 15, warmup_steps=128, warmup_rate=8, alpha_ramp=64)
+    gs.frame_index = 500
+    render_guided(gs, g, 16, train=True, progressive=False, **kw)
+    means = []
+    for c in range(16):
+        gs.frame_index = 1000 + c
+        render_guided(gs, g, 8, train=True, progressive=False, **kw)
+        means.append(float(gs.readback()[..., :3].mean()))
+    gs.close()
+    g.close()
+    m, se = float(np.mean(means)), float(np.std(means, ddof=1) / np.sqrt(len(means)))
+    print(f"  guided mean {m:.6f} +- {se:.6f}, plain 1024 spp {ref:.6f}")
+    assert abs(m - ref) <= 4 * se + 2e-3 * ref

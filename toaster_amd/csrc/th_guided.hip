@@ -350,8 +350,8 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
             park = true;
           } else {
             float pdf;
-            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, true, throughput, rng, nd,
-                          pdf, sh_light, sh_env);
+            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, A.env_nee != 0u, throughput, rng,
+                          nd, pdf, sh_light, sh_env);
             flags = GD_PREV_DIFFUSE;
             o_next = make_float4(rec.p.x, rec.p.y, rec.p.z, pdf);
             if (A.bounce + 1u >= P.max_bounces) done = true;
@@ -701,8 +701,9 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
                                                              : opt->alpha);
     A.record = train_pass ? 1u : 0u;
     A.defer = (A.alpha > 0.0f || train_pass) ? 1u : 0u;
-    // the reference's environment sampling stays on until the guide is in use
-    A.env_nee = (A.alpha > 0.0f && (opt->flags & TH_GUIDED_NO_ENV_NEE)) ? 0u : 1u;
+    // environment sampling at diffuse vertices as the caller asks, whatever alpha is, so
+    // the guide's gain can be measured against the same estimator without it
+    A.env_nee = (opt->flags & TH_GUIDED_NO_ENV_NEE) ? 0u : 1u;
     if (train_pass) {
       double pr = (double)target / (double)std::max<uint64_t>(w->diffuse_per_pass, 1);
       A.rec_all = pr >= 1.0 ? 1u : 0u;

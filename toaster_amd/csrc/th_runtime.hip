@@ -749,7 +749,10 @@ int32_t validate_bvh(const th_scene_desc* d) {
   return TH_OK;
 }
 
-int32_t validate_refs(const th_scene_desc* d) {
+// Texture and environment ranges are checked against the pixel counts of the arrays the
+// kernels will read: the descriptor's at create, the uploaded ones at an update (an update
+// never replaces texels or environment pixels).
+int32_t validate_refs(const th_scene_desc* d, uint64_t tex_count, uint64_t env_count) {
   for (uint32_t i = 0; i < d->sphere_count; ++i)
     if (d->spheres[i].material_index >= d->material_count)
       return th_set_error(TH_E_INVALID, "sphere material index out of range");
@@ -762,20 +765,20 @@ int32_t validate_refs(const th_scene_desc* d) {
   for (uint32_t i = 0; i < d->material_count; ++i) {
     const TH_GpuMaterial& m = d->materials[i];
     if (m.texture_width && m.texture_height &&
-        (uint64_t)m.texture_offset + (uint64_t)m.texture_width * m.texture_height > d->texture_pixel_count)
+        (uint64_t)m.texture_offset + (uint64_t)m.texture_width * m.texture_height > tex_count)
       return th_set_error(TH_E_INVALID, "material texture out of range");
   }
   const TH_GpuRenderParams& p = d->params;
   if (p.background_kind > 2u) return th_set_error(TH_E_INVALID, "unknown background kind");
   if (p.background_kind == 2u &&
       (p.environment_width == 0 || p.environment_height == 0 ||
-       (uint64_t)p.environment_width * p.environment_height != d->environment_pixel_count))
+       (uint64_t)p.environment_width * p.environment_height != env_count))
     return th_set_error(TH_E_INVALID, "environment background requires an environment map");
   if (p.light_count != d->light_count) return th_set_error(TH_E_MISMATCH, "light count mismatch");
   return TH_OK;
 }
 
-int32_t validate_desc(const th_scene_desc* d) {
+int32_t validate_desc(const th_scene_desc* d, uint64_t tex_count, uint64_t env_count) {
   if (!d) return th_set_error(TH_E_INVALID, "null scene descriptor");
   if (d->sphere_count + (uint64_t)d->triangle_count == 0)
     return th_set_error(TH_E_INVALID, "GPU renderer requires at least one object");
@@ -792,9 +795,14 @@ int32_t validate_desc(const th_scene_desc* d) {
       d->params.material_count != d->material_count || d->params.bvh_node_count != d->bvh_node_count ||
       d->params.bvh_primitive_count != d->bvh_primitive_count)
     return th_set_error(TH_E_MISMATCH, "params counts disagree with the arrays");
-  int32_t rc = validate_refs(d);
+  int32_t rc = validate_refs(d, tex_count, env_count);
   if (rc) return rc;
   return validate_bvh(d);
+}
+
+int32_t validate_desc(const th_scene_desc* d) {
+  if (!d) return th_set_error(TH_E_INVALID, "null scene descriptor");
+  return validate_desc(d, d->texture_pixel_count, d->environment_pixel_count);
 }
 
 // Collapses the reference binary tree into the 4-wide tree of th_wide.h (breadth-first
@@ -1213,6 +1221,16 @@ int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
     return th_set_error(TH_E_INVALID, "a refit needs new spheres or triangles");
   if (refit && s->node_count == 0)
     return th_set_error(TH_E_INVALID, "the scene has no bvh to refit");
+  // path_tracer.rs:706-719: moved spheres or triangles upload the frame's rebuilt tree
+  // with them whenever the frame has one; without a tree in the descriptor the uploaded
+  // one would no longer match the primitives, so only a refit may keep it.
+  if ((mask & (TH_CHANGE_SPHERES | TH_CHANGE_TRIANGLES)) && !refit && !(mask & TH_CHANGE_BVH)) {
+    if (d->bvh_node_count > 0)
+      mask |= TH_CHANGE_BVH;
+    else if (s->node_count > 0)
+      return th_set_error(TH_E_INVALID,
+                          "moved geometry needs its rebuilt bvh (TH_CHANGE_BVH) or a refit (TH_CHANGE_REFIT)");
+  }
   // path_tracer.rs:639-653: counts are fixed after setup; nodes within capacity.
   if (d->sphere_count != s->params.sphere_count || d->triangle_count != s->params.triangle_count ||
       d->material_count != s->params.material_count || d->light_count != s->params.light_count ||
@@ -1221,7 +1239,13 @@ int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
     return th_set_error(TH_E_MISMATCH, "scene counts changed after setup");
   if (d->bvh_node_count > s->node_capacity)
     return th_set_error(TH_E_CAPACITY, "bvh node count exceeds capacity");
-  int32_t rc = validate_desc(d);
+  // Texels and environment pixels stay as uploaded at create: a descriptor that carries
+  // them must carry the same counts (a per-frame pack without them carries 0), and
+  // material texture ranges are checked against the uploaded texels.
+  if ((d->texture_pixel_count && d->texture_pixel_count != s->tex_count) ||
+      (d->environment_pixel_count && d->environment_pixel_count != s->env_count))
+    return th_set_error(TH_E_MISMATCH, "texture or environment pixel count differs from the uploaded scene");
+  int32_t rc = validate_desc(d, s->tex_count, s->env_count);
   if (rc) return rc;
   TH_CHECK(hipSetDevice(ctx->device));
   bool geometry = false;

@@ -86,7 +86,7 @@ def mfma_selftest(ctx: GpuContext, a, b):
 
 
 # Defaults measured on the indirect-lit terrain (1280x720, 262,144 triangles): guide
-# probability 0.8, environment light by the continuation alone at guided vertices, Adam
+# probability 0.8, environment light by the continuation alone at diffuse vertices, Adam
 # steps of 2^15 samples, a warm-up of 768 steps at 16 steps per pass (48 passes) with
 # alpha ramped over the first 512, then one step every pass.
 GUIDED_DEFAULTS = dict(alpha=0.8, env_nee=False, batch=1 << 15, warmup_steps=768, warmup_rate=16,
@@ -94,11 +94,13 @@ GUIDED_DEFAULTS = dict(alpha=0.8, env_nee=False, batch=1 << 15, warmup_steps=768
 
 
 def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.8, train=True, steps_per_pass=1,
-                  batch=1 << 15, progressive=True, env_nee=False, warmup_steps=768, warmup_rate=16,
+                  batch=1 << 15, progressive=True, env_nee=True, warmup_steps=768, warmup_rate=16,
                   alpha_ramp=512, train_every=1):
     """`samples` passes of one sample per pixel with the guide (see th_render_guided);
     with progressive=True the running average and frame seed continue like
-    GpuScene.render_progressive.  Returns the call's statistics as a dict."""
+    GpuScene.render_progressive.  env_nee=False drops the environment sample at every
+    diffuse vertex (TH_GUIDED_NO_ENV_NEE); the default keeps the reference's estimator,
+    which alpha = 0 then reproduces.  Returns the call's statistics as a dict."""
     p = scene.params(samples=samples)
     if not progressive:
         p.accumulated_samples = 0
