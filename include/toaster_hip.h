@@ -381,6 +381,16 @@ int32_t th_primary_hits(th_context* ctx, th_scene* scene, const TH_GpuRenderPara
                         const TH_GpuCamera* camera, uint32_t traversal, uint32_t* slot_out,
                         float* distance_out, size_t count);
 
+/* The random stream of each listed pixel as the render kernels seed it, evaluated on the
+ * device: TH_RNG_REFERENCE seeds pcg_hash(idx ^ frame_index*9781) (wgsl:519),
+ * TH_RNG_PER_SAMPLE seeds pcg_hash((idx ^ frame_index*9781) + sample*0x9E3779B9); then
+ * `draws` draws of random_f32 (wgsl:189-199), per pixel the state after each draw and
+ * the drawn value (count x draws each, row per pixel; count x draws <= 2^24).  Parity
+ * query for the generator the kernels use; no counterpart in the reference. */
+int32_t th_rng_draws(th_context* ctx, const uint32_t* pixels, uint32_t count, uint32_t frame_index,
+                     uint32_t rng_mode, uint32_t sample, uint32_t draws, uint32_t* states_out,
+                     float* values_out);
+
 /* ---------------------------------------------------------- batching */
 /* One entry of a batched launch: a scene (or one animation frame uploaded as its own
  * scene), its dispatch params/camera and its launch shape.  No counterpart in the

@@ -71,6 +71,7 @@ _SIGS = {
     "or_power_heuristic": (F, [F, F]),
     "or_ray_color": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), F3, F3, C.POINTER(C.c_uint32), F3]),
     "or_pcg_hash": (U, [U]),
+    "or_rand01": (F, [C.POINTER(C.c_uint32)]),
     "or_sin": (F, [F]),
     "or_cos": (F, [F]),
     "or_atan2": (F, [F, F]),
@@ -217,6 +218,16 @@ def display_rgba8(pixels, exposure_stops=0.0):
 
 def pcg_hash(x):
     return lib().or_pcg_hash(int(x) & 0xFFFFFFFF)
+
+
+def stream(seed, n):
+    """n draws of random_f32 from state `seed`: (states after each draw, values)."""
+    st = C.c_uint32(int(seed) & 0xFFFFFFFF)
+    states, values = np.empty(n, np.uint32), np.empty(n, np.float32)
+    for k in range(n):
+        values[k] = lib().or_rand01(C.byref(st))
+        states[k] = st.value
+    return states, values
 
 
 def reflectance(c, r):

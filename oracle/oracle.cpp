@@ -1307,6 +1307,7 @@ int32_t or_ray_color(const or_scene* s, const TH_GpuRenderParams* params, const 
 }
 
 uint32_t or_pcg_hash(uint32_t x) { return th::pcg_hash(x); }
+float or_rand01(uint32_t* state) { return th::rand01(state); }
 float or_sin(float x) { return th::fsin(x); }
 float or_cos(float x) { return th::fcos(x); }
 float or_atan2(float y, float x) { return th::fatan2(y, x); }

@@ -184,6 +184,7 @@ SIGNATURES = {
     "th_synchronize": (C.c_int32, [P]),
     "th_get_stats": (C.c_int32, [P, C.POINTER(Stats)]),
     "th_reset_stats": (C.c_int32, [P]),
+    "th_rng_draws": (C.c_int32, [P, P, u32, u32, u32, u32, u32, P, P]),
     "th_primary_hits": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera), u32,
                                     P, P, C.c_size_t]),
     "th_render_batch": (C.c_int32, [P, C.POINTER(BatchItem), u32, C.POINTER(RenderOptions)]),

@@ -667,6 +667,24 @@ __global__ void __launch_bounds__(64) k_primary_quad(DevScene S, TH_GpuRenderPar
 }
 
 // Display conversion on the device (color.rs:42-55, image_output.rs:59-92).
+// Random streams of given pixels as the render kernels seed them: the reference stream
+// pcg_hash(idx ^ frame*9781) (wgsl:519), or the per-sample stream of sample s,
+// pcg_hash((idx ^ frame*9781) + s*0x9E3779B9); then `draws` draws of random_f32.
+__global__ void k_rng_draws(const uint32_t* __restrict__ pixels, uint32_t count, uint32_t frame_index,
+                            uint32_t per_sample, uint32_t sample, uint32_t draws, uint32_t* __restrict__ states,
+                            float* __restrict__ values) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t idx = pixels[i];
+  uint32_t st = per_sample ? th::pcg_hash((idx ^ (frame_index * 9781u)) + sample * 0x9E3779B9u)
+                           : th::pcg_hash(idx ^ (frame_index * 9781u));
+  for (uint32_t k = 0; k < draws; ++k) {
+    const float v = th::rand01(&st);
+    states[(size_t)i * draws + k] = st;
+    values[(size_t)i * draws + k] = v;
+  }
+}
+
 __global__ void k_display(const float4* __restrict__ accum, uint32_t n, float scale,
                           const float* __restrict__ thr, uchar4* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1594,6 +1612,36 @@ int32_t th_reset_stats(th_context* ctx) {
   int32_t rc = fold_timing(ctx);
   if (rc) return rc;
   memset(&ctx->stats, 0, sizeof ctx->stats);
+  return TH_OK;
+}
+
+int32_t th_rng_draws(th_context* ctx, const uint32_t* pixels, uint32_t count, uint32_t frame_index,
+                     uint32_t rng_mode, uint32_t sample, uint32_t draws, uint32_t* states_out,
+                     float* values_out) {
+  if (!ctx || !pixels || !states_out || !values_out || count == 0 || draws == 0)
+    return th_set_error(TH_E_INVALID, "null argument or empty query");
+  if (rng_mode > 1u) return th_set_error(TH_E_INVALID, "unknown rng mode");
+  if ((uint64_t)count * draws > (1u << 24)) return th_set_error(TH_E_INVALID, "query larger than 2^24 draws");
+  TH_CHECK(hipSetDevice(ctx->device));
+  const size_t n = (size_t)count * draws;
+  uint32_t *d_pix = nullptr, *d_st = nullptr;
+  float* d_v = nullptr;
+  hipError_t e = hipMalloc((void**)&d_pix, count * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_st, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_v, n * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpyAsync(d_pix, pixels, count * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rng_draws, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, d_pix, count,
+                       frame_index, rng_mode, sample, draws, d_st, d_v);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(states_out, d_st, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(values_out, d_v, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (d_pix) (void)hipFree(d_pix);
+  if (d_st) (void)hipFree(d_st);
+  if (d_v) (void)hipFree(d_v);
+  if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("rng draws: ") + hipGetErrorString(e));
   return TH_OK;
 }
 
