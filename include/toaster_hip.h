@@ -451,7 +451,7 @@ int32_t th_guide_train_batch(th_guide* guide, const float* features, const float
  * density in the environment MIS, so it stays unbiased for any network.  With train != 0
  * every pass records diffuse vertices (features, sampled direction, and the weight
  * f cos L_i / pdf once the path completes) and runs Adam steps on them; alpha is treated
- * as 0 until the guide has taken its first step.  Every schedule depends only on the
+ * as 0 until the guide has taken its first step (unless TH_GUIDED_ALPHA_FROM_START).  Every schedule depends only on the
  * guide's step count, so the estimator of each pass is fixed before the pass runs.  With alpha = 0 and train = 0 the image
  * equals th_render's TH_RNG_PER_SAMPLE image with one sample group. */
 typedef struct th_guided_options {
@@ -468,11 +468,18 @@ typedef struct th_guided_options {
                               /*   of the call only (0 or 1 = every pass)              */
     uint32_t reserved[3];
 } th_guided_options;
-/* At every diffuse vertex (also while alpha is 0, so the estimator with and without the
- * guide can be compared), estimate environment light by the continuation alone (no
- * environment sample and shadow ray; an escaping continuation counts with weight 1).
- * Without this flag the reference's environment sample and MIS are kept. */
+/* At every diffuse vertex but a path's last (also while alpha is 0, so the estimator
+ * with and without the guide can be compared), estimate environment light by the
+ * continuation alone (no environment sample and shadow ray; an escaping continuation
+ * counts with weight 1).  Without this flag the reference's environment sample and MIS
+ * are kept.  A path's last diffuse vertex, whose continuation is never traced, always
+ * takes the reference's environment sample weighted against the cosine lobe
+ * (wgsl:496-503), so every alpha and flag estimates the reference's image. */
 #define TH_GUIDED_NO_ENV_NEE 1u
+/* Apply alpha from the first pass even while the guide has taken no Adam step, so an
+ * untrained network (as created or as set by th_guide_set_params) can be measured;
+ * with alpha_ramp > 0 the ramp still starts from 0.  Other flag bits are rejected. */
+#define TH_GUIDED_ALPHA_FROM_START 2u
 
 typedef struct th_guided_stats {
     uint64_t diffuse_vertices; /* deferred diffuse vertices over the call               */

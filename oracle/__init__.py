@@ -52,6 +52,7 @@ F3 = C.POINTER(C.c_float)
 _SIGS = {
     "or_pack": (C.c_int32, [P, C.POINTER(P)]),
     "or_view": (C.c_int32, [P, C.POINTER(A.SceneDesc)]),
+    "or_drop_bvh": (C.c_int32, [P]),
     "or_free": (None, [P]),
     "or_last_error": (C.c_char_p, []),
     "or_render": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), U, U,
@@ -72,6 +73,7 @@ _SIGS = {
     "or_ray_color": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), F3, F3, C.POINTER(C.c_uint32), F3]),
     "or_pcg_hash": (U, [U]),
     "or_rand01": (F, [C.POINTER(C.c_uint32)]),
+    "or_env_search_rows": (U, [P, P, U, U, F]),
     "or_sin": (F, [F]),
     "or_cos": (F, [F]),
     "or_atan2": (F, [F, F]),
@@ -107,12 +109,15 @@ def _check(code):
 class OracleScene:
     """The oracle's own packing of a Scene (restating scene_to_gpu)."""
 
-    def __init__(self, scene: Scene):
+    def __init__(self, scene: Scene, bvh=True):
+        """bvh=False: no tree, every ray takes the linear scan (wgsl:667-698, 797-802)."""
         self.scene = scene
         inp, keep = scene.input()
         self._h = C.c_void_p()
         _check(lib().or_pack(C.byref(inp), C.byref(self._h)))
         del keep
+        if not bvh:
+            _check(lib().or_drop_bvh(self._h))
         d = A.SceneDesc()
         _check(lib().or_view(self._h, C.byref(d)))
         self.packed = PackedScene(d)
@@ -228,6 +233,13 @@ def stream(seed, n):
         values[k] = lib().or_rand01(C.byref(st))
         states[k] = st.value
     return states, values
+
+
+def env_search_rows(cdf, width, height, sel):
+    """Row-then-texel CDF search of the kernels (th::env_search_rows) on a (H*W,) CDF."""
+    cdf = np.ascontiguousarray(cdf, np.float32)
+    rows = np.ascontiguousarray(cdf.reshape(height, width)[:, -1])
+    return lib().or_env_search_rows(cdf.ctypes.data, rows.ctypes.data, width, height, float(sel))
 
 
 def reflectance(c, r):

@@ -984,6 +984,17 @@ int32_t or_pack(const th_scene_input* in, or_scene** out) {
   return TH_OK;
 }
 
+// Drops the packed BVH so renders take the linear scan (wgsl:797-802), as a scene
+// uploaded with bvh_node_count == 0 does.
+int32_t or_drop_bvh(or_scene* s) {
+  if (!s) return fail("null argument");
+  s->nodes.clear();
+  s->prims.clear();
+  s->params.bvh_node_count = 0;
+  s->params.bvh_primitive_count = 0;
+  return TH_OK;
+}
+
 int32_t or_view(const or_scene* s, th_scene_desc* d) {
   if (!s || !d) return fail("null argument");
   memset(d, 0, sizeof *d);
@@ -1308,6 +1319,11 @@ int32_t or_ray_color(const or_scene* s, const TH_GpuRenderParams* params, const 
 
 uint32_t or_pcg_hash(uint32_t x) { return th::pcg_hash(x); }
 float or_rand01(uint32_t* state) { return th::rand01(state); }
+// The row-then-texel CDF search the kernels use (th_math.h), for its check against
+// the flat partition point.
+uint32_t or_env_search_rows(const float* cdf, const float* rows, uint32_t w, uint32_t h, float sel) {
+  return th::env_search_rows(cdf, rows, w, h, sel);
+}
 float or_sin(float x) { return th::fsin(x); }
 float or_cos(float x) { return th::fcos(x); }
 float or_atan2(float y, float x) { return th::fatan2(y, x); }

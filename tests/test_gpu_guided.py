@@ -4,8 +4,9 @@
   oracle's per-sample image with one sample group (same tolerance as the plain paths),
   with and without training records (the deferred diffuse path).
 - alpha > 0: the estimator must stay unbiased for any network: the image mean of a
-  trained-online guided render agrees with a high-spp plain render within 4 standard
-  errors (measured from independent chunks).
+  trained-online guided render agrees with a high-spp render of the same estimator
+  without a network within 4 combined standard errors (measured from independent
+  chunks), with and without the environment sample.
 - Training gains, with the estimator held fixed (environment light by the continuation
   alone): at equal samples per pixel the online-trained guide lowers relRMSE against a
   high-spp plain reference on the indirect-lit terrain, compared with the same mixture
@@ -58,30 +59,64 @@ def _terrain_small():
     return scenes.terrain(64, 36, samples=16, max_bounces=6, cells=(128, 64), env_size=(512, 256))
 
 
-@pytest.mark.parametrize("alpha,env_nee", [(0.7, True), (0.7, False), (0.0, False)])
+def _reference_mean(gs, env_nee, spp, parts=4):
+    """Image mean (and its standard error over `parts` independent renders) of the
+    estimator without a network: th_render's plain kernel when the environment sample is
+    kept (the reference estimator), the guided wavefront at alpha = 0 without it."""
+    ms = []
+    for c in range(parts):
+        if env_nee:
+            gs.render(samples=spp // parts, frame_index=77 + c, accumulated_samples=0,
+                      options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=A.TH_TRAVERSE_WIDE))
+        else:
+            gs.frame_index = 7000 + c
+            g0 = Guide(gs.ctx, seed=1)
+            render_guided(gs, g0, spp // parts, alpha=0.0, train=False, env_nee=False, progressive=False,
+                          warmup_steps=0)
+            g0.close()
+        ms.append(float(gs.readback()[..., :3].mean()))
+    return float(np.mean(ms)), float(np.std(ms, ddof=1) / np.sqrt(parts))
+
+
+def guided_unbiased(gs, ctx, alpha, env_nee, batch, warmup, chunks, chunk_spp, ref_spp, seed=4):
+    """Image mean of a guided render trained online from scratch, over `chunks` independent
+    renders of `chunk_spp` passes, against the same estimator without a network; returns
+    (guided mean, its standard error, reference mean, its standard error)."""
+    r, rse = _reference_mean(gs, env_nee, ref_spp)
+    g = Guide(ctx, seed=seed)
+    kw = dict(alpha=alpha, env_nee=env_nee, batch=batch, warmup_steps=warmup, warmup_rate=4,
+              alpha_ramp=warmup)
+    gs.frame_index = 500
+    render_guided(gs, g, 2 * chunk_spp, train=True, progressive=False, **kw)
+    means = []
+    for c in range(chunks):
+        gs.frame_index = 1000 + c
+        render_guided(gs, g, chunk_spp, train=True, progressive=False, **kw)
+        means.append(float(gs.readback()[..., :3].mean()))
+    g.close()
+    return float(np.mean(means)), float(np.std(means, ddof=1) / np.sqrt(chunks)), r, rse
+
+
+@pytest.mark.parametrize("alpha,env_nee", [(0.7, True), (0.7, False), (0.9, False)])
 def test_guided_estimator_is_unbiased(gpu_ctx, alpha, env_nee):
-    """The mixture estimator with and without the environment sample, and the cosine
-    lobe alone without it (alpha = 0, environment light by the continuation only)."""
+    """The image mean of the mixture estimator (trained online, so the network changes
+    between passes) equals that of the estimator without a network within 4 combined
+    standard errors.  With the environment sample kept the comparison is the plain
+    kernel, i.e. the reference estimator.  Without it the comparison is the plain
+    estimator without it: the reference's environment shadow ray starts 0.001 along the
+    normal (wgsl:496) while a continuation starts on the surface, so near creases the
+    two strategies see slightly different geometry (0.08 % of the image mean on the
+    1280x720 terrain, measured), which is the reference's estimator, not the guide."""
     sc = _terrain_small()
     gs = GpuScene.from_scene(gpu_ctx, sc)
-    print(f"case guided unbiased alpha={alpha} env_nee={env_nee}: plain reference", flush=True)
-    gs.render(samples=4096, options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE,
-                                                  traversal=A.TH_TRAVERSE_WIDE), frame_index=77)
-    ref = gs.readback()[..., :3].mean(-1)
-    g = Guide(gpu_ctx, seed=4)
-    kw = dict(alpha=alpha, env_nee=env_nee, batch=4096, warmup_steps=16, warmup_rate=4, alpha_ramp=16)
-    render_guided(gs, g, 32, train=True, progressive=False, **kw)
-    means = []
-    for c in range(16):
-        gs.frame_index = 1000 + c
-        render_guided(gs, g, 32, train=True, progressive=False, **kw)
-        means.append(float(gs.readback()[..., :3].mean()))
-    m, se = float(np.mean(means)), float(np.std(means, ddof=1) / np.sqrt(len(means)))
-    r = float(ref.mean())
-    print(f"  alpha={alpha} env_nee={env_nee}: guided mean {m:.5f} +- {se:.5f}, plain 4096 spp {r:.5f}")
-    assert abs(m - r) <= 4 * se + 2e-3 * r
+    print(f"case guided unbiased alpha={alpha} env_nee={env_nee}", flush=True)
+    m, se, r, rse = guided_unbiased(gs, gpu_ctx, alpha, env_nee, batch=4096, warmup=16, chunks=48, chunk_spp=16,
+                                    ref_spp=8192)
     gs.close()
-    g.close()
+    z = (m - r) / np.hypot(se, rse)
+    print(f"  alpha={alpha} env_nee={env_nee}: guided mean {m:.6f} +- {se:.6f}, without network "
+          f"{r:.6f} +- {rse:.6f}, z {z:.2f}")
+    assert abs(z) <= 4.0
 
 
 def test_trained_network_gain_with_the_estimator_fixed(gpu_ctx):
@@ -89,8 +124,8 @@ def test_trained_network_gain_with_the_estimator_fixed(gpu_ctx):
     light by the continuation alone at every diffuse vertex, 64 spp, the same sample
     streams.  At equal samples the trained guide (alpha 0.8) has lower relRMSE against an
     8,192-spp plain reference than the same one-sample mixture estimator with the network
-    frozen at its initial weights (alpha 0.8; one Adam step at learning rate 0 puts it in
-    use unchanged), and lower than the estimator without a network (alpha = 0)."""
+    at its initial weights (alpha 0.8 from the first pass, no training), and lower than the
+    estimator without a network (alpha = 0)."""
     sc = _terrain_small()
     gs = GpuScene.from_scene(gpu_ctx, sc)
     o = RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=A.TH_TRAVERSE_WIDE)
@@ -104,13 +139,11 @@ def test_trained_network_gain_with_the_estimator_fixed(gpu_ctx):
     no_network = rel_rmse(gs.readback(), ref)
     gu = Guide(gpu_ctx, seed=6)
     w0 = gu.get_params()
-    gu.set_hyper(lr=0.0)
-    gs.frame_index = 3
-    st = render_guided(gs, gu, 1, alpha=0.8, train=True, warmup_steps=0, alpha_ramp=0, **common)
-    assert st["train_steps"] == 1
-    assert np.array_equal(gu.get_params(), w0)
     gs.frame_index = 12
-    render_guided(gs, gu, 64, alpha=0.8, train=False, warmup_steps=0, alpha_ramp=0, **common)
+    st = render_guided(gs, gu, 64, alpha=0.8, train=False, warmup_steps=0, alpha_ramp=0, alpha_from_start=True,
+                       **common)
+    assert st["train_steps"] == 0
+    assert np.array_equal(gu.get_params(), w0)
     untrained = rel_rmse(gs.readback(), ref)
     gt = Guide(gpu_ctx, seed=6)
     kw = dict(alpha=0.8, warmup_steps=64, warmup_rate=4, alpha_ramp=64, train_every=1, **common)

@@ -28,6 +28,7 @@ TH_SCHEDULE_STATIC, TH_SCHEDULE_DYNAMIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULAT
 
 TH_GUIDE_FEATURES, TH_GUIDE_OUTPUTS, TH_GUIDE_PARAMS = 48, 40, 14056
 TH_GUIDED_NO_ENV_NEE = 1
+TH_GUIDED_ALPHA_FROM_START = 2
 
 TH_CHANGE_CAMERA = 1
 TH_CHANGE_SPHERES = 2

@@ -55,6 +55,11 @@ struct DevScene {
   const float4* nodes4;  // 4-wide nodes (th_wide.h), 8 float4 per node
   uint32_t root4;        // 4-wide traversal entry of the root
   uint32_t node4_bytes;  // size of the nodes4 array
+  // environment CDF, compact: the .w of every texel (4 B each) and of the last texel of
+  // every row; null when the map's CDF is not non-decreasing (then the flat search)
+  const float* env_cdf;
+  const float* env_rows;
+  uint32_t env_rows_w;   // row width the row array was built for
 };
 
 struct Counters {

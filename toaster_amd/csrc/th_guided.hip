@@ -208,11 +208,19 @@ __global__ void __launch_bounds__(64) gd_shadow(GdBufs B, DevScene S, TH_GpuRend
 // continuation is the reference's cosine lobe (wgsl:849-857), bit for bit; otherwise
 // one-sample MIS between the guide and the cosine lobe.  Returns false when the
 // continuation has zero contribution (a guide direction below the surface).
+//
+// The last vertex of a path (`last`: no continuation will be traced, wgsl:812) keeps
+// the reference's environment sample with the reference's weight (power heuristic
+// against the cosine lobe, wgsl:496-503) whatever alpha and env_nee are: the reference
+// estimator credits that share of the next bounce, so every variant here estimates the
+// same quantity as the reference (without it, an estimator that drops the environment
+// sample, or weighs it against the mixture, converges to a slightly different image).
 __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, false>& it,
                                               const TH_GpuRenderParams& P, uint32_t k, const v3& p,
                                               const v3& n, const v3& albedo, const tg::Mixture* m,
-                                              float alpha, bool env_nee, v3& throughput, uint32_t& rng,
-                                              v3& nd, float& pdf, bool& sh_light, bool& sh_env) {
+                                              float alpha, bool env_nee, bool last, v3& throughput,
+                                              uint32_t& rng, v3& nd, float& pdf, bool& sh_light,
+                                              bool& sh_env) {
   const bool guided = m != nullptr && alpha > 0.0f;
   {
     v3 so, dir, term;
@@ -225,12 +233,12 @@ __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, fal
       sh_light = true;
     }
   }
-  if (P.background_kind == 2u && env_nee) {
+  if (P.background_kind == 2u && (env_nee || last)) {
     EnvSample es;
     float sc;
     if (it.env_candidate(n, &rng, &es, &sc)) {
       float bp = sc / TH_PI;
-      if (guided) bp = alpha * tg::mixture_pdf(*m, es.dir) + (1.0f - alpha) * bp;
+      if (guided && !last) bp = alpha * tg::mixture_pdf(*m, es.dir) + (1.0f - alpha) * bp;
       v3 c = th::mul(throughput, it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, bp)));
       v3 so = th::add(p, th::muls(n, TH_MIN_DISTANCE));
       B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
@@ -350,8 +358,8 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
             park = true;
           } else {
             float pdf;
-            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, A.env_nee != 0u, throughput, rng,
-                          nd, pdf, sh_light, sh_env);
+            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, A.env_nee != 0u,
+                          A.bounce + 1u >= P.max_bounces, throughput, rng, nd, pdf, sh_light, sh_env);
             flags = GD_PREV_DIFFUSE;
             o_next = make_float4(rec.p.x, rec.p.y, rec.p.z, pdf);
             if (A.bounce + 1u >= P.max_bounces) done = true;
@@ -459,7 +467,8 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       }
       float pdf;
       bool cont = shade_diffuse(B, it, P, k, p, nrm, albedo, use_guide ? &mix : nullptr, A.alpha,
-                                A.env_nee != 0u, throughput, rng, nd, pdf, sh_light, sh_env);
+                                A.env_nee != 0u, A.bounce + 1u >= P.max_bounces, throughput, rng, nd, pdf,
+                                sh_light, sh_env);
       bool done = !cont || A.bounce + 1u >= P.max_bounces;
       if (A.record && !done) {
         uint32_t h = th::pcg_hash((k * 0x9E3779B9u) ^ (A.bounce * 0x85EBCA6Bu) ^
@@ -643,6 +652,8 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if (rc) return rc;
   if (!(opt->alpha >= 0.0f && opt->alpha <= 0.95f))
     return th_set_error(TH_E_INVALID, "alpha must lie in [0, 0.95]");
+  if (opt->flags & ~(TH_GUIDED_NO_ENV_NEE | TH_GUIDED_ALPHA_FROM_START))
+    return th_set_error(TH_E_INVALID, "unknown guided flag");
   const uint32_t batch = opt->batch ? opt->batch : (1u << 18);
   const uint32_t steps = opt->steps_per_pass ? opt->steps_per_pass : 1u;
   const uint32_t wrate = opt->warmup_steps ? std::max<uint32_t>(opt->warmup_rate, 1u) : steps;
@@ -696,7 +707,8 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     A.smp = smp;
     A.nchunks = pass_steps;
     A.chunk_cap = chunk_cap;
-    A.alpha = taken == 0u ? 0.0f
+    const bool untrained = taken == 0u && !(opt->flags & TH_GUIDED_ALPHA_FROM_START);
+    A.alpha = untrained ? 0.0f
               : (opt->alpha_ramp && taken < opt->alpha_ramp ? opt->alpha * (float)taken / (float)opt->alpha_ramp
                                                              : opt->alpha);
     A.record = train_pass ? 1u : 0u;

@@ -157,6 +157,10 @@ struct Integrator {
   }
   __device__ float env_probability(uint32_t index) {  // wgsl:308-314
     float prev = 0.0f;
+    if (S.env_cdf != nullptr) {  // the same values from the compact copy
+      if (index > 0u) prev = S.env_cdf[index - 1u];
+      return th::fmax_(0.0f, S.env_cdf[index] - prev);
+    }
     if (index > 0u) prev = S.env[index - 1u].w;
     return th::fmax_(0.0f, S.env[index].w - prev);
   }
@@ -184,15 +188,20 @@ struct Integrator {
     if (count == 0u || S.env[count - 1u].w <= 0.0f) return out;
     if (COUNT) lc.env_samples++;
     float sel = th::fmin_(rnd(s), 0.99999994f);
-    uint32_t lo = 0, hi = count;
-    while (lo < hi) {
-      uint32_t mid = lo + (hi - lo) / 2u;
-      if (S.env[mid].w <= sel)
-        lo = mid + 1u;
-      else
-        hi = mid;
+    uint32_t index;
+    if (S.env_cdf != nullptr && S.env_rows_w == W) {  // row, then texel within the row
+      index = th::env_search_rows(S.env_cdf, S.env_rows, W, H, sel);
+    } else {
+      uint32_t lo = 0, hi = count;
+      while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2u;
+        if (S.env[mid].w <= sel)
+          lo = mid + 1u;
+        else
+          hi = mid;
+      }
+      index = min(lo, count - 1u);
     }
-    uint32_t index = min(lo, count - 1u);
     uint32_t x = index % W, y = index / W;
     float jx = rnd(s);
     float jy = rnd(s);

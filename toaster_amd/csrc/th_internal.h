@@ -51,6 +51,9 @@ struct th_scene {
   TH_GpuMaterial* d_mats = nullptr;
   TH_GpuLight* d_lights = nullptr;
   float4* d_env = nullptr;
+  float* d_env_cdf = nullptr;   // compact CDF (null: not non-decreasing)
+  float* d_env_rows = nullptr;  // CDF at the end of every row
+  uint32_t env_rows_w = 0;
   uint32_t* d_tex = nullptr;
   CNode* d_cnodes = nullptr;
   Child4* d_nodes4 = nullptr;

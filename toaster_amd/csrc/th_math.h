@@ -195,4 +195,33 @@ TH_HD float fatan2(float y, float x) {
 // acos(x) = atan2(sqrt((1-x)(1+x)), x) for x in [-1, 1].
 TH_HD float facos(float x) { return fatan2(fsqrt((1.0f - x) * (1.0f + x)), x); }
 
+// ------------------------------------------------------- environment CDF search
+// Index of the first texel whose CDF value exceeds `sel` in a row-major CDF of H rows of
+// W texels (the partition point of <= that wgsl:352-362 finds with one binary search
+// over all W*H entries), or W*H - 1 when none does; found row first.  rows[j] is the
+// CDF value of the last texel of row j.  With a non-decreasing CDF the first row whose
+// last value exceeds sel holds the first texel that does (every texel of the earlier
+// rows is <= their last value <= sel), and that row's last texel bounds the search.
+TH_HD uint32_t env_search_rows(const float* cdf, const float* rows, uint32_t W, uint32_t H, float sel) {
+  uint32_t lo = 0, hi = H;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2u;
+    if (rows[mid] <= sel)
+      lo = mid + 1u;
+    else
+      hi = mid;
+  }
+  if (lo >= H) return W * H - 1u;
+  const float* row = cdf + (size_t)lo * W;
+  uint32_t a = 0, b = W - 1u;  // row[W - 1] = rows[lo] > sel
+  while (a < b) {
+    const uint32_t mid = a + (b - a) / 2u;
+    if (row[mid] <= sel)
+      a = mid + 1u;
+    else
+      b = mid;
+  }
+  return lo * W + a;
+}
+
 }  // namespace th

@@ -719,7 +719,7 @@ void free_scene_buffers(th_scene* s) {
   void* ptrs[] = {s->d_nodes, s->d_prims, s->d_spheres, s->d_tris, s->d_attrs, s->d_mats,
                   s->d_lights, s->d_env, s->d_tex, s->d_cnodes, s->d_slots, s->d_accum,
                   s->d_nodes4, s->d_refit_order, s->d_wide_src, s->d_accum_save, s->d_area,
-                  s->d_rgba8};
+                  s->d_rgba8, s->d_env_cdf, s->d_env_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -886,6 +886,30 @@ void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out
   *root4 = 0u;
 }
 
+// Compact copies of the environment CDF for the row-then-texel search: the CDF (.w of
+// every texel) and its value at the end of every row.  Built only for a non-decreasing
+// CDF of the map's own width x height (what the packer writes); any other map keeps the
+// flat search over the texels, which gives the same index for a non-decreasing CDF.
+int32_t prepare_env(th_scene* s, const th_scene_desc* d) {
+  const uint32_t W = d->params.environment_width, H = d->params.environment_height;
+  const uint32_t n = d->environment_pixel_count;
+  if (n == 0 || W == 0 || H == 0 || (uint64_t)W * H != n) return TH_OK;
+  std::vector<float> cdf(n), rows(H);
+  float prev = -3.4028235e38f;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float c = d->environment_pixels[4 * (size_t)i + 3];
+    if (!(c >= prev)) return TH_OK;  // decreasing or NaN: flat search
+    cdf[i] = prev = c;
+  }
+  for (uint32_t j = 0; j < H; ++j) rows[j] = cdf[(size_t)(j + 1) * W - 1];
+  int32_t rc;
+  if ((rc = dalloc(&s->d_env_cdf, n)) || (rc = dalloc(&s->d_env_rows, H))) return rc;
+  TH_CHECK(hipMemcpy(s->d_env_cdf, cdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  TH_CHECK(hipMemcpy(s->d_env_rows, rows.data(), H * sizeof(float), hipMemcpyHostToDevice));
+  s->env_rows_w = W;
+  return TH_OK;
+}
+
 int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
   th_context* ctx = s->ctx;
   s->node_count = d->bvh_node_count;
@@ -1021,6 +1045,9 @@ DevScene dev_scene(const th_scene* s) {
   S.nodes4 = (const float4*)s->d_nodes4;
   S.root4 = s->root4;
   S.node4_bytes = (uint32_t)((size_t)s->node4_count * 4u * sizeof(Child4));
+  S.env_cdf = s->d_env_cdf;
+  S.env_rows = s->d_env_rows;
+  S.env_rows_w = s->env_rows_w;
   for (int k = 0; k < 3; ++k) {
     S.root_min[k] = s->root_min[k];
     S.root_max[k] = s->root_max[k];
@@ -1210,7 +1237,7 @@ int32_t th_scene_create(th_context* ctx, const th_scene_desc* d, uint32_t node_c
       (rc = upload(ctx, s->d_lights, d->lights, d->light_count)) ||
       (rc = upload(ctx, (float*)s->d_env, d->environment_pixels, 4 * (size_t)d->environment_pixel_count)) ||
       (rc = upload(ctx, s->d_tex, d->texture_pixels, d->texture_pixel_count)) ||
-      (rc = prepare_bvh(s, d))) {
+      (rc = prepare_env(s, d)) || (rc = prepare_bvh(s, d))) {
     (void)hipStreamSynchronize(ctx->stream);
     free_scene_buffers(s);
     delete s;
