@@ -362,6 +362,19 @@ void th_scene_destroy(th_scene* scene);
  * average with params.accumulated_samples (wgsl:538-547).  Asynchronous. */
 int32_t th_render(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
                   const TH_GpuCamera* camera, const th_render_options* options);
+/* th_render with options->kernel = TH_KERNEL_WAVEFRONT, allocating nothing: no
+ * counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * wavefront's queues, counters and per-path state live in one static device array of
+ * 13.0 MiB (65,536 path slots) that every GPU the library runs on holds until the
+ * process ends; renders needing more slots (width * height, times the sample groups
+ * with per-sample streams: min(samples, 16) rounded down to a power of two) return
+ * TH_E_CAPACITY before any device work.  Same image as th_render's wavefront for the
+ * same options, folded into the scene's accumulation buffer; max_bounces must be
+ * nonzero; no timing or counters.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (several contexts or threads) race. */
+int32_t th_render_wavefront_static(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
+                                   const TH_GpuCamera* camera, const th_render_options* options);
 /* Blocks, then copies width*height float4 linear-HDR pixels (readback.rs:8-25). */
 int32_t th_readback(th_context* ctx, th_scene* scene, float* rgba, size_t bytes);
 /* Overwrites the accumulation buffer (progressive resume from a checkpoint). */

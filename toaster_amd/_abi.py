@@ -180,6 +180,8 @@ SIGNATURES = {
     "th_scene_destroy": (None, [P]),
     "th_render": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
                               C.POINTER(RenderOptions)]),
+    "th_render_wavefront_static": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
+                                               C.POINTER(RenderOptions)]),
     "th_readback": (C.c_int32, [P, P, P, C.c_size_t]),
     "th_upload_accumulation": (C.c_int32, [P, P, P, C.c_size_t]),
     "th_synchronize": (C.c_int32, [P]),

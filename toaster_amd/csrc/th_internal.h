@@ -36,6 +36,7 @@ struct th_context {
   uint32_t* d_work = nullptr;  // work counters of persistent launches
   int cu_count = 0;            // compute units of the device
   int dyn_blocks_per_cu = 0;   // resident blocks of k_render_quad_dyn (0 = not yet queried)
+  void* wf_scratch = nullptr;  // device address of the static wavefront workspace on `device`
 };
 
 struct th_scene {
@@ -93,6 +94,7 @@ int32_t fold_timing(th_context* ctx);
 int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
                             const TH_GpuCamera* cam, const th_render_options* o);
 void th_wavefront_free(th_wavefront* wf);
+int32_t th_wavefront_scratch_init(th_context* ctx);
 void th_batch_free(th_batch_state* b);
 int32_t th_check_render_params(const th_scene* s, const TH_GpuRenderParams* p);
 

@@ -1173,6 +1173,11 @@ int32_t th_context_create(int32_t device, th_context** out) {
     th_context_destroy(c);
     return th_set_error(TH_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
   }
+  int32_t rc = th_wavefront_scratch_init(c);  // with `device` current
+  if (rc) {
+    th_context_destroy(c);
+    return rc;
+  }
   *out = c;
   return TH_OK;
 }

@@ -394,6 +394,57 @@ struct th_wavefront {
   std::vector<void*> allocs;
 };
 
+// Workspace of th_render_wavefront_static: the same structure-of-arrays buffers as a
+// scene's wavefront workspace, for at most WF_STATIC_SLOTS path slots, in one
+// file-scope device array that every GPU holds from module load to process end.  Its
+// address on the current device comes from hipGetSymbolAddress (th_context_create);
+// host code never uses the array's name as a device address.
+#define WF_STATIC_SLOTS 65536u
+#define WF_SLOT_BYTES (10u * 16u + 16u + 8u + 8u + 4u + 4u + 8u)  // o..c1, st, hit, vis, q0, q1, qs
+#define WF_STATIC_BYTES ((size_t)WF_STATIC_SLOTS * WF_SLOT_BYTES + WF_C(16) * sizeof(uint32_t))
+static __device__ __align__(256) unsigned char scratch[WF_STATIC_BYTES];
+
+int32_t th_wavefront_scratch_init(th_context* ctx) {
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    return th_set_error(TH_E_DEVICE, std::string("wavefront scratch address: ") + hipGetErrorString(e));
+  ctx->wf_scratch = ptr;
+  return TH_OK;
+}
+
+// Carves the WfBufs arrays for WF_STATIC_SLOTS slots out of the scratch array at `base`
+// (the device address of `scratch`), each array 256-byte aligned.
+static WfBufs wf_static_bufs(void* base) {
+  const size_t n = WF_STATIC_SLOTS;
+  unsigned char* p = (unsigned char*)base;
+  auto take = [&](size_t bytes) {
+    unsigned char* q = p;
+    p += (bytes + 255u) & ~(size_t)255u;
+    return (void*)q;
+  };
+  WfBufs b;
+  b.o = (float4*)take(n * 16);
+  b.d = (float4*)take(n * 16);
+  b.thr = (float4*)take(n * 16);
+  b.rad = (float4*)take(n * 16);
+  b.acc = (float4*)take(n * 16);
+  b.st = (uint4*)take(n * 16);
+  b.hit = (uint2*)take(n * 8);
+  b.sho = (float4*)take(n * 16);
+  b.sh0 = (float4*)take(n * 16);
+  b.c0 = (float4*)take(n * 16);
+  b.sh1 = (float4*)take(n * 16);
+  b.c1 = (float4*)take(n * 16);
+  b.vis = (uint32_t*)take(n * 8);
+  b.q0 = (uint32_t*)take(n * 4);
+  b.q1 = (uint32_t*)take(n * 4);
+  b.qs = (uint32_t*)take(n * 8);
+  b.cnt = (uint32_t*)take(WF_C(16) * sizeof(uint32_t));
+  return b;
+}
+static_assert(WF_STATIC_BYTES % 256u == 0u, "scratch layout assumes 256-byte multiples");
+
 void th_wavefront_free(th_wavefront* wf) {
   if (!wf) return;
   for (void* p : wf->allocs) (void)hipFree(p);
@@ -434,8 +485,10 @@ static int32_t wf_reserve(th_scene* s, uint32_t n) {
   return TH_OK;
 }
 
-int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
-                            const TH_GpuCamera* cam, const th_render_options* o) {
+// Sample groups per pixel of a wavefront render (per-sample streams: up to 16, fewer
+// when the image would need more than 2^24 slots) and the slot count.
+static int32_t wf_shape(const TH_GpuRenderParams* p, const th_render_options* o, uint32_t* G_out,
+                        uint32_t* n_out) {
   uint32_t npix = p->width * p->height;
   uint32_t G = 1;
   if (o->rng_mode == TH_RNG_PER_SAMPLE) {
@@ -444,10 +497,48 @@ int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderPara
   }
   uint64_t n64 = (uint64_t)npix * G;
   if (n64 > 0x7fffffffull) return th_set_error(TH_E_UNSUPPORTED, "image too large for the wavefront");
-  uint32_t n = (uint32_t)n64;
-  int32_t rc = wf_reserve(s, n);
+  *G_out = G;
+  *n_out = (uint32_t)n64;
+  return TH_OK;
+}
+
+static int32_t wf_launch(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                         const th_render_options* o, const WfBufs& B, uint32_t n, uint32_t G);
+
+int32_t th_wavefront_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
+                            const TH_GpuCamera* cam, const th_render_options* o) {
+  uint32_t G, n;
+  int32_t rc = wf_shape(p, o, &G, &n);
   if (rc) return rc;
-  WfBufs B = s->wf->b;
+  if ((rc = wf_reserve(s, n))) return rc;
+  return wf_launch(ctx, s, p, cam, o, s->wf->b, n, G);
+}
+
+extern "C" int32_t th_render_wavefront_static(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
+                                              const TH_GpuCamera* cam, const th_render_options* opt) {
+  if (!ctx || !s || !cam) return th_set_error(TH_E_INVALID, "null argument");
+  if (s->ctx != ctx) return th_set_error(TH_E_INVALID, "scene belongs to another context");
+  int32_t rc = th_check_render_params(s, p);
+  if (rc) return rc;
+  th_render_options o;
+  memset(&o, 0, sizeof o);
+  if (opt) o = *opt;
+  if (o.rng_mode > 1u || o.traversal > 2u) return th_set_error(TH_E_INVALID, "unknown render option");
+  if (o.collect_stats) return th_set_error(TH_E_UNSUPPORTED, "statistics are counted by the megakernel");
+  if (ctx->wf_scratch == nullptr) return th_set_error(TH_E_DEVICE, "no wavefront scratch on this context");
+  if (p->max_bounces == 0u) return th_set_error(TH_E_UNSUPPORTED, "zero-bounce renders use th_render");
+  uint32_t G, n;
+  if ((rc = wf_shape(p, &o, &G, &n))) return rc;
+  if (n > WF_STATIC_SLOTS)
+    return th_set_error(TH_E_CAPACITY, "the static wavefront workspace holds 65536 path slots "
+                                       "(width * height, times the sample groups with per-sample streams)");
+  TH_CHECK(hipSetDevice(ctx->device));
+  return wf_launch(ctx, s, p, cam, &o, wf_static_bufs(ctx->wf_scratch), n, G);
+}
+
+static int32_t wf_launch(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                         const th_render_options* o, const WfBufs& B, uint32_t n, uint32_t G) {
+  uint32_t npix = p->width * p->height;
   WfArgs A{n, G, o->rng_mode == TH_RNG_PER_SAMPLE ? 1u : 0u};
   DevScene S = dev_scene(s);
   const bool quad = o->traversal == TH_TRAVERSE_WIDE && s->node_count > 0;
