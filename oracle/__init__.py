@@ -59,6 +59,8 @@ _SIGS = {
                               C.c_int32, P, C.POINTER(Counters)]),
     "or_render_ex": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), U, U,
                                  C.c_int32, U, U, P, C.POINTER(Counters)]),
+    "or_render_cpu": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), U, U, C.c_int32,
+                                  P]),
     "or_primary_hits": (C.c_int32, [P, C.POINTER(A.GpuRenderParams), C.POINTER(A.GpuCamera), P, P]),
     "or_closest": (C.c_int32, [P, F3, F3, C.c_int32, C.POINTER(Hit)]),
     "or_intersect_sphere": (C.c_int32, [F3, F3, F3, F, F, F, C.POINTER(Hit)]),
@@ -143,6 +145,17 @@ class OracleScene:
                                   int(rng_mode), int(group), out.ctypes.data,
                                   C.byref(cnt) if counters else None))
         return out, (cnt.as_dict() if counters else None)
+
+    def render_cpu(self, params=None, rows=None, threads=1, accum=None):
+        """toaster-cpu mode (the CPU renderer's estimator on its own generator, for
+        converged comparison only) of rows [r0, r1); returns rgba (H, W, 4)."""
+        p = params if params is not None else self.params
+        h, w = p.height, p.width
+        out = np.zeros((h, w, 4), np.float32) if accum is None else np.ascontiguousarray(accum, np.float32).copy()
+        r0, r1 = (0, h) if rows is None else rows
+        _check(lib().or_render_cpu(self._h, C.byref(p), C.byref(self.camera), r0, r1, int(threads),
+                                   out.ctypes.data))
+        return out
 
     def primary_hits(self, params=None):
         p = params if params is not None else self.params

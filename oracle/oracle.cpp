@@ -761,6 +761,453 @@ uint8_t to_u8(float c) {  // (c.clamp(0,1)*255).round() as u8
   return (uint8_t)roundf(c * 255.0f);
 }
 
+// ============================================================ toaster-cpu mode
+// The CPU renderer's estimator, for converged-image comparison with the GPU kernels:
+// crates/toaster-cpu/src/integrator.rs:47-63 (pixel loop), 99-165 (recursive
+// ray_color_inner: scatter, then the subpath, then the light and environment samples),
+// 168-244 (direct light and environment), 270-389 (scatter, reflect/refract/Schlick,
+// rejection-sampled unit vectors: unit-ball metal fuzz); intersect.rs:28-148 (sphere,
+// triangle), 172-263 (front-to-back traversal, infinite far bound), 266-289 (linear),
+// 322-338 (entry distance); toaster-bvh/src/aabb.rs:74-111 (slab test with infinite
+// inverse components); light.rs:126-171 (strict `t < cumulative` pick, sphere points
+// from z and azimuth ranges); toaster-scene/src/environment.rs:132-242 (sample,
+// importance sample with the 1 - f32::EPSILON clamp, solid-angle pdf);
+// toaster-core/src/camera.rs:70-77.  It shares no sampling routine, traversal,
+// transcendental or generator with the "pcg" mode above or with the kernels: libm
+// sin/cos/atan2/acos, and a SplitMix64 stream per pixel (Steele, Lea & Flood 2014) in
+// place of rand's StdRng, whose stream is not restated -- images agree with toaster-cpu
+// in distribution, not per pixel.
+namespace cpu {
+
+const float EPS = 0.001f;  // integrator.rs:11 RAY_EPSILON
+const float PI_F = 3.14159265358979323846f;
+const float TAU_F = 6.28318530717958647692f;
+
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct Rng {
+  uint64_t s;
+  uint64_t next64() { return mix64(s += 0x9E3779B97F4A7C15ull); }
+  float f() { return (float)(next64() >> 40) * (1.0f / 16777216.0f); }  // [0, 1)
+  float range(float lo, float hi) {
+    float v = lo + (hi - lo) * f();
+    return v < hi ? v : lo;  // half-open as rand's random_range
+  }
+};
+
+v3 lerp(v3 a, v3 b, float t) { return th::add(a, th::muls(th::sub(b, a), t)); }  // glam Vec3::lerp
+float rem1(float x) {  // f32::rem_euclid(1.0)
+  float r = fmodf(x, 1.0f);
+  return r < 0.0f ? r + 1.0f : r;
+}
+
+struct CHit {
+  float t = 0.0f;
+  v3 p{0, 0, 0}, n{0, 0, 0};
+  bool front = false;
+  uint32_t mat = 0;
+  float tu = 0.0f, tv = 0.0f;
+};
+
+bool sphere(const Ray& r, const TH_GpuSphere& s, float tmin, float tmax, CHit* h) {  // intersect.rs:28-68
+  v3 c = v3_of(s.center_radius);
+  float rad = s.center_radius[3];
+  v3 oc = th::sub(r.o, c);
+  float a = th::dot(r.d, r.d);
+  float hb = th::dot(oc, r.d);
+  float cc = th::dot(oc, oc) - rad * rad;
+  float disc = hb * hb - a * cc;
+  if (disc < 0.0f) return false;
+  float sq = sqrtf(disc);
+  float t = (-hb - sq) / a;
+  if (t < tmin || t > tmax) {
+    t = (-hb + sq) / a;
+    if (t < tmin || t > tmax) return false;
+  }
+  v3 p = th::add(r.o, th::smul(t, r.d));
+  v3 out = th::divs(th::sub(p, c), rad);
+  h->front = th::dot(r.d, out) < 0.0f;
+  h->t = t;
+  h->p = p;
+  h->n = h->front ? out : th::neg(out);
+  h->mat = s.material_index;
+  h->tu = h->tv = 0.0f;
+  return true;
+}
+
+bool triangle(const Ray& r, const TH_GpuTriangle& tr, const TH_GpuTriangleAttributes& at, float tmin, float tmax,
+              CHit* h) {  // intersect.rs:87-148
+  v3 v0 = v3_of(tr.v0), v1 = v3_of(tr.v1), v2 = v3_of(tr.v2);
+  v3 e1 = th::sub(v1, v0), e2 = th::sub(v2, v0);
+  v3 pv = th::cross(r.d, e2);
+  float det = th::dot(e1, pv);
+  if (fabsf(det) < 1e-8f) return false;
+  float inv = 1.0f / det;
+  v3 ov = th::sub(r.o, v0);
+  float u = th::dot(ov, pv) * inv;
+  if (!(u >= 0.0f && u <= 1.0f)) return false;
+  v3 qv = th::cross(ov, e1);
+  float v = th::dot(r.d, qv) * inv;
+  if (v < 0.0f || u + v > 1.0f) return false;
+  float t = th::dot(e2, qv) * inv;
+  if (t < tmin || t > tmax) return false;
+  v3 gn = glam_normalize(th::cross(e1, e2));
+  bool front = th::dot(r.d, gn) < 0.0f;
+  float w0 = 1.0f - u - v;
+  v3 out = gn;
+  if (at.flags & 1u)
+    out = glam_normalize(th::add(th::add(th::smul(w0, v3_of(at.n0)), th::smul(u, v3_of(at.n1))),
+                                 th::smul(v, v3_of(at.n2))));
+  if (th::dot(out, gn) < 0.0f) out = th::neg(out);
+  h->tu = h->tv = 0.0f;
+  if (at.flags & 2u) {
+    h->tu = w0 * at.uv0[0] + u * at.uv1[0] + v * at.uv2[0];
+    h->tv = w0 * at.uv0[1] + u * at.uv1[1] + v * at.uv2[1];
+  }
+  h->t = t;
+  h->p = th::add(r.o, th::smul(t, r.d));
+  h->n = front ? out : th::neg(out);
+  h->front = front;
+  h->mat = tr.material_index;
+  return true;
+}
+
+bool aabb_hit(const TH_GpuBvhNode& b, v3 o, v3 inv, float tmin, float tmax) {  // aabb.rs:74-111
+  const float oa[3] = {o.x, o.y, o.z}, ia[3] = {inv.x, inv.y, inv.z};
+  for (int k = 0; k < 3; ++k) {
+    if (isinf(ia[k])) {
+      if (oa[k] < b.min[k] || oa[k] > b.max[k]) return false;
+      continue;
+    }
+    float n = (b.min[k] - oa[k]) * ia[k], f = (b.max[k] - oa[k]) * ia[k];
+    if (n > f) std::swap(n, f);
+    tmin = std::fmax(tmin, n);  // Rust f32::max / min ignore a NaN operand
+    tmax = std::fmin(tmax, f);
+    if (tmax < tmin) return false;
+  }
+  return true;
+}
+
+float entry_distance(const TH_GpuBvhNode& b, v3 o, v3 inv) {  // intersect.rs:322-338
+  const float oa[3] = {o.x, o.y, o.z}, ia[3] = {inv.x, inv.y, inv.z};
+  float e = -INFINITY;
+  for (int k = 0; k < 3; ++k) {
+    if (isinf(ia[k])) continue;
+    float n = (b.min[k] - oa[k]) * ia[k], f = (b.max[k] - oa[k]) * ia[k];
+    if (n > f) std::swap(n, f);
+    e = std::fmax(e, n);
+  }
+  return e;
+}
+
+bool prim_hit(const or_scene& s, uint32_t kind, uint32_t index, const Ray& r, float tmin, float tmax, CHit* h) {
+  if (kind == 0u) return sphere(r, s.spheres[index], tmin, tmax, h);
+  return triangle(r, s.triangles[index], s.attributes[index], tmin, tmax, h);
+}
+
+// intersect_scene (intersect.rs:172-263); a scene without a tree takes the linear scan
+// (intersect_scene_linear, 266-289).
+bool scene_hit(const or_scene& s, const Ray& r, float tmin, CHit* out) {
+  float closest = INFINITY;
+  bool hit = false;
+  CHit c;
+  if (s.nodes.empty()) {
+    for (uint32_t i = 0; i < (uint32_t)s.spheres.size(); ++i)
+      if (sphere(r, s.spheres[i], tmin, closest, &c)) {
+        closest = c.t;
+        *out = c;
+        hit = true;
+      }
+    for (uint32_t i = 0; i < (uint32_t)s.triangles.size(); ++i)
+      if (triangle(r, s.triangles[i], s.attributes[i], tmin, closest, &c)) {
+        closest = c.t;
+        *out = c;
+        hit = true;
+      }
+    return hit;
+  }
+  v3 inv = th::mk(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+  uint32_t stack[512];
+  int sp = 0;
+  stack[sp++] = 0u;
+  while (sp > 0) {
+    uint32_t ni = stack[--sp];
+    const TH_GpuBvhNode& node = s.nodes[ni];
+    if (!aabb_hit(node, r.o, inv, tmin, closest)) continue;
+    if (node.primitive_count > 0u) {
+      for (uint32_t k = node.first_or_right; k < node.first_or_right + node.primitive_count; ++k)
+        if (prim_hit(s, s.prims[k].kind, s.prims[k].index, r, tmin, closest, &c)) {
+          closest = c.t;
+          *out = c;
+          hit = true;
+        }
+    } else {
+      uint32_t l = ni + 1u, rr = node.first_or_right;
+      bool lh = aabb_hit(s.nodes[l], r.o, inv, tmin, closest);
+      bool rh = aabb_hit(s.nodes[rr], r.o, inv, tmin, closest);
+      if (sp + 2 > 512) abort();  // a median-split tree is far shallower
+      if (lh && rh) {
+        if (entry_distance(s.nodes[l], r.o, inv) < entry_distance(s.nodes[rr], r.o, inv)) {
+          stack[sp++] = rr;
+          stack[sp++] = l;
+        } else {
+          stack[sp++] = l;
+          stack[sp++] = rr;
+        }
+      } else if (lh) {
+        stack[sp++] = l;
+      } else if (rh) {
+        stack[sp++] = rr;
+      }
+    }
+  }
+  return hit;
+}
+
+v3 random_in_unit_sphere(Rng& g) {  // integrator.rs:377-389
+  for (;;) {
+    v3 v = th::mk(g.range(-1.0f, 1.0f), g.range(-1.0f, 1.0f), g.range(-1.0f, 1.0f));
+    float l2 = th::dot(v, v);
+    if (l2 >= 1e-8f && l2 <= 1.0f) return v;
+  }
+}
+v3 random_unit_vector(Rng& g) { return glam_normalize(random_in_unit_sphere(g)); }  // 372-374
+
+v3 reflect(v3 d, v3 n) { return th::sub(d, th::muls(n, 2.0f * th::dot(d, n))); }  // 349-351
+v3 refract(v3 d, v3 n, float ratio) {                                             // 354-359
+  float ct = std::fmin(th::dot(th::neg(d), n), 1.0f);
+  v3 perp = th::smul(ratio, th::add(d, th::smul(ct, n)));
+  v3 par = th::smul(-sqrtf(fabsf(1.0f - th::dot(perp, perp))), n);
+  return th::add(perp, par);
+}
+float schlick(float c, float ratio) {  // 362-365
+  float r0 = (1.0f - ratio) / (1.0f + ratio);
+  r0 = r0 * r0;
+  return r0 + (1.0f - r0) * powf(1.0f - c, 5.0f);
+}
+float power_heuristic(float a, float b) {  // 246-256
+  float a2 = a * a, b2 = b * b, den = a2 + b2;
+  return den > 0.0f ? a2 / den : 0.0f;
+}
+
+struct Estimator {
+  const or_scene& s;
+  const TH_GpuRenderParams& P;
+
+  // ---- environment.rs:132-242
+  v3 env_texel(int x, int y) const {
+    int W = (int)P.environment_width, H = (int)P.environment_height;
+    int xx = ((x % W) + W) % W, yy = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+    const float* p = &s.env[4 * ((size_t)yy * W + xx)];
+    return th::mk(p[0], p[1], p[2]);
+  }
+  bool dir_uv(v3 d, float* u, float* v, float* sin_t) const {
+    float l = glam_length(d);
+    if (!(l > 0.0f) || isinf(l)) return false;  // normalize_or_zero
+    v3 n = glam_normalize(d);
+    *u = rem1(atan2f(n.z, n.x) / (2.0f * PI_F) + 0.5f + P.environment_rotation_degrees / 360.0f);
+    float th = acosf(std::fmin(std::fmax(n.y, -1.0f), 1.0f));
+    *v = th / PI_F;
+    *sin_t = sinf(th);
+    return true;
+  }
+  v3 env_sample(v3 d) const {
+    float u, v, st;
+    if (!dir_uv(d, &u, &v, &st)) return th::mk(0, 0, 0);
+    float x = u * (float)P.environment_width - 0.5f, y = v * (float)P.environment_height - 0.5f;
+    int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    float ax = x - floorf(x), ay = y - floorf(y);
+    v3 top = lerp(env_texel(x0, y0), env_texel(x0 + 1, y0), ax);
+    v3 bottom = lerp(env_texel(x0, y0 + 1), env_texel(x0 + 1, y0 + 1), ax);
+    return th::muls(lerp(top, bottom, ay), P.environment_intensity);
+  }
+  float cdf(size_t i) const { return s.env[4 * i + 3]; }
+  float texel_pdf(size_t index, float sin_t) const {
+    float prev = index > 0 ? cdf(index - 1) : 0.0f;
+    float p = std::fmax(cdf(index) - prev, 0.0f);
+    if (p <= 0.0f || sin_t <= 0.0f) return 0.0f;
+    return p * (float)P.environment_width * (float)P.environment_height / (2.0f * PI_F * PI_F * sin_t);
+  }
+  float env_pdf(v3 d) const {
+    float u, v, st;
+    if (!dir_uv(d, &u, &v, &st)) return 0.0f;
+    size_t W = P.environment_width, H = P.environment_height;
+    size_t x = (size_t)floorf(u * (float)W) % W;
+    size_t y = std::min((size_t)floorf(v * (float)H), H - 1);
+    return texel_pdf(y * W + x, st);
+  }
+  static float unit_interval(float x) {
+    return isfinite(x) ? std::fmin(std::fmax(x, 0.0f), 1.0f - 1.1920929e-7f) : 0.0f;
+  }
+  bool env_importance(float sel, float ju, float jv, v3* dir, v3* rad, float* pdf) const {
+    size_t n = (size_t)P.environment_width * P.environment_height;
+    if (n == 0 || cdf(n - 1) <= 0.0f) return false;
+    float x_sel = unit_interval(sel);
+    size_t lo = 0, hi = n;  // partition_point(c <= x_sel)
+    while (lo < hi) {
+      size_t mid = lo + (hi - lo) / 2;
+      if (cdf(mid) <= x_sel)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    size_t index = std::min(lo, n - 1);
+    size_t W = P.environment_width;
+    float u = ((float)(index % W) + unit_interval(ju)) / (float)W;
+    float v = ((float)(index / W) + unit_interval(jv)) / (float)P.environment_height;
+    float theta = PI_F * v;
+    float phi = 2.0f * PI_F * (u - 0.5f - P.environment_rotation_degrees / 360.0f);
+    float st = sinf(theta);
+    *dir = th::mk(st * cosf(phi), cosf(theta), st * sinf(phi));
+    *pdf = texel_pdf(index, st);
+    *rad = env_sample(*dir);
+    return true;
+  }
+
+  // ---- materials (toaster-scene Material via the packed records; textured diffuse
+  // resolved to its hit albedo as material_at_hit, integrator.rs:340-346)
+  v3 albedo_at(const TH_GpuMaterial& m, const CHit& h) const {
+    if (m.texture_width != 0u && m.texture_height != 0u) {
+      Tracer tr{s, P, nullptr};
+      Hit wh;
+      wh.tu = h.tu;
+      wh.tv = h.tv;
+      return tr.texture_albedo(m, wh);
+    }
+    return v3_of(m.albedo);
+  }
+
+  // ---- light.rs:126-171 + integrator.rs:168-205
+  v3 direct_area_light(const CHit& h, v3 albedo, Rng& g) const {
+    if (P.light_count == 0u || P.total_light_weight <= 0.0f) return th::mk(0, 0, 0);
+    float target = g.f() * P.total_light_weight;
+    const TH_GpuLight* L = &s.lights.back();
+    for (const TH_GpuLight& c : s.lights)
+      if (target < c.area_weight_cumulative[2]) {
+        L = &c;
+        break;
+      }
+    v3 pos, ln;
+    if (L->kind == 0u) {
+      float su = sqrtf(g.f());
+      float v = g.f();
+      v3 a = v3_of(L->v0), b = v3_of(L->v1), c = v3_of(L->v2);
+      pos = th::add(th::add(th::smul(1.0f - su, a), th::smul(su * (1.0f - v), b)), th::smul(su * v, c));
+      ln = glam_normalize(th::cross(th::sub(b, a), th::sub(c, a)));
+    } else {
+      float z = g.range(-1.0f, 1.0f);
+      float az = g.range(0.0f, TAU_F);
+      float radial = sqrtf(std::fmax(1.0f - z * z, 0.0f));
+      ln = th::mk(radial * cosf(az), radial * sinf(az), z);
+      pos = th::add(v3_of(L->center_radius), th::smul(L->center_radius[3], ln));
+    }
+    const TH_GpuMaterial& em = s.materials[L->material_index];
+    v3 emission = th::muls(v3_of(em.albedo), em.params[2]);
+    float sel_pdf = L->area_weight_cumulative[1] / P.total_light_weight;
+    float pdf_area = sel_pdf / L->area_weight_cumulative[0];
+    v3 so = th::add(h.p, th::muls(h.n, EPS));
+    v3 tl = th::sub(pos, so);
+    float d2 = th::dot(tl, tl);
+    if (d2 <= EPS * EPS) return th::mk(0, 0, 0);
+    float dist = sqrtf(d2);
+    v3 dir = th::divs(tl, dist);
+    float sc = std::fmax(th::dot(h.n, dir), 0.0f);
+    float lc = std::fmax(th::dot(ln, th::neg(dir)), 0.0f);
+    if (sc <= 0.0f || lc <= 0.0f) return th::mk(0, 0, 0);
+    CHit b;
+    if (scene_hit(s, Ray{so, dir}, EPS, &b) && b.t < dist - 2.0f * EPS) return th::mk(0, 0, 0);
+    v3 r = th::divs(albedo, PI_F);
+    r = th::mul(r, emission);
+    r = th::muls(r, sc);
+    r = th::muls(r, lc);
+    return th::divs(r, d2 * pdf_area);
+  }
+
+  // ---- integrator.rs:208-244
+  v3 direct_environment(const CHit& h, v3 albedo, Rng& g) const {
+    if (P.background_kind != 2u) return th::mk(0, 0, 0);
+    float r1 = g.f(), r2 = g.f(), r3 = g.f();
+    v3 dir, rad;
+    float pdf;
+    if (!env_importance(r1, r2, r3, &dir, &rad, &pdf)) return th::mk(0, 0, 0);
+    float sc = std::fmax(th::dot(h.n, dir), 0.0f);
+    if (sc <= 0.0f || pdf <= 0.0f) return th::mk(0, 0, 0);
+    CHit b;
+    if (scene_hit(s, Ray{th::add(h.p, th::muls(h.n, EPS)), dir}, EPS, &b)) return th::mk(0, 0, 0);
+    float mis = power_heuristic(pdf, sc / PI_F);
+    v3 r = th::divs(albedo, PI_F);
+    r = th::mul(r, rad);
+    r = th::muls(r, sc);
+    r = th::muls(r, mis);
+    return th::divs(r, pdf);
+  }
+
+  // ---- integrator.rs:99-165
+  v3 ray_color(const Ray& ray, uint32_t depth, bool include_emission, bool has_pdf, float bsdf_pdf,
+               Rng& g) const {
+    if (depth == 0u) return th::mk(0, 0, 0);
+    CHit h;
+    if (scene_hit(s, ray, EPS, &h)) {
+      const TH_GpuMaterial& m = s.materials[h.mat];
+      if (m.kind == 3u) return include_emission ? th::muls(v3_of(m.albedo), m.params[2]) : th::mk(0, 0, 0);
+      Ray next;
+      v3 att;
+      bool next_has_pdf = false;
+      float next_pdf = 0.0f;
+      v3 albedo = th::mk(0, 0, 0);
+      if (m.kind == 0u) {  // diffuse, textured or not
+        albedo = albedo_at(m, h);
+        v3 d = th::add(h.n, random_unit_vector(g));
+        if (th::dot(d, d) < 1e-8f) d = h.n;
+        v3 nd = glam_normalize(d);
+        next = Ray{h.p, nd};
+        att = albedo;
+        next_has_pdf = true;
+        next_pdf = std::fmax(th::dot(h.n, nd), 0.0f) / PI_F;
+      } else if (m.kind == 1u) {  // metal: unit-ball fuzz
+        v3 refl = reflect(glam_normalize(ray.d), h.n);
+        float rough = std::fmin(std::fmax(m.params[0], 0.0f), 1.0f);
+        v3 d = th::add(refl, th::smul(rough, random_in_unit_sphere(g)));
+        if (th::dot(d, h.n) <= 0.0f) return th::mk(0, 0, 0);
+        next = Ray{h.p, glam_normalize(d)};
+        att = v3_of(m.albedo);
+      } else if (m.kind == 2u) {  // dielectric
+        float ratio = h.front ? 1.0f / m.params[1] : m.params[1];
+        v3 ud = glam_normalize(ray.d);
+        float ct = std::fmin(th::dot(th::neg(ud), h.n), 1.0f);
+        float st = sqrtf(1.0f - ct * ct);
+        bool cannot = ratio * st > 1.0f;
+        v3 d = (cannot || schlick(ct, ratio) > g.f()) ? reflect(ud, h.n) : refract(ud, h.n, ratio);
+        next = Ray{h.p, glam_normalize(d)};
+        att = th::mk(1, 1, 1);
+      } else {
+        return th::mk(0, 0, 0);
+      }
+      v3 indirect = th::mul(att, ray_color(next, depth - 1u, m.kind != 0u, next_has_pdf, next_pdf, g));
+      v3 direct = th::mk(0, 0, 0);
+      if (m.kind == 0u)
+        direct = th::add(direct_area_light(h, albedo, g), direct_environment(h, albedo, g));
+      return th::add(direct, indirect);
+    }
+    if (P.background_kind == 0u) {
+      v3 d = glam_normalize(ray.d);
+      return lerp(th::mk(1, 1, 1), th::mk(0.35f, 0.65f, 1.0f), 0.5f * (d.y + 1.0f));
+    }
+    if (P.background_kind == 2u) {
+      v3 rad = env_sample(ray.d);
+      float w = has_pdf ? power_heuristic(bsdf_pdf, env_pdf(ray.d)) : 1.0f;
+      return th::muls(rad, w);
+    }
+    return th::mk(0, 0, 0);
+  }
+};
+
+}  // namespace cpu
+
 }  // namespace
 
 // ============================================================ C ABI
@@ -1130,6 +1577,54 @@ int32_t or_render_ex(const or_scene* s, const TH_GpuRenderParams* params, const 
       total.env_samples += c.env_samples;
     }
     *counters = total;
+  }
+  return TH_OK;
+}
+
+// toaster-cpu mode render of rows [row_begin, row_end): per pixel `samples` camera rays
+// (integrator.rs:47-63) on a SplitMix64 stream seeded from the pixel index, the frame
+// index and the CPU renderer's seed 0x00544F4153544552 (integrator.rs:10), averaged, then
+// folded into `rgba` as the progressive contract does (accumulated_samples).
+int32_t or_render_cpu(const or_scene* s, const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
+                      uint32_t row_begin, uint32_t row_end, int32_t threads, float* rgba) {
+  if (!s || !params || !camera || !rgba) return fail("null argument");
+  const TH_GpuRenderParams P = *params;
+  if (P.samples == 0) return fail("samples must be nonzero");
+  if (row_end > P.height) row_end = P.height;
+  const TH_GpuCamera cam = *camera;
+  const int64_t nrows = row_end > row_begin ? (int64_t)(row_end - row_begin) : 0;
+  const int64_t npix = nrows * (int64_t)P.width;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 1 ? threads : 1)
+  for (int64_t k = 0; k < npix; ++k) {
+    cpu::Estimator est{*s, P};
+    uint32_t y = row_begin + (uint32_t)(k / P.width), x = (uint32_t)(k % P.width);
+    uint32_t idx = y * P.width + x;
+    cpu::Rng g{cpu::mix64(0x00544F4153544552ull ^ ((uint64_t)P.frame_index << 32) ^ (uint64_t)idx)};
+    v3 color = th::mk(0, 0, 0);
+    for (uint32_t i = 0; i < P.samples; ++i) {
+      float u = ((float)x + g.f()) / (float)P.width;
+      float v = 1.0f - ((float)y + g.f()) / (float)P.height;
+      v3 d = th::sub(th::add(th::add(v3_of(cam.lower_left_corner), th::smul(u, v3_of(cam.horizontal))),
+                             th::smul(v, v3_of(cam.vertical))),
+                     v3_of(cam.origin));  // camera.rs:70-77
+      Ray r{v3_of(cam.origin), glam_normalize(d)};
+      color = th::add(color, est.ray_color(r, P.max_bounces, true, false, 0.0f, g));
+    }
+    float ns = (float)P.samples;
+    float b0 = color.x / ns, b1 = color.y / ns, b2 = color.z / ns, b3 = 1.0f;
+    float* o = rgba + 4 * (size_t)idx;
+    if (P.accumulated_samples == 0u) {
+      o[0] = b0;
+      o[1] = b1;
+      o[2] = b2;
+      o[3] = b3;
+    } else {
+      float prev = (float)P.accumulated_samples, tot = prev + ns;
+      o[0] = (o[0] * prev + b0 * ns) / tot;
+      o[1] = (o[1] * prev + b1 * ns) / tot;
+      o[2] = (o[2] * prev + b2 * ns) / tot;
+      o[3] = (o[3] * prev + b3 * ns) / tot;
+    }
   }
   return TH_OK;
 }

@@ -11,18 +11,26 @@
  *   environment  crates/toaster-scene/src/environment.rs:245-278 (CDF build)
  *   estimator    shaders/pathtrace_spheres.wgsl:189-937 ("pcg" mode: per-pixel PCG
  *                stream, draw order and formulas of the reference GPU kernel)
+ *   CPU renderer crates/toaster-cpu/src/{integrator.rs:47-389, intersect.rs:28-338,
+ *                light.rs:126-171}, toaster-scene/src/environment.rs:132-242,
+ *                toaster-bvh/src/aabb.rs:74-111 ("toaster_cpu" mode, or_render_cpu: its
+ *                recursion and draw order, sampling routines and traversal, on libm and
+ *                a SplitMix64 stream per pixel; for converged comparison)
  *   display      crates/toaster-core/src/color.rs:23-75, toaster-gpu/src/image_output.rs:59-92
  *
  * Pinning (tests/test_oracle_known_answers.py, tests/test_reference_scenes.py): the
  * reference's own known-answer tests for intersection, AABB, BVH, lights, environment,
  * integrator, camera, packing and display, plus closed forms (BVH == brute force, env pdf
  * integrates to 1, white furnace, transcendental ulp bounds).
+ * The PCG generator (wgsl:189-199, 519) is pinned by known answers computed from the
+ * shader's text (tests/golden/make_pcg_vectors.py, tests/test_pcg_vectors.py).  The two
+ * estimator modes are pinned against each other at convergence (tests/
+ * test_oracle_cpu_mode.py), the toaster_cpu mode also by closed forms.
  * Parity unpinned:
- *   pcg_hash constants (wgsl:186-199) -- no reference test holds a hash value; only the
- *     bijection property is checked.
  *   or_render_ex rng_mode 1 (per-sample streams) -- this project's extension, no
  *     reference counterpart; it shares every formula with rng_mode 0.
- *   toaster-cpu's StdRng (rand 0.9.4 / rand_chacha 0.9.0) stream -- not restated.
+ *   toaster-cpu's StdRng (rand 0.9.4 / rand_chacha 0.9.0) stream -- not restated (the
+ *     toaster_cpu mode agrees with the CPU renderer in distribution, not per pixel).
  */
 #ifndef TOASTER_ORACLE_H
 #define TOASTER_ORACLE_H
