@@ -375,6 +375,26 @@ int32_t th_render(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* pa
  * overlap on one GPU (several contexts or threads) race. */
 int32_t th_render_wavefront_static(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
                                    const TH_GpuCamera* camera, const th_render_options* options);
+/* Progressive batches replayed from a captured hipGraph (the progressive loop of
+ * path_tracer.rs:655-674, 900-923 as one graph per batch shape; no counterpart in the
+ * reference, which re-records a dispatch per frame).  _create captures one batch of
+ * params->samples samples with the given options -- the megakernel with a fixed schedule
+ * (not TH_SCHEDULE_AUTO, not the wavefront, no statistics) reading the frame seed and
+ * accumulated sample count from device memory, then a kernel advancing both -- starting
+ * at params->frame_index and params->accumulated_samples.  Each _replay frame launches the
+ * graph once: batch k renders with frame_index = start + k and folds into the scene's
+ * accumulation over accumulated = start + k * samples, exactly as th_render would.
+ * Asynchronous; _state returns the next frame index and accumulated count.  The graph
+ * holds the scene as captured: after any th_scene_update, _replay returns
+ * TH_E_MISMATCH (capture a new graph).  Destroy graphs before their scene. */
+typedef struct th_progressive_graph th_progressive_graph;
+int32_t th_progressive_graph_create(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
+                                    const TH_GpuCamera* camera, const th_render_options* options,
+                                    th_progressive_graph** out);
+int32_t th_progressive_graph_replay(th_progressive_graph* graph, uint32_t frames);
+int32_t th_progressive_graph_state(const th_progressive_graph* graph, uint32_t* frame_index,
+                                   uint32_t* accumulated_samples);
+void th_progressive_graph_destroy(th_progressive_graph* graph);
 /* Blocks, then copies width*height float4 linear-HDR pixels (readback.rs:8-25). */
 int32_t th_readback(th_context* ctx, th_scene* scene, float* rgba, size_t bytes);
 /* Overwrites the accumulation buffer (progressive resume from a checkpoint). */

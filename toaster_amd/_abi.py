@@ -182,6 +182,11 @@ SIGNATURES = {
                               C.POINTER(RenderOptions)]),
     "th_render_wavefront_static": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
                                                C.POINTER(RenderOptions)]),
+    "th_progressive_graph_create": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
+                                                C.POINTER(RenderOptions), C.POINTER(P)]),
+    "th_progressive_graph_replay": (C.c_int32, [P, u32]),
+    "th_progressive_graph_state": (C.c_int32, [P, C.POINTER(u32), C.POINTER(u32)]),
+    "th_progressive_graph_destroy": (None, [P]),
     "th_readback": (C.c_int32, [P, P, P, C.c_size_t]),
     "th_upload_accumulation": (C.c_int32, [P, P, P, C.c_size_t]),
     "th_synchronize": (C.c_int32, [P]),

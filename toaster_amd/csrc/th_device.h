@@ -60,7 +60,17 @@ struct DevScene {
   const float* env_cdf;
   const float* env_rows;
   uint32_t env_rows_w;   // row width the row array was built for
+  // progressive graph replay: {frame_index, accumulated_samples} in device memory that
+  // the graph advances between replays; null for an ordinary launch (params rule)
+  const uint32_t* frame_state;
 };
+
+__device__ __forceinline__ uint32_t frame_of(const DevScene& S, const TH_GpuRenderParams& P) {
+  return S.frame_state != nullptr ? S.frame_state[0] : P.frame_index;
+}
+__device__ __forceinline__ uint32_t accumulated_of(const DevScene& S, const TH_GpuRenderParams& P) {
+  return S.frame_state != nullptr ? S.frame_state[1] : P.accumulated_samples;
+}
 
 struct Counters {
   uint64_t paths, closest_rays, shadow_rays, nodes, primrefs, tris, spheres, closest_hits,

@@ -82,6 +82,7 @@ struct th_scene {
   th_wavefront* wf = nullptr;  // allocated by the first wavefront render
   th_guided_ws* gw = nullptr;  // allocated by the first guided render
   th_tuning tuning = {0, 0, 0.0f, 0};
+  uint64_t generation = 0;  // bumped by every th_scene_update (progressive graphs check it)
 };
 
 

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderPar
   LaneCounters<COUNT> lc;
   Integrator<TMODE, COUNT> it{S, P, srgb_lut, st, lc};
   uint32_t idx = y * P.width + x;
-  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   for (uint32_t i = 0; i < P.samples; ++i) {
     v3 o, d;
@@ -190,10 +190,10 @@ __global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderPar
   if (COUNT) lc.paths += P.samples;
   float ns = (float)P.samples;
   float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-  if (P.accumulated_samples == 0u) {
+  if (accumulated_of(S, P) == 0u) {
     accum[idx] = b;
   } else {
-    float prev = (float)P.accumulated_samples;
+    float prev = (float)accumulated_of(S, P);
     float tot = prev + ns;
     float4 o = accum[idx];
     accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(64) k_render_sample(DevScene S, TH_GpuRenderPa
   uint32_t idx = y * P.width + x;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   if (active) {
-    uint32_t base = idx ^ (P.frame_index * 9781u);
+    uint32_t base = idx ^ (frame_of(S, P) * 9781u);
     for (uint32_t smp = j; smp < P.samples; smp += G) {
       uint32_t s = th::pcg_hash(base + smp * 0x9E3779B9u);
       v3 o, d;
@@ -253,10 +253,10 @@ __global__ void __launch_bounds__(64) k_render_sample(DevScene S, TH_GpuRenderPa
   if (active && j == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-    if (P.accumulated_samples == 0u) {
+    if (accumulated_of(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)P.accumulated_samples;
+      float prev = (float)accumulated_of(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(64) k_primary(DevScene S, TH_GpuRenderParams P
   if (!tile_pixel(P.width, P.height, &x, &y)) return;
   Stack st{lds_stack + threadIdx.x, (int)blockDim.x, 0};
   uint32_t idx = y * P.width + x;
-  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
   v3 o, d;
   camera_ray(cam, P, x, y, &s, &o, &d);
   RayStats<false> rs;
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   const uint32_t idx = y * P.width + x;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   if (active) {
-    uint32_t base = idx ^ (P.frame_index * 9781u);
+    uint32_t base = idx ^ (frame_of(S, P) * 9781u);
     uint32_t s = th::pcg_hash(base);
     for (uint32_t smp = PER_SAMPLE ? j : 0u; smp < P.samples; smp += PER_SAMPLE ? G : 1u) {
       if (PER_SAMPLE) s = th::pcg_hash(base + smp * 0x9E3779B9u);
@@ -344,10 +344,10 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   if (active && (lane % (4u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-    if (P.accumulated_samples == 0u) {
+    if (accumulated_of(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)P.accumulated_samples;
+      float prev = (float)accumulated_of(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_oct
   const uint32_t idx = y * P.width + x;
   OctetPath<ENV, 3, false> path{it, shadow};
   if (active) {
-    uint32_t base = idx ^ (P.frame_index * 9781u);
+    uint32_t base = idx ^ (frame_of(S, P) * 9781u);
     uint32_t s = th::pcg_hash(base);
     for (uint32_t smp = PER_SAMPLE ? j : 0u; smp < P.samples; smp += PER_SAMPLE ? G : 1u) {
       if (PER_SAMPLE) s = th::pcg_hash(base + smp * 0x9E3779B9u);
@@ -410,10 +410,10 @@ __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_oct
   if (active && (lane % (8u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-    if (P.accumulated_samples == 0u) {
+    if (accumulated_of(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)P.accumulated_samples;
+      float prev = (float)accumulated_of(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRender
         path.abandon();
       }
     };
-    assign(th::pcg_hash(idx ^ (P.frame_index * 9781u)), false);
+    assign(th::pcg_hash(idx ^ (frame_of(S, P) * 9781u)), false);
     for (;;) {
       if (role != TH_SPEC_IDLE && !fin) {
         if (!begun) {
@@ -560,10 +560,10 @@ __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRender
   if (active && (lane % (8u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(pr / ns, pg / ns, pb / ns, pa / ns);
-    if (P.accumulated_samples == 0u) {
+    if (accumulated_of(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)P.accumulated_samples;
+      float prev = (float)accumulated_of(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(256) k_render_quad_dyn(DevScene S, TH_GpuRende
     const uint32_t x = (tile % tiles_x) * 4u + (w & 3u), y = (tile / tiles_x) * 4u + (w >> 2);
     if (x >= P.width || y >= P.height) continue;
     const uint32_t idx = y * P.width + x;
-    uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+    uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
     for (uint32_t smp = 0; smp < P.samples; ++smp) {
       v3 o, d;
@@ -626,10 +626,10 @@ __global__ void __launch_bounds__(256) k_render_quad_dyn(DevScene S, TH_GpuRende
     if (lq == 0u) {
       float ns = (float)P.samples;
       float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-      if (P.accumulated_samples == 0u) {
+      if (accumulated_of(S, P) == 0u) {
         accum[idx] = b;
       } else {
-        float prev = (float)P.accumulated_samples;
+        float prev = (float)accumulated_of(S, P);
         float tot = prev + ns;
         float4 o = accum[idx];
         accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -654,7 +654,7 @@ __global__ void __launch_bounds__(64) k_primary_quad(DevScene S, TH_GpuRenderPar
   if (x >= P.width || y >= P.height) return;
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
   const uint32_t idx = y * P.width + x;
-  uint32_t s = th::pcg_hash(idx ^ (P.frame_index * 9781u));
+  uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
   v3 o, d;
   camera_ray(cam, P, x, y, &s, &o, &d);
   float t;
@@ -1048,6 +1048,7 @@ DevScene dev_scene(const th_scene* s) {
   S.env_cdf = s->d_env_cdf;
   S.env_rows = s->d_env_rows;
   S.env_rows_w = s->env_rows_w;
+  S.frame_state = nullptr;
   for (int k = 0; k < 3; ++k) {
     S.root_min[k] = s->root_min[k];
     S.root_max[k] = s->root_max[k];
@@ -1320,6 +1321,7 @@ int32_t th_scene_update(th_scene* s, uint32_t mask, const th_scene_desc* d) {
     geometry = true;
   }
   s->params = d->params;
+  s->generation++;
   if (geometry || refit) s->auto_valid = false;  // the fastest schedule may change with the tree
   if (refit) {
     s->params.bvh_node_count = s->node_count;
@@ -1348,113 +1350,15 @@ void th_scene_destroy(th_scene* s) {
   delete s;
 }
 
-// TH_SCHEDULE_AUTO (reference RNG, 4-wide megakernel): the first render of a scene with
-// these samples / bounces / background times the static, octet and speculative schedules
-// (same image) on this frame -- one warm-up and two timed launches each, at seed 0 of the
-// accumulation -- and keeps the fastest until the scene's geometry changes.  The trials
-// overwrite the accumulation, so a render that folds into it gets it back first; their
-// launches are left out of th_get_stats.
-static int32_t auto_schedule(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
-                             th_render_options* o) {
-  if (s->auto_valid && s->auto_samples == p->samples && s->auto_bounces == p->max_bounces &&
-      s->auto_background == p->background_kind) {
-    o->schedule = s->auto_choice;
-    return TH_OK;
-  }
-  int32_t rc = fold_timing(ctx);  // the caller's earlier renders keep their timings
-  if (rc) return rc;
-  const size_t bytes = (size_t)s->width * s->height * sizeof(float4);
-  const bool keep = p->accumulated_samples > 0u;
-  if (keep) {
-    if (!s->d_accum_save) TH_CHECK(hipMalloc((void**)&s->d_accum_save, bytes));
-    TH_CHECK(hipMemcpyAsync(s->d_accum_save, s->d_accum, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-  }
-  TH_GpuRenderParams tp = *p;
-  tp.accumulated_samples = 0u;
-  const uint32_t cands[3] = {TH_SCHEDULE_STATIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULATIVE};
-  float best = 0.0f;
-  uint32_t choice = TH_SCHEDULE_STATIC;
-  for (int c = 0; c < 3; ++c) {
-    th_render_options t = *o;
-    t.schedule = cands[c];
-    // a failed trial: the caller's accumulation comes back, the trials leave no timings
-    auto fail = [&](int32_t code) {
-      for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
-      ctx->ev_pending.clear();
-      if (keep) (void)hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream);
-      return code;
-    };
-    for (int r = 0; r < 3; ++r)
-      if ((rc = th_render(ctx, s, &tp, cam, &t))) return fail(rc);
-    float total = 0.0f;
-    for (size_t i = 0; i < ctx->ev_pending.size(); ++i) {
-      const auto& pr = ctx->ev_pending[i];
-      float ms = 0.0f;
-      hipError_t e = hipEventSynchronize(pr.second);
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
-      if (e != hipSuccess)
-        return fail(th_set_error(TH_E_DEVICE, std::string("schedule trial timing: ") + hipGetErrorString(e)));
-      if (i > 0) total += ms;  // the first launch warms up
-    }
-    for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
-    ctx->ev_pending.clear();
-    if (c == 0 || total < best) {
-      best = total;
-      choice = cands[c];
-    }
-  }
-  if (keep) TH_CHECK(hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-  s->auto_valid = true;
-  s->auto_samples = p->samples;
-  s->auto_bounces = p->max_bounces;
-  s->auto_background = p->background_kind;
-  s->auto_choice = choice;
-  o->schedule = choice;
-  return TH_OK;
-}
-
-int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
-                  const th_render_options* opt) {
-  if (!ctx || !s || !cam) return th_set_error(TH_E_INVALID, "null argument");
-  if (s->ctx != ctx) return th_set_error(TH_E_INVALID, "scene belongs to another context");
-  int32_t rc = check_render_params(s, p);
-  if (rc) return rc;
-  th_render_options o;
-  memset(&o, 0, sizeof o);
-  if (opt) o = *opt;
-  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 4u)
-    return th_set_error(TH_E_INVALID, "unknown render option");
-  if (o.schedule == TH_SCHEDULE_AUTO) {
-    if (o.rng_mode == TH_RNG_REFERENCE && o.traversal == TH_TRAVERSE_WIDE && o.kernel == TH_KERNEL_MEGA &&
-        !o.collect_stats && s->node_count > 0) {
-      if ((rc = auto_schedule(ctx, s, p, cam, &o))) return rc;
-    } else {
-      o.schedule = TH_SCHEDULE_STATIC;
-    }
-  }
-  if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
-    return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
-  if (o.kernel == TH_KERNEL_WAVEFRONT && o.collect_stats)
-    return th_set_error(TH_E_UNSUPPORTED, "statistics are counted by the megakernel");
-  TH_CHECK(hipSetDevice(ctx->device));
-  DevScene S = dev_scene(s);
-  uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
-  const uint32_t block = 64;
-  size_t lds = (size_t)depth * block * sizeof(uint2);
-  if (lds > 64 * 1024) return th_set_error(TH_E_UNSUPPORTED, "bvh too deep for the traversal stack");
-  if (o.collect_stats) {  // a fill kernel, not a memset node, so a caller may capture the render
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->d_counters,
-                       (uint32_t)(16 * sizeof(unsigned long long) / sizeof(uint32_t)));
-    TH_CHECK(hipGetLastError());
-  }
-  unsigned long long* cnt = ctx->d_counters;
-  std::pair<hipEvent_t, hipEvent_t> evp;
-  if ((rc = begin_timed(ctx, &evp))) return rc;
+// The kernel launches of one render on ctx->stream (th_render's dispatch by kernel,
+// traversal, schedule and random-stream mode); also what a progressive graph captures,
+// with S.frame_state set.
+static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                             const th_render_options& o, const DevScene& S, size_t lds, uint32_t block,
+                             unsigned long long* cnt) {
   if (o.kernel == TH_KERNEL_WAVEFRONT && p->max_bounces > 0) {
-    if ((rc = th_wavefront_render(ctx, s, p, cam, &o))) {
-      (void)end_timed(ctx, evp);
-      return rc;
-    }
+    int32_t rc = th_wavefront_render(ctx, s, p, cam, &o);
+    if (rc) return rc;
   } else if (o.traversal == TH_TRAVERSE_WIDE && s->node_count > 0) {
     const uint32_t qblock = 256;
     uint32_t depth = 3u * s->depth4 + 2u;
@@ -1574,11 +1478,242 @@ int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, con
 #undef TH_LAUNCH_G
 #undef TH_LAUNCH
   }
+  return TH_OK;
+}
+
+// TH_SCHEDULE_AUTO (reference RNG, 4-wide megakernel): the first render of a scene with
+// these samples / bounces / background times the static, octet and speculative schedules
+// (same image) on this frame -- one warm-up and two timed launches each, at seed 0 of the
+// accumulation -- and keeps the fastest until the scene's geometry changes.  The trials
+// overwrite the accumulation, so a render that folds into it gets it back first; their
+// launches are left out of th_get_stats.
+static int32_t auto_schedule(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                             th_render_options* o) {
+  if (s->auto_valid && s->auto_samples == p->samples && s->auto_bounces == p->max_bounces &&
+      s->auto_background == p->background_kind) {
+    o->schedule = s->auto_choice;
+    return TH_OK;
+  }
+  int32_t rc = fold_timing(ctx);  // the caller's earlier renders keep their timings
+  if (rc) return rc;
+  const size_t bytes = (size_t)s->width * s->height * sizeof(float4);
+  const bool keep = p->accumulated_samples > 0u;
+  if (keep) {
+    if (!s->d_accum_save) TH_CHECK(hipMalloc((void**)&s->d_accum_save, bytes));
+    TH_CHECK(hipMemcpyAsync(s->d_accum_save, s->d_accum, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  TH_GpuRenderParams tp = *p;
+  tp.accumulated_samples = 0u;
+  const uint32_t cands[3] = {TH_SCHEDULE_STATIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULATIVE};
+  float best = 0.0f;
+  uint32_t choice = TH_SCHEDULE_STATIC;
+  for (int c = 0; c < 3; ++c) {
+    th_render_options t = *o;
+    t.schedule = cands[c];
+    // a failed trial: the caller's accumulation comes back, the trials leave no timings
+    auto fail = [&](int32_t code) {
+      for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
+      ctx->ev_pending.clear();
+      if (keep) (void)hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+      return code;
+    };
+    for (int r = 0; r < 3; ++r)
+      if ((rc = th_render(ctx, s, &tp, cam, &t))) return fail(rc);
+    float total = 0.0f;
+    for (size_t i = 0; i < ctx->ev_pending.size(); ++i) {
+      const auto& pr = ctx->ev_pending[i];
+      float ms = 0.0f;
+      hipError_t e = hipEventSynchronize(pr.second);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
+      if (e != hipSuccess)
+        return fail(th_set_error(TH_E_DEVICE, std::string("schedule trial timing: ") + hipGetErrorString(e)));
+      if (i > 0) total += ms;  // the first launch warms up
+    }
+    for (auto& pr : ctx->ev_pending) ctx->ev_pool.push_back(pr);
+    ctx->ev_pending.clear();
+    if (c == 0 || total < best) {
+      best = total;
+      choice = cands[c];
+    }
+  }
+  if (keep) TH_CHECK(hipMemcpyAsync(s->d_accum, s->d_accum_save, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  s->auto_valid = true;
+  s->auto_samples = p->samples;
+  s->auto_bounces = p->max_bounces;
+  s->auto_background = p->background_kind;
+  s->auto_choice = choice;
+  o->schedule = choice;
+  return TH_OK;
+}
+
+int32_t th_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                  const th_render_options* opt) {
+  if (!ctx || !s || !cam) return th_set_error(TH_E_INVALID, "null argument");
+  if (s->ctx != ctx) return th_set_error(TH_E_INVALID, "scene belongs to another context");
+  int32_t rc = check_render_params(s, p);
+  if (rc) return rc;
+  th_render_options o;
+  memset(&o, 0, sizeof o);
+  if (opt) o = *opt;
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 4u)
+    return th_set_error(TH_E_INVALID, "unknown render option");
+  if (o.schedule == TH_SCHEDULE_AUTO) {
+    if (o.rng_mode == TH_RNG_REFERENCE && o.traversal == TH_TRAVERSE_WIDE && o.kernel == TH_KERNEL_MEGA &&
+        !o.collect_stats && s->node_count > 0) {
+      if ((rc = auto_schedule(ctx, s, p, cam, &o))) return rc;
+    } else {
+      o.schedule = TH_SCHEDULE_STATIC;
+    }
+  }
+  if (o.traversal == TH_TRAVERSE_WIDE && o.collect_stats)
+    return th_set_error(TH_E_UNSUPPORTED, "statistics are counted with the scalar traversals");
+  if (o.kernel == TH_KERNEL_WAVEFRONT && o.collect_stats)
+    return th_set_error(TH_E_UNSUPPORTED, "statistics are counted by the megakernel");
+  TH_CHECK(hipSetDevice(ctx->device));
+  DevScene S = dev_scene(s);
+  uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
+  const uint32_t block = 64;
+  size_t lds = (size_t)depth * block * sizeof(uint2);
+  if (lds > 64 * 1024) return th_set_error(TH_E_UNSUPPORTED, "bvh too deep for the traversal stack");
+  if (o.collect_stats) {  // a fill kernel, not a memset node, so a caller may capture the render
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)ctx->d_counters,
+                       (uint32_t)(16 * sizeof(unsigned long long) / sizeof(uint32_t)));
+    TH_CHECK(hipGetLastError());
+  }
+  unsigned long long* cnt = ctx->d_counters;
+  std::pair<hipEvent_t, hipEvent_t> evp;
+  if ((rc = begin_timed(ctx, &evp))) return rc;
+  if ((rc = launch_render(ctx, s, p, cam, o, S, lds, block, cnt))) {
+    (void)end_timed(ctx, evp);
+    return rc;
+  }
   hipError_t le = hipGetLastError();
   if ((rc = end_timed(ctx, evp))) return rc;
   if (le != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("render launch: ") + hipGetErrorString(le));
   ctx->counters_pending = o.collect_stats != 0;
   ctx->stats.paths = (uint64_t)p->width * p->height * p->samples;
+  return TH_OK;
+}
+
+// ---------------------------------------------------------------- progressive graph
+// One progressive batch captured as a hipGraph -- the render's kernels reading the frame
+// seed and accumulated sample count from device memory, then a one-lane kernel that
+// advances both -- so each replay is the next batch of the reference's progressive loop
+// (path_tracer.rs:655-674, 900-923: frame seed = frame counter, running average over
+// accumulated + samples) with no host work but the graph launch.
+struct th_progressive_graph {
+  th_context* ctx = nullptr;
+  th_scene* scene = nullptr;
+  uint32_t* d_state = nullptr;  // {frame_index, accumulated_samples}
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t samples = 0, frame_index = 0, accumulated = 0;
+  uint64_t generation = 0;
+};
+
+namespace {
+__global__ void k_frame_state_set(uint32_t* st, uint32_t frame_index, uint32_t accumulated) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[0] = frame_index;
+    st[1] = accumulated;
+  }
+}
+__global__ void k_frame_state_advance(uint32_t* st, uint32_t samples) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[0] = st[0] + 1u;
+    st[1] = st[1] + samples;
+  }
+}
+}  // namespace
+
+void th_progressive_graph_destroy(th_progressive_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_state) (void)hipFree(g->d_state);
+  delete g;
+}
+
+int32_t th_progressive_graph_create(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
+                                    const TH_GpuCamera* cam, const th_render_options* opt,
+                                    th_progressive_graph** out) {
+  if (!ctx || !s || !cam || !out) return th_set_error(TH_E_INVALID, "null argument");
+  *out = nullptr;
+  if (s->ctx != ctx) return th_set_error(TH_E_INVALID, "scene belongs to another context");
+  int32_t rc = check_render_params(s, p);
+  if (rc) return rc;
+  th_render_options o;
+  memset(&o, 0, sizeof o);
+  if (opt) o = *opt;
+  if (o.rng_mode > 1u || o.traversal > 2u || o.kernel > 1u || o.schedule > 4u)
+    return th_set_error(TH_E_INVALID, "unknown render option");
+  if (o.kernel != TH_KERNEL_MEGA || o.schedule == TH_SCHEDULE_AUTO || o.collect_stats)
+    return th_set_error(TH_E_UNSUPPORTED, "progressive graphs capture the megakernels with a fixed schedule and "
+                                          "no statistics");
+  TH_CHECK(hipSetDevice(ctx->device));
+  DevScene S = dev_scene(s);
+  uint32_t depth = std::max<uint32_t>(s->max_depth + 1u, 2u);
+  const uint32_t block = 64;
+  size_t lds = (size_t)depth * block * sizeof(uint2);
+  if (lds > 64 * 1024) return th_set_error(TH_E_UNSUPPORTED, "bvh too deep for the traversal stack");
+  th_progressive_graph* g = new th_progressive_graph();
+  g->ctx = ctx;
+  g->scene = s;
+  g->samples = p->samples;
+  g->frame_index = p->frame_index;
+  g->accumulated = p->accumulated_samples;
+  g->generation = s->generation;
+  hipError_t e = hipMalloc((void**)&g->d_state, 2 * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    delete g;
+    return th_set_error(TH_E_DEVICE, std::string("progressive graph state: ") + hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(k_frame_state_set, dim3(1), dim3(64), 0, ctx->stream, g->d_state, p->frame_index,
+                     p->accumulated_samples);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    th_progressive_graph_destroy(g);
+    return th_set_error(TH_E_DEVICE, std::string("progressive graph capture: ") + hipGetErrorString(e));
+  }
+  S.frame_state = g->d_state;
+  rc = launch_render(ctx, s, p, cam, o, S, lds, block, ctx->d_counters);
+  hipLaunchKernelGGL(k_frame_state_advance, dim3(1), dim3(64), 0, ctx->stream, g->d_state, p->samples);
+  hipError_t le = hipGetLastError();
+  e = hipStreamEndCapture(ctx->stream, &g->graph);  // always ends the capture
+  if (rc) {
+    th_progressive_graph_destroy(g);
+    return rc;
+  }
+  if (e == hipSuccess && le != hipSuccess) e = le;
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    th_progressive_graph_destroy(g);
+    return th_set_error(TH_E_DEVICE, std::string("progressive graph capture: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return TH_OK;
+}
+
+int32_t th_progressive_graph_replay(th_progressive_graph* g, uint32_t frames) {
+  if (!g) return th_set_error(TH_E_INVALID, "null argument");
+  if (g->generation != g->scene->generation)
+    return th_set_error(TH_E_MISMATCH, "the scene was updated after the graph was captured");
+  TH_CHECK(hipSetDevice(g->ctx->device));
+  for (uint32_t f = 0; f < frames; ++f) {
+    TH_CHECK(hipGraphLaunch(g->exec, g->ctx->stream));
+    g->frame_index += 1u;
+    g->accumulated += g->samples;
+  }
+  return TH_OK;
+}
+
+int32_t th_progressive_graph_state(const th_progressive_graph* g, uint32_t* frame_index, uint32_t* accumulated) {
+  if (!g || !frame_index || !accumulated) return th_set_error(TH_E_INVALID, "null argument");
+  *frame_index = g->frame_index;
+  *accumulated = g->accumulated;
   return TH_OK;
 }
 

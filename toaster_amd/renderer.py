@@ -117,6 +117,7 @@ class GpuScene:
         self.width, self.height = packed.params.width, packed.params.height
         self.accumulated_samples = 0
         self.frame_index = 0
+        self._graphs = weakref.WeakSet()  # progressive graphs, destroyed before the scene
 
     @classmethod
     def from_scene(cls, ctx, scene: Scene, node_capacity=0):
@@ -178,6 +179,12 @@ class GpuScene:
         self.frame_index += 1
         return p
 
+    def progressive_graph(self, samples, options: RenderOptions = None):
+        """A captured progressive batch of `samples` samples (th_progressive_graph_*)
+        continuing this scene's frame seed and running average; replay(n) renders the
+        next n batches."""
+        return ProgressiveGraph(self, samples, options)
+
     def readback(self):
         out = np.empty((self.height, self.width, 4), np.float32)
         A.check(self._lib.th_readback(self.ctx.handle, self._h, out.ctypes.data, out.nbytes))
@@ -208,8 +215,43 @@ class GpuScene:
 
     def close(self):
         if self._h:
+            for g in list(self._graphs):
+                g.close()
             self._lib.th_scene_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ProgressiveGraph:
+    """One progressive batch captured as a hipGraph (th_progressive_graph_create); each
+    replayed frame is the next batch of the scene's progressive loop, and the scene's
+    frame seed and sample count follow the replays."""
+
+    def __init__(self, scene: GpuScene, samples, options: RenderOptions = None):
+        self.scene = scene
+        p = scene.params(samples=samples)
+        o = (options or RenderOptions()).struct()
+        h = C.c_void_p()
+        A.check(scene._lib.th_progressive_graph_create(scene.ctx.handle, scene._h, C.byref(p),
+                                                       C.byref(scene.packed.camera), C.byref(o), C.byref(h)))
+        self._h = h
+        scene._graphs.add(self)
+
+    def replay(self, frames=1):
+        A.check(self.scene._lib.th_progressive_graph_replay(self._h, int(frames)))
+        f, a = C.c_uint32(), C.c_uint32()
+        A.check(self.scene._lib.th_progressive_graph_state(self._h, C.byref(f), C.byref(a)))
+        self.scene.frame_index, self.scene.accumulated_samples = f.value, a.value
+
+    def close(self):
+        if self._h:
+            self.scene._lib.th_progressive_graph_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:
