@@ -13,7 +13,11 @@ roofline.achieved uses algorithmic bytes per launch: the reference-layout bytes 
 reference traversal reads for this frame (48 B per node visited, 8 B per primitive
 reference, 64/32 B per triangle/sphere test, 48+80 B per closest hit, 96 B per light
 record scanned, 16 B per output pixel), counted by the oracle on the same seeds, divided
-by the kernel's mean duration from HIP events around each launch.
+by the kernel's mean duration from HIP events around each launch.  roofline.nearest_first
+prices the node visits of a nearer-child-first walk (the GPU's ordered-traversal
+counters on the same frame), the order the 4-wide kernels visit in.  parity: the timed
+frame against the oracle's image of the same frame (in the cpu_baseline leg, outside the
+timed region); a frame beyond tolerance makes the run exit with status 1.
 
 equal_time compares the guided tracer with the plain tracer on the indirect-lit terrain
 (262,144 triangles, 1280x720, 8 bounces): a plain reference at --reference-spp, the
@@ -22,7 +26,12 @@ scratch (all training inside the timed region), five times with independent guid
 and sample streams.  From the two plain points, relRMSE^2 = C / spp + E (E = the
 reference's own noise); the plain time to reach a guided run's error is
 T_plain(256) * (C / (relRMSE_guided^2 - E)) / 256, and time_ratio is the guided time over
-that, averaged over the five runs.
+that, averaged over the five runs.  Ablations with the estimator held fixed: the same
+ratio against the plain estimator without the environment sample (what the guided default
+drops at all but a path's last vertex), the mixture estimator around the untrained network
+(relRMSE at 256 spp), and the guided tracer keeping the environment sample.  mfma: the
+network's inference and training launches timed by event pairs in a separate default run,
+their algorithmic FLOP/s against the 2.5 PFLOP/s dense BF16 peak.
 
 reference_suite times the reference's own benchmark scenes (scenes/benchmarks/001-007)
 as its benchmark mode does; batched renders 16 scenes x 4 frames in one persistent
@@ -42,6 +51,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    """Progress on stderr (stdout carries the one JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(c, pixels, accumulated=False):
     b = (48 * c["nodes_visited"] + 8 * c["primref_tests"] + 64 * c["triangle_tests"] +
          32 * c["sphere_tests"] + 48 * c["closest_hits"] + 80 * c["triangle_hits"] +
@@ -52,13 +66,66 @@ def algorithmic_bytes(c, pixels, accumulated=False):
     return b
 
 
-def cpu_baseline(sc, budget_s=8.0):
-    """The oracle (C++ restatement, pcg mode) on this host's cores: full frames until the
-    budget is spent; also returns its counters for the algorithmic-byte figure."""
+def _native_oracle():
+    """Builds the oracle with -O3 -march=native for this host into a temporary directory
+    (the in-tree build is portable); returns (path, flags) or (None, reason)."""
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "oracle", "oracle.cpp")
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle.so")
+    flags = ["-O3", "-march=native", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared"]
+    try:
+        subprocess.run(["g++"] + flags + ["-o", out, src], check=True, capture_output=True, timeout=240)
+    except Exception as e:  # no compiler on the host: the portable build is used
+        return None, f"native build failed ({type(e).__name__})"
+    return out, " ".join(flags)
+
+
+def _cpu_identity():
+    info = {}
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)",
+                             "CPU family", "Model", "Stepping", "CPU max MHz", "L3 cache"):
+                info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    if "Model name" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["Model name"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
+
+
+def cpu_baseline(sc, gpu_image, budget_s=8.0):
+    """The oracle (C++ restatement, pcg mode, built -O3 -march=native for this host) on
+    this host's cores: full frames of the headline workload until the budget is spent,
+    and one frame on one thread.  Its first frame is the headline frame (frame seed 0):
+    the GPU image is compared with it here, outside the timed region (|gpu - oracle| <=
+    1e-4 (1 + |oracle|) per channel on >= 99.5 % of pixels).  Also returns its counters
+    for the algorithmic-byte figure."""
+    import numpy as np
     import oracle
+    native, flags = _native_oracle()
+    if native:
+        oracle.LIB_PATH = native
     oc = oracle.OracleScene(sc)
     threads = max(1, min(os.cpu_count() or 1, 16))
-    _, counters = oc.render(threads=threads, counters=True)
+    ref, counters = oc.render(threads=threads, counters=True)
+    close = np.abs(gpu_image - ref) <= 1e-4 * (1.0 + np.abs(ref))
+    parity = {"within_tolerance": round(float(np.mean(np.all(close, axis=-1))), 6),
+              "bit_identical": round(float(np.mean(np.all(gpu_image == ref, axis=-1))), 6),
+              "tolerance": "|gpu - oracle| <= 1e-4 (1 + |oracle|) per channel, >= 99.5 % of pixels",
+              "image": "headline frame, frame seed 0"}
+    parity["ok"] = parity["within_tolerance"] >= 0.995
     frames, t0 = 0, time.perf_counter()
     while True:
         oc.render(threads=threads)
@@ -71,23 +138,16 @@ def cpu_baseline(sc, budget_s=8.0):
     oc.render(threads=1)
     dt1 = time.perf_counter() - t1
     oc.close()
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     paths = sc.paths_per_frame
+    ident = _cpu_identity()
     return {
         "value": round(frames * paths / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads,
         "kind": "port",
         "sample": f"{frames} full frames of the same workload (oracle, pcg mode, OpenMP over pixels)",
         "single_thread_mpaths_s": round(sc.paths_per_frame / dt1 / 1e6, 4),
-        "cpu_model": model, "nproc": os.cpu_count(),
-    }, counters
+        "build": flags if native else "in-tree oracle/liboracle.so (" + flags + ")",
+        "cpu_model": ident.get("Model name", ""), "lscpu": ident, "nproc": os.cpu_count(),
+    }, counters, parity
 
 
 def scene_update_leg(ctx, sc, frames=10):
@@ -207,8 +267,16 @@ def reference_suite(warmup=5, runs=21):
     return out
 
 
+MLP_MACS = 48 * 64 + 64 * 64 + 64 * 64 + 64 * 40  # 13,824 per evaluation
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
 def equal_time_guided(reference_spp):
-    """Equal-time relRMSE of guided vs plain tracing on the terrain workload."""
+    """Equal-time relRMSE of guided vs plain tracing on the terrain workload, with
+    ablations that hold the estimator fixed: the plain estimator without the environment
+    sample (what the guided default drops), the same mixture estimator around the
+    untrained network, and the guided tracer keeping the environment sample; plus the
+    network's MFMA throughput from a separately timed run."""
     from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
     from toaster_amd.guide import GUIDED_DEFAULTS, Guide, rel_rmse, render_guided
     sc = scenes.terrain(1280, 720, samples=16, max_bounces=8)
@@ -246,7 +314,10 @@ def equal_time_guided(reference_spp):
     fn = plain[best]
     chunk = 256
     t_ref = 0.0
+    log(f"equal_time: plain kernel {best}; reference {reference_spp} spp")
     for c in range(max(1, reference_spp // chunk)):  # progressive chunks into one average
+        if c % 16 == 0:
+            log(f"equal_time: reference {c * chunk}/{reference_spp} spp")
         if best == "wavefront_1spp_passes":
             gs.frame_index = 1000 + c
             gs.accumulated_samples = c * chunk
@@ -258,39 +329,97 @@ def equal_time_guided(reference_spp):
                                                                    kernel=(A.TH_KERNEL_WAVEFRONT if best == "wavefront"
                                                                            else A.TH_KERNEL_MEGA))))[0]
     ref = gs.readback()
-    pts = {}
-    for n in (64, 256):
-        dt, _ = timed(lambda: fn(n, 7 + n))
-        pts[n] = (dt, rel_rmse(gs.readback(), ref))
-    (t64, r64), (t256, r256) = pts[64], pts[256]
-    C = (r64 ** 2 - r256 ** 2) / (1 / 64 - 1 / 256)
-    E = max(r256 ** 2 - C / 256, 0.0)
+
+    def curve(render):
+        """Two points (64, 256 spp) of an estimator: relRMSE^2 = C / spp + E, where E is
+        the reference's own noise plus any difference in what the estimator converges to."""
+        pts = {}
+        for n in (64, 256):
+            dt, _ = timed(lambda: render(n, 7 + n))
+            pts[n] = (dt, rel_rmse(gs.readback(), ref))
+        (t64, r64), (t256, r256) = pts[64], pts[256]
+        C = (r64 ** 2 - r256 ** 2) / (1 / 64 - 1 / 256)
+        E = max(r256 ** 2 - C / 256, 0.0)
+        return {"t256": t256, "r256": r256, "r64": r64, "C": C, "E": E}
+
+    def t_equal(cv, rel):
+        """Time the estimator of curve `cv` needs to reach relRMSE `rel`."""
+        return cv["t256"] * (cv["C"] / max(rel ** 2 - cv["E"], 1e-12)) / 256
+
+    log("equal_time: plain curves, guided runs, ablations")
+    base = curve(fn)
+    no_nee = curve(lambda n, f: (setattr(gs, "frame_index", f),
+                                 render_guided(gs, g0, n, alpha=0.0, train=False, env_nee=False, progressive=False,
+                                               warmup_steps=0)))
     runs = []
     for seed in (1, 2, 3, 4, 5):  # independent guides and sample streams: the ratio's spread
         g = Guide(ctx, seed=seed)
         gs.frame_index = 2 + seed
         dtg, st = timed(lambda: render_guided(gs, g, 256, progressive=False, **GUIDED_DEFAULTS))
         relg = rel_rmse(gs.readback(), ref)
-        n_eq = C / max(relg ** 2 - E, 1e-12)
-        t_eq = t256 * n_eq / 256
+        t_eq, t_eq_nn = t_equal(base, relg), t_equal(no_nee, relg)
         runs.append({"seconds": round(dtg, 4), "rel_rmse": round(relg, 5),
                      "plain_seconds_to_guided_error": round(t_eq, 4), "time_ratio": round(dtg / t_eq, 4),
+                     "time_ratio_vs_plain_without_env_sample": round(dtg / t_eq_nn, 4),
                      "train_steps": st["train_steps"], "records": st["records"]})
         g.close()
+    # the same mixture estimator around the untrained network (no training, alpha from the
+    # first pass): what the trained weights add
+    gu = Guide(ctx, seed=1)
+    kw_untrained = dict(GUIDED_DEFAULTS, alpha_ramp=0, warmup_steps=0)
+    gs.frame_index = 3
+    dtu, _ = timed(lambda: render_guided(gs, gu, 256, train=False, progressive=False, alpha_from_start=True,
+                                         **kw_untrained))
+    relu = rel_rmse(gs.readback(), ref)
+    gu.close()
+    # the guided tracer keeping the reference's environment sample at every vertex
+    nee_runs = []
+    for seed in (1, 2):
+        g = Guide(ctx, seed=seed)
+        gs.frame_index = 2 + seed
+        dtn, _ = timed(lambda: render_guided(gs, g, 256, progressive=False, **dict(GUIDED_DEFAULTS, env_nee=True)))
+        reln = rel_rmse(gs.readback(), ref)
+        nee_runs.append({"seconds": round(dtn, 4), "rel_rmse": round(reln, 5),
+                         "time_ratio": round(dtn / t_equal(base, reln), 4)})
+        g.close()
+    # the network's launches timed on their own (event pairs around every inference and
+    # training launch of a default guided run; not part of the timed runs above)
+    gt = Guide(ctx, seed=1)
+    gs.frame_index = 3
+    st = render_guided(gs, gt, 256, progressive=False, time_network=True, **GUIDED_DEFAULTS)
+    gt.close()
+    infer_tf = st["inferred_rows"] * 2 * MLP_MACS / max(st["infer_ms"] * 1e-3, 1e-12) / 1e12
+    train_tf = st["trained_samples"] * 3 * 2 * MLP_MACS / max(st["train_ms"] * 1e-3, 1e-12) / 1e12
+    mfma = {"inference": {"rows": st["inferred_rows"], "ms": round(st["infer_ms"], 3),
+                          "tflops": round(infer_tf, 2), "utilization": round(infer_tf / BF16_PEAK_TFLOPS, 5)},
+            "training": {"samples": st["trained_samples"], "steps": st["train_steps"], "ms": round(st["train_ms"], 3),
+                         "tflops": round(train_tf, 2), "utilization": round(train_tf / BF16_PEAK_TFLOPS, 5)},
+            "flops": "2 x 13,824 per evaluation; 3 x that per trained sample (forward, two backward products); "
+                     "peak 2.5 PFLOP/s dense BF16"}
     g0.close()
     gs.close()
     ctx.close()
+    mean = lambda rs, k: sum(r[k] for r in rs) / len(rs)  # noqa: E731
     return {
         "workload": "terrain_262144: 512x256-cell fBm height field under a roof, sky + sun HDR "
                     "(sun blocked), 1280x720, 8 bounces",
         "reference_spp": reference_spp, "reference_seconds": round(t_ref, 2),
         "plain_kernel": best, "plain_ms_per_spp": {k: round(v * 1e3, 3) for k, v in speed.items()},
-        "plain": {"spp": 256, "seconds": round(t256, 4), "rel_rmse": round(r256, 5),
-                  "mpaths_s": round(256 * px / t256 / 1e6, 2), "rel_rmse_64spp": round(r64, 5)},
+        "plain": {"spp": 256, "seconds": round(base["t256"], 4), "rel_rmse": round(base["r256"], 5),
+                  "mpaths_s": round(256 * px / base["t256"] / 1e6, 2), "rel_rmse_64spp": round(base["r64"], 5)},
+        "plain_without_env_sample": {"spp": 256, "seconds": round(no_nee["t256"], 4),
+                                     "rel_rmse": round(no_nee["r256"], 5),
+                                     "rel_rmse_64spp": round(no_nee["r64"], 5)},
         "guided": {"spp": 256, "runs": runs, "settings": GUIDED_DEFAULTS,
-                   "mpaths_s": round(256 * px / (sum(r["seconds"] for r in runs) / len(runs)) / 1e6, 2)},
-        "reference_noise_rel_rmse": round(E ** 0.5, 5),
-        "time_ratio": round(sum(r["time_ratio"] for r in runs) / len(runs), 4), "target_time_ratio": 0.5,
+                   "mpaths_s": round(256 * px / mean(runs, "seconds") / 1e6, 2)},
+        "guided_untrained_network": {"spp": 256, "seconds": round(dtu, 4), "rel_rmse": round(relu, 5),
+                                     "trained_over_untrained_rel_rmse": round(mean(runs, "rel_rmse") / relu, 4)},
+        "guided_with_env_sample": {"spp": 256, "runs": nee_runs,
+                                   "time_ratio": round(mean(nee_runs, "time_ratio"), 4)},
+        "mfma": mfma,
+        "reference_noise_rel_rmse": round(base["E"] ** 0.5, 5),
+        "time_ratio": round(mean(runs, "time_ratio"), 4), "target_time_ratio": 0.5,
+        "time_ratio_vs_plain_without_env_sample": round(mean(runs, "time_ratio_vs_plain_without_env_sample"), 4),
     }
 
 
@@ -336,6 +465,20 @@ def batched_leg(runs=3):
             "one_launch_per_frame_gpaths_s": round(paths / dseq / 1e9, 3)}
 
 
+def copy_calibration(mib=1024, copies=10):
+    """A plain device-to-device copy of `mib` MiB, `copies` times (torch's copy kernel on
+    device memory); each copy reads and writes exactly mib MiB, so the profiler's
+    FETCH_SIZE / WRITE_SIZE of that kernel over mib * 1024 KiB is its scale factor."""
+    import torch
+    n = mib * (1 << 20)
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(copies):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {copies} copies of {n} bytes ({mib * 1024} KiB each)", file=sys.stderr)
+
+
 def profile_guided(passes):
     """The guided tracer alone, as the equal-time leg runs it (fresh guide, default
     settings, one sample per pixel per pass) on the terrain workload."""
@@ -368,11 +511,17 @@ def main():
     ap.add_argument("--no-guided", action="store_true")
     ap.add_argument("--no-suite", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--reference-spp", type=int, default=4096)
+    ap.add_argument("--reference-spp", type=int, default=16384)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 1 GiB device buffer 10 times (a copy kernel of known size for the "
+                         "profiler's FETCH_SIZE / WRITE_SIZE calibration); prints no JSON line")
     ap.add_argument("--profile-guided", type=int, default=0, metavar="PASSES",
                     help="only render PASSES guided passes on the terrain workload (for a profiler); "
                          "prints no JSON line")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.profile_guided:
         profile_guided(args.profile_guided)
         return
@@ -404,6 +553,7 @@ def main():
                                     collect_stats=True))
     st = ctx.stats()
     rays_per_frame = st["rays"]
+    ordered_counters = dict(st)  # nearer-child-first traversal: the bytes a nearest-first walk reads
     for _ in range(args.warmup):
         gs.render(options=opts, frame_index=0, accumulated_samples=0)
     ctx.synchronize()
@@ -414,6 +564,7 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     st = ctx.stats()
+    headline_image = gs.readback().copy()  # the timed frame (seed 0, no accumulation)
     # other kernel variants on the same frame (short runs, reported beside the headline)
     variants = {}
     for name, o in (("megakernel_reference_traversal", RenderOptions(rng_mode=rng, traversal=A.TH_TRAVERSE_REFERENCE)),
@@ -453,16 +604,21 @@ def main():
     ms_step = dt * 1e3 / args.steps
     kernel_ms = st["total_kernel_ms"] / max(1, st["launches"])
 
+    log("headline done; equal_time leg")
     equal_time = None if args.no_guided else equal_time_guided(args.reference_spp)
+    log("reference suite leg")
     suite = None if args.no_suite else reference_suite()
+    log("batched leg")
     batched = None if args.no_batch else batched_leg()
-    cpu, counters = (None, None)
+    cpu, counters, parity = (None, None, None)
     if not args.no_cpu:
-        cpu, counters = cpu_baseline(sc)
+        log("cpu baseline leg (native oracle build, parity check)")
+        cpu, counters, parity = cpu_baseline(sc, headline_image)
     roofline = None
     if counters is not None:
         bytes_launch = algorithmic_bytes(counters, sc.width * sc.height)
         achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+        bytes_nf = algorithmic_bytes(ordered_counters, sc.width * sc.height)
         traffic = None
         tpath = os.path.join(ROOT, "profiles", f"traffic_bvh_stress_{args.triangles}.json")
         if os.path.exists(tpath):
@@ -472,7 +628,15 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "algorithmic_bytes_per_launch": bytes_launch,
                     "kernel": kernel_name,
-                    "kernel_ms": round(kernel_ms, 5)}
+                    "kernel_ms": round(kernel_ms, 5),
+                    # the same record sizes over the node visits of a nearer-child-first
+                    # walk (the GPU's ordered traversal counters), the order the 4-wide
+                    # kernels visit in: what the headline kernel actually has to read
+                    "nearest_first": {"algorithmic_bytes_per_launch": bytes_nf,
+                                      "achieved": round(bytes_nf / (kernel_ms * 1e-3) / 1e9, 2),
+                                      "frac": round(bytes_nf / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                                      "nodes_visited": ordered_counters["nodes_visited"],
+                                      "reference_order_nodes_visited": counters["nodes_visited"]}}
     line = {
         "metric": "Mpaths/s + equal-time relRMSE: guided vs plain HIP tracer, 1 MI355X",
         "value": round(args.steps * paths / dt / 1e6, 3),
@@ -494,9 +658,13 @@ def main():
         "reference_suite": suite,
         "batched": batched,
         "cpu_baseline": cpu,
+        "parity": parity,
         "host": platform.node(),
     }
     print(json.dumps(line))
+    if parity is not None and not parity["ok"]:
+        print("bench.py: the headline image differs from the oracle beyond tolerance", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -513,12 +513,20 @@ typedef struct th_guided_options {
  * untrained network (as created or as set by th_guide_set_params) can be measured;
  * with alpha_ramp > 0 the ramp still starts from 0.  Other flag bits are rejected. */
 #define TH_GUIDED_ALPHA_FROM_START 2u
+/* Time every inference and training launch of the network with a HIP event pair
+ * (infer_ms, train_ms of the statistics); for measurement, it adds two event records per
+ * launch. */
+#define TH_GUIDED_TIME_NETWORK 4u
 
 typedef struct th_guided_stats {
     uint64_t diffuse_vertices; /* deferred diffuse vertices over the call               */
     uint64_t records;          /* training samples recorded                            */
     uint32_t train_steps;      /* Adam steps taken by this call                        */
     float last_loss;           /* loss of the last step                                */
+    uint64_t inferred_rows;    /* network evaluations (one per guided diffuse vertex)  */
+    uint64_t trained_samples;  /* samples in the Adam steps taken                      */
+    float infer_ms, train_ms;  /* device time of inference / training launches         */
+                               /*   (TH_GUIDED_TIME_NETWORK; 0 otherwise)              */
     uint64_t reserved[2];
 } th_guided_stats;
 

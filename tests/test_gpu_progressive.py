@@ -25,12 +25,13 @@ class Recorder(FrameSink):
         return self.batches.pop(0) if self.batches else None
 
 
-def test_session_equals_manual_batches(gpu_ctx):
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_session_equals_manual_batches(gpu_ctx, use_graph):
     sc = scenes.cornell(samples=4)
     opts = RenderOptions(traversal=A.TH_TRAVERSE_WIDE)
-    print("case progressive session cornell target 10", flush=True)
+    print(f"case progressive session cornell target 10 graph={use_graph}", flush=True)
     sink = Recorder()
-    out = render_progressive_with_sink(gpu_ctx, sc, 10, sink, options=opts)
+    out = render_progressive_with_sink(gpu_ctx, sc, 10, sink, options=opts, use_graph=use_graph)
     assert out["accumulated_samples"] == 10 and out["frames"] == 3
     assert [(i, n, acc) for i, n, acc, _, _ in sink.frames] == [(0, 4, 4), (1, 4, 8), (2, 2, 10)]
     gs = GpuScene.from_scene(gpu_ctx, sc)
@@ -52,6 +53,24 @@ def test_sink_stops_and_sets_batches(gpu_ctx):
     sink = Recorder(batches=[1, 3])
     out = render_progressive_with_sink(gpu_ctx, sc, 6, sink)
     assert [(n, acc) for _, n, acc, _, _ in sink.frames] == [(1, 1), (3, 4), (2, 6)]
+
+
+def test_graph_session_follows_changing_batches(gpu_ctx):
+    """Replayed batches (one captured graph per run of equal batch sizes) deliver the
+    eager session's frames bit for bit while the sink changes the batch size."""
+    sc = scenes.cornell(64, 36, samples=2)
+    opts = RenderOptions(traversal=A.TH_TRAVERSE_WIDE, schedule=A.TH_SCHEDULE_SPECULATIVE)
+    print("case progressive graph session with changing batches", flush=True)
+    runs = []
+    for use_graph in (False, True):
+        sink = Recorder(batches=[2, 2, 3, 3, 1, 2])
+        render_progressive_with_sink(gpu_ctx, sc, 14, sink, options=opts, use_graph=use_graph)
+        runs.append(sink.frames)
+    assert [(n, acc) for _, n, acc, _, _ in runs[1]] == [(2, 2), (2, 4), (3, 7), (3, 10), (1, 11), (2, 13), (1, 14)]
+    for a, b in zip(*runs):
+        assert a[:3] == b[:3]
+        assert np.array_equal(a[4].view(np.uint32), b[4].view(np.uint32))
+        assert np.array_equal(a[3], b[3])
 
 
 def _moving_cornell(frame):

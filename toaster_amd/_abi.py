@@ -29,6 +29,7 @@ TH_SCHEDULE_STATIC, TH_SCHEDULE_DYNAMIC, TH_SCHEDULE_OCTET, TH_SCHEDULE_SPECULAT
 TH_GUIDE_FEATURES, TH_GUIDE_OUTPUTS, TH_GUIDE_PARAMS = 48, 40, 14056
 TH_GUIDED_NO_ENV_NEE = 1
 TH_GUIDED_ALPHA_FROM_START = 2
+TH_GUIDED_TIME_NETWORK = 4
 
 TH_CHANGE_CAMERA = 1
 TH_CHANGE_SPHERES = 2
@@ -160,7 +161,8 @@ class GuidedOptions(C.Structure):
 
 class GuidedStats(C.Structure):
     _fields_ = [("diffuse_vertices", C.c_uint64), ("records", C.c_uint64), ("train_steps", u32),
-                ("last_loss", f32), ("reserved", C.c_uint64 * 2)]
+                ("last_loss", f32), ("inferred_rows", C.c_uint64), ("trained_samples", C.c_uint64),
+                ("infer_ms", f32), ("train_ms", f32), ("reserved", C.c_uint64 * 2)]
 
 
 SIGNATURES = {

@@ -429,7 +429,10 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
 __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
                                                  const float* __restrict__ srgb_lut, uint32_t li) {
   const uint32_t n = B.cnt[GD_C(3)];
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.cnt[GD_C(5)], n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(&B.cnt[GD_C(5)], n);
+    if (A.alpha > 0.0f) atomicAdd(&B.cnt[GD_C(6)], n);  // rows the network evaluated
+  }
   Stack st{nullptr, 0, 0};
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
@@ -652,8 +655,9 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if (rc) return rc;
   if (!(opt->alpha >= 0.0f && opt->alpha <= 0.95f))
     return th_set_error(TH_E_INVALID, "alpha must lie in [0, 0.95]");
-  if (opt->flags & ~(TH_GUIDED_NO_ENV_NEE | TH_GUIDED_ALPHA_FROM_START))
+  if (opt->flags & ~(TH_GUIDED_NO_ENV_NEE | TH_GUIDED_ALPHA_FROM_START | TH_GUIDED_TIME_NETWORK))
     return th_set_error(TH_E_INVALID, "unknown guided flag");
+  const bool time_net = (opt->flags & TH_GUIDED_TIME_NETWORK) != 0u;
   const uint32_t batch = opt->batch ? opt->batch : (1u << 18);
   const uint32_t steps = opt->steps_per_pass ? opt->steps_per_pass : 1u;
   const uint32_t wrate = opt->warmup_steps ? std::max<uint32_t>(opt->warmup_rate, 1u) : steps;
@@ -695,6 +699,29 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   std::pair<hipEvent_t, hipEvent_t> evp;
   if ((rc = begin_timed(ctx, &evp))) return rc;
   hipStream_t sm = ctx->stream;
+  // TH_GUIDED_TIME_NETWORK: an event pair around every inference and training launch
+  std::vector<hipEvent_t> ev_infer, ev_train;
+  auto mark = [&](std::vector<hipEvent_t>& v) {
+    if (!time_net) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) == hipSuccess) {
+      if (hipEventRecord(e, sm) == hipSuccess)
+        v.push_back(e);
+      else
+        (void)hipEventDestroy(e);
+    }
+  };
+  auto span_ms = [&](std::vector<hipEvent_t>& v) {
+    float total = 0.0f;
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess) total += ms;
+    }
+    for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    v.clear();
+    return total;
+  };
+  uint64_t trained = 0;
   hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 1u);
   uint32_t cnt_host[GD_C(32)];
   for (uint32_t smp = 0; smp < p->samples; ++smp) {
@@ -730,10 +757,16 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
       hipLaunchKernelGGL(gd_shade_a, dim3(grid_scalar), dim3(block), 0, sm, B, A, S, *p,
                          (const float*)ctx->d_srgb_lut, li);
       if (A.defer) {
-        if (A.alpha > 0.0f &&
-            (rc = guide_infer(g, B.cnt + GD_C(3), npix, B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm))) {
-          (void)end_timed(ctx, evp);
-          return rc;
+        if (A.alpha > 0.0f) {
+          mark(ev_infer);
+          rc = guide_infer(g, B.cnt + GD_C(3), npix, B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm);
+          mark(ev_infer);
+          if (rc) {
+            (void)end_timed(ctx, evp);
+            (void)span_ms(ev_infer);
+            (void)span_ms(ev_train);
+            return rc;
+          }
         }
         hipLaunchKernelGGL(gd_shade_b, dim3(grid_scalar), dim3(block), 0, sm, B, A, S, *p,
                            (const float*)ctx->d_srgb_lut, li);
@@ -755,11 +788,17 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
         st.records += cnt;
         if (cnt < 1024u) continue;  // too few samples for a step
         const size_t off = (size_t)j * chunk_cap;
-        if ((rc = guide_train(g, B.rX + off * 64u, B.rdir + off * 3u, B.rw + off, cnt))) {
+        mark(ev_train);
+        rc = guide_train(g, B.rX + off * 64u, B.rdir + off * 3u, B.rw + off, cnt);
+        mark(ev_train);
+        if (rc) {
           (void)end_timed(ctx, evp);
+          (void)span_ms(ev_infer);
+          (void)span_ms(ev_train);
           return rc;
         }
         st.train_steps++;
+        trained += cnt;
       }
     }
   }
@@ -771,10 +810,17 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   ctx->stats.paths = npix64 * p->samples;
   if (st.train_steps > 0 && (rc = guide_last_loss(g, &st.last_loss))) return rc;
   if (train) TH_CHECK(hipStreamSynchronize(sm));  // the last loss has landed in `st`
+  if (time_net) {
+    TH_CHECK(hipStreamSynchronize(sm));
+    st.infer_ms = span_ms(ev_infer);
+    st.train_ms = span_ms(ev_train);
+  }
   if (stats) {
     TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
     TH_CHECK(hipStreamSynchronize(sm));
     st.diffuse_vertices = cnt_host[GD_C(5)];
+    st.inferred_rows = cnt_host[GD_C(6)];
+    st.trained_samples = trained;
     *stats = st;
   }
   return TH_OK;

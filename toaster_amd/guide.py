@@ -95,20 +95,22 @@ GUIDED_DEFAULTS = dict(alpha=0.8, env_nee=False, batch=1 << 15, warmup_steps=768
 
 def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.8, train=True, steps_per_pass=1,
                   batch=1 << 15, progressive=True, env_nee=True, warmup_steps=768, warmup_rate=16,
-                  alpha_ramp=512, train_every=1, alpha_from_start=False):
+                  alpha_ramp=512, train_every=1, alpha_from_start=False, time_network=False):
     """`samples` passes of one sample per pixel with the guide (see th_render_guided);
     with progressive=True the running average and frame seed continue like
     GpuScene.render_progressive.  env_nee=False drops the environment sample at every
     diffuse vertex (TH_GUIDED_NO_ENV_NEE); the default keeps the reference's estimator,
     which alpha = 0 then reproduces.  alpha_from_start=True uses the network from the
     first pass even before it has taken a step (TH_GUIDED_ALPHA_FROM_START: an untrained
-    network with train=False).  Returns the call's statistics as a dict."""
+    network with train=False).  time_network=True times the network's launches
+    (TH_GUIDED_TIME_NETWORK).  Returns the call's statistics as a dict."""
     p = scene.params(samples=samples)
     if not progressive:
         p.accumulated_samples = 0
     o = A.GuidedOptions(alpha=alpha, train=1 if train else 0, steps_per_pass=steps_per_pass,
                         batch=batch, flags=((0 if env_nee else A.TH_GUIDED_NO_ENV_NEE)
-                                            | (A.TH_GUIDED_ALPHA_FROM_START if alpha_from_start else 0)),
+                                            | (A.TH_GUIDED_ALPHA_FROM_START if alpha_from_start else 0)
+                                            | (A.TH_GUIDED_TIME_NETWORK if time_network else 0)),
                         warmup_steps=warmup_steps, warmup_rate=warmup_rate, alpha_ramp=alpha_ramp,
                         train_every=train_every)
     st = A.GuidedStats()
@@ -118,7 +120,8 @@ def render_guided(scene: GpuScene, guide: Guide, samples, alpha=0.8, train=True,
         scene.accumulated_samples = p.accumulated_samples + samples
         scene.frame_index += 1
     return {"diffuse_vertices": st.diffuse_vertices, "records": st.records,
-            "train_steps": st.train_steps, "last_loss": st.last_loss}
+            "train_steps": st.train_steps, "last_loss": st.last_loss, "inferred_rows": st.inferred_rows,
+            "trained_samples": st.trained_samples, "infer_ms": st.infer_ms, "train_ms": st.train_ms}
 
 
 def rel_rmse(image, reference):

@@ -68,14 +68,31 @@ class FrameSink:
 
 
 def render_progressive_with_sink(ctx: GpuContext, scene: Scene, target_samples: int, sink: FrameSink,
-                                 options: RenderOptions = None, exposure_stops: float = 0.0) -> dict:
+                                 options: RenderOptions = None, exposure_stops: float = 0.0,
+                                 use_graph: bool = False) -> dict:
     """Renders `scene` progressively until `target_samples` are accumulated or the sink
-    stops; returns {frames, accumulated_samples, total_time}."""
+    stops; returns {frames, accumulated_samples, total_time}.  use_graph=True replays each
+    batch from a captured hipGraph (one graph per batch size, GpuScene.progressive_graph)
+    instead of launching it: same images, frame for frame."""
     if target_samples <= 0:
         raise ValueError("sample target must be greater than zero")
     t_total = time.perf_counter()
     gs = GpuScene.from_scene(ctx, scene)
+    graph = {}
     frame, accumulated = 0, 0
+
+    def render_batch(n):
+        if not use_graph:
+            gs.render(samples=n, frame_index=frame, accumulated_samples=accumulated, options=options)
+            return
+        g = graph.get("g")
+        if g is None or graph["samples"] != n:  # (re)captured where the loop stands
+            if g is not None:
+                g.close()
+            gs.frame_index, gs.accumulated_samples = frame, accumulated
+            graph["g"], graph["samples"] = gs.progressive_graph(n, options), n
+        graph["g"].replay(1)
+
     try:
         while accumulated < target_samples:
             if not sink.should_continue():
@@ -86,7 +103,7 @@ def render_progressive_with_sink(ctx: GpuContext, scene: Scene, target_samples: 
             if requested <= 0:
                 raise ValueError("frame sink sample count must be greater than zero")
             n = progressive_batch(requested, accumulated, target_samples)
-            gs.render(samples=n, frame_index=frame, accumulated_samples=accumulated, options=options)
+            render_batch(n)
             ctx.synchronize()
             t1 = time.perf_counter()
             image = gs.display_rgba8(exposure_stops)
