@@ -65,11 +65,17 @@ struct DevScene {
   const uint32_t* frame_state;
 };
 
+// The frame seed and accumulated sample count of a launch: from the params, or (FS, a
+// kernel instantiated for a progressive graph) from the graph's device state.
+template <bool FS>
 __device__ __forceinline__ uint32_t frame_of(const DevScene& S, const TH_GpuRenderParams& P) {
-  return S.frame_state != nullptr ? S.frame_state[0] : P.frame_index;
+  if constexpr (FS) return S.frame_state[0];
+  return P.frame_index;
 }
+template <bool FS>
 __device__ __forceinline__ uint32_t accumulated_of(const DevScene& S, const TH_GpuRenderParams& P) {
-  return S.frame_state != nullptr ? S.frame_state[1] : P.accumulated_samples;
+  if constexpr (FS) return S.frame_state[1];
+  return P.accumulated_samples;
 }
 
 struct Counters {

@@ -164,7 +164,7 @@ __device__ __forceinline__ bool tile_pixel(uint32_t W, uint32_t H, uint32_t* x, 
 
 // The megakernel: one lane per pixel, params.samples samples from the pixel's
 // reference PCG stream, then the progressive running average (wgsl:508-548).
-template <int TMODE, bool COUNT>
+template <int TMODE, bool COUNT, bool FS = false>
 __global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderParams P,
                                                      TH_GpuCamera cam, float4* __restrict__ accum,
                                                      const float* __restrict__ srgb_lut,
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderPar
   LaneCounters<COUNT> lc;
   Integrator<TMODE, COUNT> it{S, P, srgb_lut, st, lc};
   uint32_t idx = y * P.width + x;
-  uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
+  uint32_t s = th::pcg_hash(idx ^ (frame_of<FS>(S, P) * 9781u));
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   for (uint32_t i = 0; i < P.samples; ++i) {
     v3 o, d;
@@ -190,10 +190,10 @@ __global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderPar
   if (COUNT) lc.paths += P.samples;
   float ns = (float)P.samples;
   float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-  if (accumulated_of(S, P) == 0u) {
+  if (accumulated_of<FS>(S, P) == 0u) {
     accum[idx] = b;
   } else {
-    float prev = (float)accumulated_of(S, P);
+    float prev = (float)accumulated_of<FS>(S, P);
     float tot = prev + ns;
     float4 o = accum[idx];
     accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(64) k_render_pixel(DevScene S, TH_GpuRenderPar
 // (lane j takes samples j, j+G, ...); sample s of pixel idx draws from the stream
 // seeded pcg_hash((idx ^ frame*9781) + s*0x9E3779B9), which for s = 0 is the
 // reference seed.  The G partial sums are combined in a fixed tree order.
-template <int TMODE, bool COUNT, int G>
+template <int TMODE, bool COUNT, int G, bool FS = false>
 __global__ void __launch_bounds__(64) k_render_sample(DevScene S, TH_GpuRenderParams P,
                                                       TH_GpuCamera cam, float4* __restrict__ accum,
                                                       const float* __restrict__ srgb_lut,
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(64) k_render_sample(DevScene S, TH_GpuRenderPa
   uint32_t idx = y * P.width + x;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   if (active) {
-    uint32_t base = idx ^ (frame_of(S, P) * 9781u);
+    uint32_t base = idx ^ (frame_of<FS>(S, P) * 9781u);
     for (uint32_t smp = j; smp < P.samples; smp += G) {
       uint32_t s = th::pcg_hash(base + smp * 0x9E3779B9u);
       v3 o, d;
@@ -253,10 +253,10 @@ __global__ void __launch_bounds__(64) k_render_sample(DevScene S, TH_GpuRenderPa
   if (active && j == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-    if (accumulated_of(S, P) == 0u) {
+    if (accumulated_of<FS>(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)accumulated_of(S, P);
+      float prev = (float)accumulated_of<FS>(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(64) k_primary(DevScene S, TH_GpuRenderParams P
   if (!tile_pixel(P.width, P.height, &x, &y)) return;
   Stack st{lds_stack + threadIdx.x, (int)blockDim.x, 0};
   uint32_t idx = y * P.width + x;
-  uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
+  uint32_t s = th::pcg_hash(idx ^ (frame_of<false>(S, P) * 9781u));
   v3 o, d;
   camera_ray(cam, P, x, y, &s, &o, &d);
   RayStats<false> rs;
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(64) k_primary(DevScene S, TH_GpuRenderParams P
 // per-sample mode (sample j, j+G, ... per quad); in the reference RNG mode G = 1 and the
 // quad runs the pixel's samples in stream order.  A wave covers 16/G pixels.
 // LDS layout of the quad kernels: `cached` 4-wide nodes, then the per-quad stacks.
-template <bool PER_SAMPLE, int G>
+template <bool PER_SAMPLE, int G, bool FS = false>
 __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderParams P,
                                                      TH_GpuCamera cam, float4* __restrict__ accum,
                                                      const float* __restrict__ srgb_lut,
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   const uint32_t idx = y * P.width + x;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
   if (active) {
-    uint32_t base = idx ^ (frame_of(S, P) * 9781u);
+    uint32_t base = idx ^ (frame_of<FS>(S, P) * 9781u);
     uint32_t s = th::pcg_hash(base);
     for (uint32_t smp = PER_SAMPLE ? j : 0u; smp < P.samples; smp += PER_SAMPLE ? G : 1u) {
       if (PER_SAMPLE) s = th::pcg_hash(base + smp * 0x9E3779B9u);
@@ -344,10 +344,10 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
   if (active && (lane % (4u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-    if (accumulated_of(S, P) == 0u) {
+    if (accumulated_of<FS>(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)accumulated_of(S, P);
+      float prev = (float)accumulated_of<FS>(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(256) k_render_quad(DevScene S, TH_GpuRenderPar
 // (samples j, j+G, ... per octet), combined by the same butterfly as k_render_quad.  A
 // wave covers 8/G pixels.  The reference-RNG variant without an environment is held to
 // 96 VGPRs (5 waves per SIMD), which it fits without spilling.
-template <bool ENV, bool PER_SAMPLE, int G>
+template <bool ENV, bool PER_SAMPLE, int G, bool FS = false>
 __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_octet(DevScene S, TH_GpuRenderParams P,
                                                       TH_GpuCamera cam, float4* __restrict__ accum,
                                                       const float* __restrict__ srgb_lut,
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_oct
   const uint32_t idx = y * P.width + x;
   OctetPath<ENV, 3, false> path{it, shadow};
   if (active) {
-    uint32_t base = idx ^ (frame_of(S, P) * 9781u);
+    uint32_t base = idx ^ (frame_of<FS>(S, P) * 9781u);
     uint32_t s = th::pcg_hash(base);
     for (uint32_t smp = PER_SAMPLE ? j : 0u; smp < P.samples; smp += PER_SAMPLE ? G : 1u) {
       if (PER_SAMPLE) s = th::pcg_hash(base + smp * 0x9E3779B9u);
@@ -410,10 +410,10 @@ __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_oct
   if (active && (lane % (8u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-    if (accumulated_of(S, P) == 0u) {
+    if (accumulated_of<FS>(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)accumulated_of(S, P);
+      float prev = (float)accumulated_of<FS>(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(256, (ENV || PER_SAMPLE) ? 4 : 5) k_render_oct
 #define TH_SPEC_CAND 1u
 #define TH_SPEC_IDLE 2u
 
-template <bool ENV, int G>
+template <bool ENV, int G, bool FS = false>
 __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRenderParams P, TH_GpuCamera cam,
                                                         float4* __restrict__ accum,
                                                         const float* __restrict__ srgb_lut, uint32_t cached,
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRender
         path.abandon();
       }
     };
-    assign(th::pcg_hash(idx ^ (frame_of(S, P) * 9781u)), false);
+    assign(th::pcg_hash(idx ^ (frame_of<FS>(S, P) * 9781u)), false);
     for (;;) {
       if (role != TH_SPEC_IDLE && !fin) {
         if (!begun) {
@@ -560,10 +560,10 @@ __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRender
   if (active && (lane % (8u * G)) == 0u) {
     float ns = (float)P.samples;
     float4 b = make_float4(pr / ns, pg / ns, pb / ns, pa / ns);
-    if (accumulated_of(S, P) == 0u) {
+    if (accumulated_of<FS>(S, P) == 0u) {
       accum[idx] = b;
     } else {
-      float prev = (float)accumulated_of(S, P);
+      float prev = (float)accumulated_of<FS>(S, P);
       float tot = prev + ns;
       float4 o = accum[idx];
       accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -582,6 +582,7 @@ __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRender
 // its slowest pixel.  Here every quad is persistent: when its pixel is done it takes the
 // next one (4x4-tile order, one atomic per wave for the quads that fetch together), so a
 // wave's time is its share of the frame.  Same per-pixel work and order as k_render_quad.
+template <bool FS = false>
 __global__ void __launch_bounds__(256) k_render_quad_dyn(DevScene S, TH_GpuRenderParams P,
                                                          TH_GpuCamera cam, float4* __restrict__ accum,
                                                          const float* __restrict__ srgb_lut,
@@ -612,7 +613,7 @@ __global__ void __launch_bounds__(256) k_render_quad_dyn(DevScene S, TH_GpuRende
     const uint32_t x = (tile % tiles_x) * 4u + (w & 3u), y = (tile / tiles_x) * 4u + (w >> 2);
     if (x >= P.width || y >= P.height) continue;
     const uint32_t idx = y * P.width + x;
-    uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
+    uint32_t s = th::pcg_hash(idx ^ (frame_of<FS>(S, P) * 9781u));
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;
     for (uint32_t smp = 0; smp < P.samples; ++smp) {
       v3 o, d;
@@ -626,10 +627,10 @@ __global__ void __launch_bounds__(256) k_render_quad_dyn(DevScene S, TH_GpuRende
     if (lq == 0u) {
       float ns = (float)P.samples;
       float4 b = make_float4(cr / ns, cg / ns, cb / ns, ca / ns);
-      if (accumulated_of(S, P) == 0u) {
+      if (accumulated_of<FS>(S, P) == 0u) {
         accum[idx] = b;
       } else {
-        float prev = (float)accumulated_of(S, P);
+        float prev = (float)accumulated_of<FS>(S, P);
         float tot = prev + ns;
         float4 o = accum[idx];
         accum[idx] = make_float4((o.x * prev + b.x * ns) / tot, (o.y * prev + b.y * ns) / tot,
@@ -654,7 +655,7 @@ __global__ void __launch_bounds__(64) k_primary_quad(DevScene S, TH_GpuRenderPar
   if (x >= P.width || y >= P.height) return;
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
   const uint32_t idx = y * P.width + x;
-  uint32_t s = th::pcg_hash(idx ^ (frame_of(S, P) * 9781u));
+  uint32_t s = th::pcg_hash(idx ^ (frame_of<false>(S, P) * 9781u));
   v3 o, d;
   camera_ray(cam, P, x, y, &s, &o, &d);
   float t;
@@ -1353,9 +1354,11 @@ void th_scene_destroy(th_scene* s) {
 // The kernel launches of one render on ctx->stream (th_render's dispatch by kernel,
 // traversal, schedule and random-stream mode); also what a progressive graph captures,
 // with S.frame_state set.
-static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
-                             const th_render_options& o, const DevScene& S, size_t lds, uint32_t block,
-                             unsigned long long* cnt) {
+extern "C++" {
+template <bool FS>
+static int32_t launch_render_t(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                               const th_render_options& o, const DevScene& S, size_t lds, uint32_t block,
+                               unsigned long long* cnt) {
   if (o.kernel == TH_KERNEL_WAVEFRONT && p->max_bounces > 0) {
     int32_t rc = th_wavefront_render(ctx, s, p, cam, &o);
     if (rc) return rc;
@@ -1371,18 +1374,18 @@ static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderPar
     uint32_t waves = ((p->width + tw - 1) / tw) * ((p->height + tht - 1) / tht);
     dim3 grid((waves + 3) / 4), blk(qblock);
 #define TH_LAUNCHQ(PS, GG) \
-  hipLaunchKernelGGL((k_render_quad<PS, GG>), grid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cached)
+  hipLaunchKernelGGL((k_render_quad<PS, GG, FS>), grid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cached)
     if (o.rng_mode == TH_RNG_REFERENCE && o.schedule == TH_SCHEDULE_DYNAMIC) {
       if (ctx->dyn_blocks_per_cu == 0) {
         int nb = 0;
-        TH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_quad_dyn, (int)qblock, lds4));
+        TH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_quad_dyn<FS>, (int)qblock, lds4));
         ctx->dyn_blocks_per_cu = std::max(nb, 1);
       }
       // persistent: as many blocks as stay resident, but no more quads than pixels
       uint32_t pblocks = (uint32_t)(std::max(ctx->cu_count, 1) * ctx->dyn_blocks_per_cu);
       pblocks = std::min<uint32_t>(pblocks, (p->width * p->height + 63u) / 64u);
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_work, 1u);
-      hipLaunchKernelGGL(k_render_quad_dyn, dim3(std::max(pblocks, 1u)), blk, lds4, ctx->stream, S, *p, *cam,
+      hipLaunchKernelGGL(k_render_quad_dyn<FS>, dim3(std::max(pblocks, 1u)), blk, lds4, ctx->stream, S, *p, *cam,
                          s->d_accum, ctx->d_srgb_lut, cached, ctx->d_work);
     } else if (o.rng_mode == TH_RNG_REFERENCE && o.schedule == TH_SCHEDULE_SPECULATIVE) {
       // two octets per pixel (one with a single sample); the draws per diffuse vertex:
@@ -1397,7 +1400,7 @@ static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderPar
       uint32_t swaves = ((p->width + stw - 1) / stw) * ((p->height + sth - 1) / sth);
       dim3 sgrid((swaves + 3u) / 4u);
 #define TH_LAUNCHS(E, GG)                                                                               \
-  hipLaunchKernelGGL((k_render_spec<E, GG>), sgrid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, \
+  hipLaunchKernelGGL((k_render_spec<E, GG, FS>), sgrid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, \
                      ctx->d_srgb_lut, cached, per_vertex)
       switch (GS * (env ? -1 : 1)) {
         case 2: TH_LAUNCHS(false, 2); break;
@@ -1415,7 +1418,7 @@ static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderPar
       dim3 ogrid((owaves + 3u) / 4u);
       bool env = p->background_kind == 2u;
 #define TH_LAUNCHO(E, PS, GG)                                                                       \
-  hipLaunchKernelGGL((k_render_octet<E, PS, GG>), ogrid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, \
+  hipLaunchKernelGGL((k_render_octet<E, PS, GG, FS>), ogrid, blk, lds4, ctx->stream, S, *p, *cam, s->d_accum, \
                      ctx->d_srgb_lut, cached)
       if (o.rng_mode == TH_RNG_REFERENCE) {
         if (env) TH_LAUNCHO(true, false, 1);
@@ -1449,7 +1452,7 @@ static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderPar
     uint32_t tiles = ((p->width + 7) / 8) * ((p->height + 7) / 8);
     dim3 grid(tiles), blk(block);
 #define TH_LAUNCH(TM, C) \
-  hipLaunchKernelGGL((k_render_pixel<TM, C>), grid, blk, lds, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cnt)
+  hipLaunchKernelGGL((k_render_pixel<TM, C, FS>), grid, blk, lds, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cnt)
     if (o.traversal == 0 && !o.collect_stats) TH_LAUNCH(0, false);
     else if (o.traversal == 0) TH_LAUNCH(0, true);
     else if (!o.collect_stats) TH_LAUNCH(1, false);
@@ -1463,7 +1466,7 @@ static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderPar
     uint32_t waves = ((p->width + tw - 1) / tw) * ((p->height + th_ - 1) / th_);
     dim3 grid(waves), blk(block);
 #define TH_LAUNCH(TM, C, GG) \
-  hipLaunchKernelGGL((k_render_sample<TM, C, GG>), grid, blk, lds, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cnt)
+  hipLaunchKernelGGL((k_render_sample<TM, C, GG, FS>), grid, blk, lds, ctx->stream, S, *p, *cam, s->d_accum, ctx->d_srgb_lut, cnt)
 #define TH_LAUNCH_G(TM, C)            \
   switch (G) {                        \
     case 8: TH_LAUNCH(TM, C, 8); break; \
@@ -1479,6 +1482,15 @@ static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderPar
 #undef TH_LAUNCH
   }
   return TH_OK;
+}
+
+}  // extern "C++"
+
+static int32_t launch_render(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
+                             const th_render_options& o, const DevScene& S, size_t lds, uint32_t block,
+                             unsigned long long* cnt) {
+  return S.frame_state != nullptr ? launch_render_t<true>(ctx, s, p, cam, o, S, lds, block, cnt)
+                                  : launch_render_t<false>(ctx, s, p, cam, o, S, lds, block, cnt);
 }
 
 // TH_SCHEDULE_AUTO (reference RNG, 4-wide megakernel): the first render of a scene with
