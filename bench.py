@@ -426,25 +426,36 @@ def equal_time_guided(reference_spp):
 def batched_leg(runs=3):
     """The batched workload: 16 scenes (128 - 262,144 triangles, 0 - 256 spheres, three
     resolutions) x 4 rigid-motion frames, 16 spp, 6 bounces, per-sample RNG, rendered by
-    one persistent launch (th_render_batch), beside the same 64 frames rendered one
-    th_render each."""
-    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
+    one persistent launch (th_render_batch) with the library's default launch shapes and
+    with the autotuner's (per scene: quads per pixel, LDS treelet, tile width, tuned on
+    its first frame; per batch: blocks per compute unit), beside the same 64 frames
+    rendered one th_render each."""
+    from toaster_amd import GpuContext, RenderOptions, _abi as A, scenes
     from toaster_amd.batch import BatchRenderer
-    frames = [f for per_scene in scenes.batched_workload() for f in per_scene]
+    workload = scenes.batched_workload()
     ctx = GpuContext(0)
     br = BatchRenderer(ctx)
-    for sc in frames:
-        br.add(sc)
+    for k, per_scene in enumerate(workload):
+        for sc in per_scene:
+            br.add(sc, group=k)
     o = RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=A.TH_TRAVERSE_WIDE)
-    br.render(o, use_tuning=False)
-    ctx.synchronize()
-    ctx.reset_stats()
-    t = time.perf_counter()
-    for _ in range(runs):
-        br.render(o, use_tuning=False)
-    ctx.synchronize()
-    dt = (time.perf_counter() - t) / runs
-    kms = ctx.stats()["total_kernel_ms"] / runs
+
+    def timed_batch(use_tuning):
+        br.render(o, use_tuning=use_tuning)
+        ctx.synchronize()
+        ctx.reset_stats()
+        t = time.perf_counter()
+        for _ in range(runs):
+            br.render(o, use_tuning=use_tuning)
+        ctx.synchronize()
+        return (time.perf_counter() - t) / runs, ctx.stats()["total_kernel_ms"] / runs
+
+    dt, kms = timed_batch(False)
+    t0 = time.perf_counter()
+    tuning = br.autotune(o)
+    bpc, _ = br.autotune_batch(o)
+    tune_s = time.perf_counter() - t0
+    dt_t, kms_t = timed_batch(True)
     # the same frames, one launch each
     for gs in br.scenes:
         gs.render(options=o, frame_index=0, accumulated_samples=0)
@@ -455,12 +466,17 @@ def batched_leg(runs=3):
     ctx.synchronize()
     dseq = time.perf_counter() - t
     paths = br.paths
+    chosen = sorted({(g, c, tw) for g, c, _, tw in tuning})
     br.close()
     ctx.close()
     return {"workload": "16 scenes x 4 rigid-motion frames, 128 - 262,144 triangles, 0 - 256 spheres, "
                         "320x180 / 640x360 / 1280x720, 16 spp, 6 bounces, per-sample RNG",
-            "frames": len(frames), "paths": paths, "launch_wall_ms": round(dt * 1e3, 2),
+            "frames": len(br.groups), "paths": paths, "launch_wall_ms": round(dt * 1e3, 2),
             "launch_kernel_ms": round(kms, 2), "gpaths_s": round(paths / dt / 1e9, 3),
+            "tuned": {"launch_wall_ms": round(dt_t * 1e3, 2), "launch_kernel_ms": round(kms_t, 2),
+                      "gpaths_s": round(paths / dt_t / 1e9, 3), "tuning_seconds": round(tune_s, 2),
+                      "blocks_per_cu": bpc,
+                      "shapes_quads_cached_tilewidth": [list(x) for x in chosen]},
             "one_launch_per_frame_wall_ms": round(dseq * 1e3, 2),
             "one_launch_per_frame_gpaths_s": round(paths / dseq / 1e9, 3)}
 

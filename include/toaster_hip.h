@@ -261,7 +261,9 @@ typedef struct th_render_options {
     uint32_t block_size;     /* 0 = autotuned / default                          */
     uint32_t collect_stats;  /* 1 = count rays / nodes / tests (slower launch)   */
     uint32_t schedule;       /* TH_SCHEDULE_*                                    */
-    uint32_t reserved[2];
+    uint32_t batch_blocks_per_cu; /* th_render_batch: persistent blocks per compute  */
+                                  /*   unit pulling from the work queue (0 = 4)     */
+    uint32_t reserved;
 } th_render_options;
 
 /* Pixel-to-lane assignment of the 4-lane (TH_TRAVERSE_WIDE) megakernel.
@@ -434,13 +436,17 @@ typedef struct th_batch_item {
     TH_GpuCamera camera;
     uint32_t quads_per_pixel; /* per-sample RNG: 1, 2, 4, 8, 16 quads share a pixel; 0 = default */
     uint32_t cached_nodes;    /* 4-wide nodes copied to LDS (<= 85); UINT32_MAX = default   */
+    uint32_t tile_width;      /* pixels per row of a wave's tile: a power of two dividing   */
+                              /*   16 / quads_per_pixel (the tile is that many wide and the */
+                              /*   rest high); 0 = default (4, 2 or 1)                      */
+    uint32_t reserved;
 } th_batch_item;
 
 typedef struct th_tuning {
     uint32_t quads_per_pixel;
     uint32_t cached_nodes;
-    float best_ms;
-    uint32_t reserved;
+    float best_ms;            /* median device time of the chosen shape                      */
+    uint32_t tile_width;
 } th_tuning;
 
 /* Renders every item in one persistent launch (work items handed out by an atomic
@@ -449,9 +455,17 @@ typedef struct th_tuning {
 int32_t th_render_batch(th_context* ctx, const th_batch_item* items, uint32_t count,
                         const th_render_options* options);
 /* Times the launch-shape candidates for one scene on the device and returns the
- * fastest (also remembered by the scene). */
+ * fastest (also remembered by the scene): quads per pixel (per-sample RNG), then the
+ * treelet nodes cached in LDS (0, 21, 85), then the wave's tile width, each chosen with
+ * the others held (coordinate descent).  Every candidate is launched once untimed (its
+ * graph is captured then) and timed as the median of three launches. */
 int32_t th_autotune(th_context* ctx, th_scene* scene, const TH_GpuRenderParams* params,
                     const TH_GpuCamera* camera, const th_render_options* options, th_tuning* out);
+/* Times a whole batch (items with their launch shapes) at 2, 4, 6 and 8 persistent blocks
+ * per compute unit and returns the fastest count (for options->batch_blocks_per_cu) and
+ * its median time; no counterpart in the reference. */
+int32_t th_autotune_batch(th_context* ctx, const th_batch_item* items, uint32_t count,
+                          const th_render_options* options, uint32_t* blocks_per_cu, float* best_ms);
 
 /* ---------------------------------------------------------- path guiding */
 /* Online-trained guiding distribution (no counterpart in the reference): an MLP

@@ -53,14 +53,53 @@ def test_batch_graph_replay_is_stable(gpu_ctx):
     br.close()
 
 
-def test_autotune_returns_valid_shape(gpu_ctx):
-    items = _small_batch()[:2]
+def test_autotuned_batch_matches_oracle(gpu_ctx):
+    """The autotuner's choices (quads per pixel, LDS treelet, tile width per scene; blocks
+    per compute unit for the batch) are launch shapes: the tuned batch renders every
+    scene as the oracle does with the tuned number of sample groups."""
+    items = _small_batch()
     br = BatchRenderer(gpu_ctx)
-    for sc in items:
-        br.add(sc)
-    tun = br.autotune(RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE))
-    for g, c, ms in tun:
-        assert g in (1, 2, 4, 8, 16) and c in (0, 21, 85) and ms > 0
-    br.render(RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE), use_tuning=True)
-    gpu_ctx.synchronize()
+    for k, sc in enumerate(items):
+        br.add(sc, group=k // 2)  # the two frames of each animation share a tuning
+    o = RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE)
+    print("case autotuned batch", flush=True)
+    tun = br.autotune(o)
+    n, ms = br.autotune_batch(o)
+    print(f"  tuning {tun} blocks/CU {n} ({ms:.3f} ms)")
+    assert n in (2, 4, 6, 8) and ms > 0
+    for (g, c, t_ms, tw), (g2, c2, _, tw2) in zip(tun[::2], tun[1::2]):
+        assert (g, c, tw) == (g2, c2, tw2)
+        assert g in (1, 2, 4, 8, 16) and c in (0, 21, 85) and t_ms > 0
+        assert tw in (1, 2, 4, 8, 16) and tw <= 16 // g
+    br.render(o, use_tuning=True)
+    for sc, gs, (g, _, _, _) in zip(items, br.scenes, tun):
+        ref, _ = oracle.OracleScene(sc).render(threads=8, rng_mode=A.TH_RNG_PER_SAMPLE, group=g)
+        img = gs.readback()
+        close = np.abs(img - ref) <= 1e-4 * (1.0 + np.abs(ref))
+        frac = float(np.mean(np.all(close, axis=-1)))
+        print(f"  {sc.name} G={g}: within tol {frac:.5f}")
+        assert frac >= 0.995
+    br.close()
+
+
+@pytest.mark.parametrize("g,tw", [(1, 1), (1, 16), (2, 8), (4, 1), (16, 1)])
+def test_tile_shapes_give_the_same_image(gpu_ctx, g, tw):
+    """A wave's tile shape only moves pixels between lanes: the image equals the default
+    shape's bit for bit."""
+    sc = _small_batch()[0]
+    br = BatchRenderer(gpu_ctx)
+    br.add(sc)
+    o = RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE)
+    imgs = []
+    for width in (0, tw):
+        arr = br.items(use_tuning=False)
+        arr[0].quads_per_pixel, arr[0].tile_width = g, width
+        br.render(o, items=arr)
+        imgs.append(br.scenes[0].readback().copy())
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+    arr = br.items(use_tuning=False)
+    arr[0].quads_per_pixel, arr[0].tile_width = g, 32 // g  # wider than 16 / g pixels
+    import ctypes as C
+    rc = A.lib().th_render_batch(gpu_ctx.handle, arr, 1, C.byref(o.struct()))
+    assert rc == A.TH_E_INVALID
     br.close()

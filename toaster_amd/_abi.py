@@ -130,17 +130,17 @@ class SceneDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("quads_per_pixel", u32), ("cached_nodes", u32), ("best_ms", f32), ("reserved", u32)]
+    _fields_ = [("quads_per_pixel", u32), ("cached_nodes", u32), ("best_ms", f32), ("tile_width", u32)]
 
 
 class BatchItem(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("params", GpuRenderParams), ("camera", GpuCamera),
-                ("quads_per_pixel", u32), ("cached_nodes", u32)]
+                ("quads_per_pixel", u32), ("cached_nodes", u32), ("tile_width", u32), ("reserved", u32)]
 
 
 class RenderOptions(C.Structure):
     _fields_ = [("rng_mode", u32), ("kernel", u32), ("traversal", u32), ("block_size", u32),
-                ("collect_stats", u32), ("schedule", u32), ("reserved", u32 * 2)]
+                ("collect_stats", u32), ("schedule", u32), ("batch_blocks_per_cu", u32), ("reserved", u32)]
 
 
 class Stats(C.Structure):
@@ -200,6 +200,8 @@ SIGNATURES = {
     "th_render_batch": (C.c_int32, [P, C.POINTER(BatchItem), u32, C.POINTER(RenderOptions)]),
     "th_autotune": (C.c_int32, [P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
                                 C.POINTER(RenderOptions), C.POINTER(Tuning)]),
+    "th_autotune_batch": (C.c_int32, [P, C.POINTER(BatchItem), u32, C.POINTER(RenderOptions), C.POINTER(u32),
+                                      C.POINTER(f32)]),
     "th_display_rgba8": (C.c_int32, [P, u32, u32, f32, P, C.c_size_t]),
     "th_display_rgba8_device": (C.c_int32, [P, P, f32, P, C.c_size_t]),
     "th_guide_create": (C.c_int32, [P, u32, C.POINTER(P)]),

@@ -166,9 +166,12 @@ static int32_t fill_entry(th_scene* s, const th_batch_item& it, uint32_t rng_mod
   e->per_sample = rng_mode == TH_RNG_PER_SAMPLE ? 1u : 0u;
   uint32_t cached = it.cached_nodes == 0xffffffffu ? BATCH_MAX_CACHED : it.cached_nodes;
   e->cached = std::min<uint32_t>(std::min<uint32_t>(cached, BATCH_MAX_CACHED), s->node4_count);
-  uint32_t ppw = 16u / G;
-  e->tw = ppw >= 8 ? 4 : (ppw >= 2 ? 2 : 1);
-  e->th = ppw / e->tw;
+  uint32_t ppw = 16u / G;  // pixels per wave
+  uint32_t tw = it.tile_width ? it.tile_width : (ppw >= 8 ? 4u : (ppw >= 2 ? 2u : 1u));
+  if ((tw & (tw - 1u)) != 0u || tw > ppw)
+    return th_set_error(TH_E_INVALID, "tile_width must be a power of two up to 16 / quads_per_pixel");
+  e->tw = tw;
+  e->th = ppw / tw;
   e->tiles_x = (it.params.width + e->tw - 1) / e->tw;
   uint32_t waves = e->tiles_x * ((it.params.height + e->th - 1) / e->th);
   e->items = (waves + 3) / 4;
@@ -184,6 +187,7 @@ extern "C" int32_t th_render_batch(th_context* ctx, const th_batch_item* items, 
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
   if (o.rng_mode > 1u) return th_set_error(TH_E_INVALID, "unknown rng mode");
+  if (o.batch_blocks_per_cu > 64u) return th_set_error(TH_E_INVALID, "batch_blocks_per_cu above 64");
   TH_CHECK(hipSetDevice(ctx->device));
   std::vector<BatchScene> table(count);
   uint32_t rows = 2;
@@ -226,7 +230,8 @@ extern "C" int32_t th_render_batch(th_context* ctx, const th_batch_item* items, 
                           ctx->stream));
   int cus = 0;
   TH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  uint32_t grid = std::min<uint32_t>(total, (uint32_t)std::max(cus, 1) * 4u);
+  const uint32_t per_cu = o.batch_blocks_per_cu ? o.batch_blocks_per_cu : 4u;
+  uint32_t grid = std::min<uint32_t>(total, (uint32_t)std::max(cus, 1) * per_cu);
   size_t lds = (size_t)BATCH_MAX_CACHED * 4 * sizeof(Child4) + (size_t)rows * (BATCH_BLOCK / 4) * sizeof(uint2);
   bool same = b->exec && b->g_nscenes == count && b->g_items == total && b->g_grid == grid &&
               b->g_rows == rows && b->g_lds == lds;
@@ -272,52 +277,110 @@ extern "C" int32_t th_render_batch(th_context* ctx, const th_batch_item* items, 
   return TH_OK;
 }
 
-// Times each candidate (quads per pixel x treelet size) alone on the device and keeps the
-// fastest; the result is also stored on the scene for later batches.
+// Median device time of a batch: one untimed launch first (a new launch shape captures
+// its graph then, on the host), then three timed launches.
+static int32_t time_batch(th_context* ctx, const th_batch_item* items, uint32_t count, const th_render_options& o,
+                          float* ms) {
+  int32_t rc = th_render_batch(ctx, items, count, &o);
+  if (rc) return rc;
+  if ((rc = th_synchronize(ctx))) return rc;
+  float t[3];
+  for (int r = 0; r < 3; ++r) {
+    if ((rc = th_render_batch(ctx, items, count, &o))) return rc;
+    th_stats st;
+    if ((rc = th_get_stats(ctx, &st))) return rc;  // waits for the launch
+    t[r] = (float)st.last_kernel_ms;
+  }
+  std::sort(t, t + 3);
+  *ms = t[1];
+  return TH_OK;
+}
+
+// Coordinate descent over the launch shape of one scene, each candidate alone on the
+// device: quads per pixel (per-sample RNG), then the treelet nodes cached in LDS, then
+// the wave's tile width; the result is also stored on the scene for later batches.
 extern "C" int32_t th_autotune(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p,
                                const TH_GpuCamera* cam, const th_render_options* opt, th_tuning* out) {
   if (!ctx || !s || !p || !cam || !out) return th_set_error(TH_E_INVALID, "null argument");
   th_render_options o;
   memset(&o, 0, sizeof o);
   if (opt) o = *opt;
-  std::vector<uint32_t> gs;
-  if (o.rng_mode == TH_RNG_REFERENCE) {
-    gs.push_back(1);
-  } else {
-    for (uint32_t g = 1; g <= 16 && g <= std::max(1u, p->samples); g <<= 1) gs.push_back(g);
-  }
-  const uint32_t caches[3] = {0u, 21u, 85u};
-  float best = 3.4e38f;
-  th_tuning bt;
-  memset(&bt, 0, sizeof bt);
   int32_t rc = th_synchronize(ctx);
   if (rc) return rc;
   th_stats saved = ctx->stats;
-  for (uint32_t g : gs)
-    for (uint32_t c : caches) {
-      th_batch_item it;
-      memset(&it, 0, sizeof it);
-      it.scene = s;
-      it.params = *p;
-      it.camera = *cam;
-      it.quads_per_pixel = g;
-      it.cached_nodes = c;
-      if ((rc = th_reset_stats(ctx))) return rc;
-      for (int r = 0; r < 3; ++r)
-        if ((rc = th_render_batch(ctx, &it, 1, &o))) return rc;
-      th_stats st;
-      if ((rc = th_get_stats(ctx, &st))) return rc;
-      // first launch of a new shape includes graph capture on the host, not device time
-      float ms = (float)(st.total_kernel_ms / std::max<uint64_t>(1, st.launches));
-      if (ms < best) {
-        best = ms;
-        bt.quads_per_pixel = g;
-        bt.cached_nodes = c;
-        bt.best_ms = ms;
-      }
+  th_batch_item it;
+  memset(&it, 0, sizeof it);
+  it.scene = s;
+  it.params = *p;
+  it.camera = *cam;
+  it.quads_per_pixel = default_G(o.rng_mode, p->samples);
+  it.cached_nodes = BATCH_MAX_CACHED;
+  it.tile_width = 0;
+  float best = 3.4e38f;
+  auto consider = [&](const th_batch_item& c) -> int32_t {
+    float ms = 0.0f;
+    int32_t r = time_batch(ctx, &c, 1, o, &ms);
+    if (r) return r;
+    if (ms < best) {
+      best = ms;
+      it = c;
     }
+    return TH_OK;
+  };
+  if ((rc = consider(it))) return rc;
+  if (o.rng_mode == TH_RNG_PER_SAMPLE)
+    for (uint32_t g = 1; g <= 16 && g <= std::max(1u, p->samples); g <<= 1) {
+      th_batch_item c = it;
+      c.quads_per_pixel = g;
+      c.tile_width = 0;
+      if (g != it.quads_per_pixel && (rc = consider(c))) return rc;
+    }
+  for (uint32_t cached : {0u, 21u, 85u}) {
+    th_batch_item c = it;
+    c.cached_nodes = cached;
+    if (cached != it.cached_nodes && (rc = consider(c))) return rc;
+  }
+  const uint32_t ppw = 16u / it.quads_per_pixel;
+  const uint32_t tw0 = it.tile_width ? it.tile_width : (ppw >= 8 ? 4u : (ppw >= 2 ? 2u : 1u));
+  for (uint32_t tw = 1; tw <= ppw; tw <<= 1) {
+    th_batch_item c = it;
+    c.tile_width = tw;
+    if (tw != tw0 && (rc = consider(c))) return rc;
+  }
   ctx->stats = saved;
+  th_tuning bt;
+  bt.quads_per_pixel = it.quads_per_pixel;
+  bt.cached_nodes = it.cached_nodes;
+  bt.best_ms = best;
+  bt.tile_width = it.tile_width ? it.tile_width : tw0;
   s->tuning = bt;
   *out = bt;
+  return TH_OK;
+}
+
+extern "C" int32_t th_autotune_batch(th_context* ctx, const th_batch_item* items, uint32_t count,
+                                     const th_render_options* opt, uint32_t* blocks_per_cu, float* best_ms) {
+  if (!ctx || !items || count == 0 || !blocks_per_cu || !best_ms)
+    return th_set_error(TH_E_INVALID, "null argument or empty batch");
+  th_render_options o;
+  memset(&o, 0, sizeof o);
+  if (opt) o = *opt;
+  int32_t rc = th_synchronize(ctx);
+  if (rc) return rc;
+  th_stats saved = ctx->stats;
+  float best = 3.4e38f;
+  uint32_t best_n = 4;
+  for (uint32_t n : {2u, 4u, 6u, 8u}) {
+    o.batch_blocks_per_cu = n;
+    float ms = 0.0f;
+    if ((rc = time_batch(ctx, items, count, o, &ms))) return rc;
+    if (ms < best) {
+      best = ms;
+      best_n = n;
+    }
+  }
+  ctx->stats = saved;
+  *blocks_per_cu = best_n;
+  *best_ms = best;
   return TH_OK;
 }

@@ -24,12 +24,14 @@ class RenderOptions:
     traversal: int = A.TH_TRAVERSE_REFERENCE
     collect_stats: bool = False
     schedule: int = A.TH_SCHEDULE_STATIC
+    batch_blocks_per_cu: int = 0  # th_render_batch workers per compute unit (0 = 4)
 
     def struct(self):
         o = A.RenderOptions()
         o.rng_mode, o.kernel, o.traversal = self.rng_mode, self.kernel, self.traversal
         o.collect_stats = 1 if self.collect_stats else 0
         o.schedule = self.schedule
+        o.batch_blocks_per_cu = self.batch_blocks_per_cu
         return o
 
 
