@@ -495,6 +495,39 @@ def copy_calibration(mib=1024, copies=10):
     print(f"copy calibration: {copies} copies of {n} bytes ({mib * 1024} KiB each)", file=sys.stderr)
 
 
+def profile_plain_wavefront(passes):
+    """The plain estimator as the equal-time leg's fastest plain kernel runs it (the 1-spp
+    guided wavefront at alpha 0 with the environment sample, no network) on the terrain
+    workload, for a profiler.  First one counting render of the same rays (th_render,
+    per-sample streams, one sample, frame seed 0, nearer-first scalar traversal; the
+    alpha-0 wavefront equals it bit for bit), whose closest-ray counters give the
+    reference-layout bytes the extend kernel's rays read per pass; printed on stderr."""
+    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
+    from toaster_amd.guide import Guide, render_guided
+    ctx = GpuContext(0)
+    sc = scenes.terrain(1280, 720, samples=1, max_bounces=8)
+    gs = GpuScene.from_scene(ctx, sc)
+    gs.render(samples=1, frame_index=0, accumulated_samples=0,
+              options=RenderOptions(rng_mode=A.TH_RNG_PER_SAMPLE, traversal=A.TH_TRAVERSE_ORDERED,
+                                    collect_stats=True))
+    c = ctx.stats()
+    trav = (48 * c["closest_nodes_visited"] + 8 * c["closest_primref_tests"] + 64 * c["closest_triangle_tests"]
+            + 32 * c["closest_sphere_tests"])
+    io = 40 * c["closest_rays"]  # the wavefront's ray state: origin + direction read, hit record written
+    g = Guide(ctx, seed=1)
+    t = time.perf_counter()
+    render_guided(gs, g, passes, alpha=0.0, train=False, progressive=False, warmup_steps=0)
+    ctx.synchronize()
+    dt = time.perf_counter() - t
+    print(json.dumps({"passes": passes, "seconds": round(dt, 4), "closest_rays_per_pass": c["closest_rays"],
+                      "closest_nodes_per_pass": c["closest_nodes_visited"],
+                      "extend_traversal_bytes_per_pass": trav, "extend_ray_io_bytes_per_pass": io}),
+          file=sys.stderr)
+    g.close()
+    gs.close()
+    ctx.close()
+
+
 def profile_guided(passes):
     """The guided tracer alone, as the equal-time leg runs it (fresh guide, default
     settings, one sample per pixel per pass) on the terrain workload."""
@@ -531,6 +564,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 10 times (a copy kernel of known size for the "
                          "profiler's FETCH_SIZE / WRITE_SIZE calibration); prints no JSON line")
+    ap.add_argument("--profile-plain-wavefront", type=int, default=0, metavar="PASSES",
+                    help="only render PASSES plain 1-spp wavefront passes on the terrain workload (for a "
+                         "profiler), after one counting render of the same rays; prints no JSON line")
     ap.add_argument("--profile-guided", type=int, default=0, metavar="PASSES",
                     help="only render PASSES guided passes on the terrain workload (for a profiler); "
                          "prints no JSON line")
@@ -540,6 +576,9 @@ def main():
         return
     if args.profile_guided:
         profile_guided(args.profile_guided)
+        return
+    if args.profile_plain_wavefront:
+        profile_plain_wavefront(args.profile_plain_wavefront)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported",

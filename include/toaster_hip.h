@@ -308,6 +308,10 @@ typedef struct th_stats {
     uint64_t lights_scanned;  /* light records read by emitter selection          */
     uint64_t env_lookups;     /* bilinear environment evaluations                 */
     uint64_t env_samples;     /* environment CDF searches                         */
+    uint64_t closest_nodes_visited;  /* the node / primitive counts above for the  */
+    uint64_t closest_primref_tests;  /*   closest-hit rays alone (the rest are     */
+    uint64_t closest_triangle_tests; /*   shadow rays)                             */
+    uint64_t closest_sphere_tests;
 } th_stats;
 
 int32_t th_device_count(int32_t* count);

@@ -255,6 +255,26 @@ def test_display_device_matches_host(gpu_ctx):
     gs.close()
     assert np.array_equal(dev, display_rgba8(img, 0.5))
     assert np.array_equal(dev2, display_rgba8(img2, -1.0))
+    # and against the oracle's restatement of color.rs / image_output.rs directly
+    assert np.array_equal(dev, oracle.display_rgba8(img, 0.5))
+    assert np.array_equal(dev2, oracle.display_rgba8(img2, -1.0))
+
+
+def test_display_device_matches_oracle_on_extreme_values(gpu_ctx):
+    """k_display against the oracle's display transform on an accumulation holding
+    exact zeros, negatives, the sRGB and ACES knees, large values, infinities and NaN
+    (color.rs:61-74 rules), uploaded as a progressive checkpoint."""
+    sc = scenes.cornell(width=32, height=16)
+    gs = GpuScene.from_scene(gpu_ctx, sc)
+    rng = np.random.default_rng(7)
+    vals = np.concatenate([np.array([0.0, -0.0, -1.0, 1e-9, 0.0031308, 0.04045, 0.18, 0.5, 1.0, 2.0, 16.0, 1e6,
+                                     3.4e38, np.inf, -np.inf, np.nan], np.float32),
+                           rng.exponential(0.5, 32 * 16 * 4 - 16).astype(np.float32)])
+    acc = vals.reshape(16, 32, 4)
+    gs.upload_accumulation(acc, sc.samples, 1)
+    for stops in (0.0, 1.5, -3.0):
+        assert np.array_equal(gs.display_rgba8(stops), oracle.display_rgba8(acc, stops))
+    gs.close()
 
 
 def test_converged_per_sample_matches_oracle_reference_stream(gpu_ctx):

@@ -148,7 +148,9 @@ class Stats(C.Structure):
                 ("launches", C.c_uint64)] + [
         (n, C.c_uint64) for n in ("paths", "rays", "closest_rays", "shadow_rays", "nodes_visited",
                                   "primref_tests", "triangle_tests", "sphere_tests", "closest_hits",
-                                  "triangle_hits", "lights_scanned", "env_lookups", "env_samples")]
+                                  "triangle_hits", "lights_scanned", "env_lookups", "env_samples",
+                                  "closest_nodes_visited", "closest_primref_tests", "closest_triangle_tests",
+                                  "closest_sphere_tests")]
 
 
 # name -> (restype, argtypes) for every entry point declared in include/toaster_hip.h

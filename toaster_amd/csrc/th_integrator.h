@@ -15,24 +15,30 @@ template <bool COUNT>
 struct LaneCounters {
   uint32_t paths = 0, closest_rays = 0, shadow_rays = 0, nodes = 0, primrefs = 0, tris = 0,
            spheres = 0, closest_hits = 0, tri_hits = 0, lights_scanned = 0, env_lookups = 0,
-           env_samples = 0;
+           env_samples = 0, c_nodes = 0, c_primrefs = 0, c_tris = 0, c_spheres = 0;
   __device__ __forceinline__ void add_ray(const RayStats<COUNT>& r, bool shadow) {
     if (!COUNT) return;
     nodes += r.nodes;
     primrefs += r.primrefs;
     tris += r.tris;
     spheres += r.spheres;
-    if (shadow)
+    if (shadow) {
       shadow_rays++;
-    else
+    } else {  // closest-hit rays alone as well
       closest_rays++;
+      c_nodes += r.nodes;
+      c_primrefs += r.primrefs;
+      c_tris += r.tris;
+      c_spheres += r.spheres;
+    }
   }
   __device__ __forceinline__ void flush(unsigned long long* g) {
     if (!COUNT) return;
-    uint32_t v[12] = {paths, closest_rays, shadow_rays, nodes, primrefs, tris, spheres,
-                      closest_hits, tri_hits, lights_scanned, env_lookups, env_samples};
+    uint32_t v[16] = {paths, closest_rays, shadow_rays, nodes, primrefs, tris, spheres,
+                      closest_hits, tri_hits, lights_scanned, env_lookups, env_samples,
+                      c_nodes, c_primrefs, c_tris, c_spheres};
 #pragma unroll
-    for (int i = 0; i < 12; ++i)
+    for (int i = 0; i < 16; ++i)
       if (v[i]) atomicAdd(g + i, (unsigned long long)v[i]);
   }
 };

@@ -1779,6 +1779,10 @@ int32_t th_get_stats(const th_context* cctx, th_stats* out) {
     s.lights_scanned = h[9];
     s.env_lookups = h[10];
     s.env_samples = h[11];
+    s.closest_nodes_visited = h[12];
+    s.closest_primref_tests = h[13];
+    s.closest_triangle_tests = h[14];
+    s.closest_sphere_tests = h[15];
     ctx->counters_pending = false;
   }
   *out = ctx->stats;
