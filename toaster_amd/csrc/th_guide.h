@@ -30,13 +30,32 @@ namespace tg {
 using th::v3;
 
 // ------------------------------------------------------------------ bf16
-TH_HD uint16_t f2bf(float f) {  // round to nearest even (finite inputs)
+// Round to nearest even.  The device uses the gfx950 conversion instruction
+// (v_cvt_pk_bf16_f32), which gives the same result as the integer rounding below for every
+// non-NaN input, denormals and infinities included (compared over all 2^32 patterns on an
+// MI355X); a NaN stays a NaN there.
+TH_HD uint16_t f2bf(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
+#else
   union { float f; uint32_t u; } v;
   v.f = f;
   uint32_t u = v.u;
   if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+#endif
+}
+// Two values packed low | high << 16 (one conversion instruction on the device).
+TH_HD uint32_t f2bf2(float lo, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  typedef float f32x2_t __attribute__((ext_vector_type(2)));
+  const f32x2_t v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+#else
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+#endif
 }
 TH_HD float bf2f(uint16_t b) {
   union { float f; uint32_t u; } v;

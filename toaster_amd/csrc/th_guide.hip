@@ -86,8 +86,8 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const uint16_t* __restrict__ 
 
 // Loss gradient of one batch: L = -sum_b w_b log q_b(dir_b) / sum_b w_b.  The weight sum
 // is reduced in a fixed order (256 block partials here, then the same 256-to-1 tree in
-// every block of the training kernel), so gradients are reproducible call to call.
-// Block 0 also clears the step's scalars (sums[1]: the loss accumulator).
+// every wave of the training kernel), so gradients are reproducible call to call.
+// Block 0 also clears the step's scalars (sums[1] later receives the loss from k_adam).
 __global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict__ w, uint32_t n,
                                                         float* __restrict__ part, float* __restrict__ sums) {
   __shared__ float red[256];
@@ -103,181 +103,267 @@ __global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// The batch's total weight from the 256 partials (fixed-order tree; `red` holds 256
-// floats of LDS).  Every thread of the block gets it.
-__device__ __forceinline__ float weight_total(const float* __restrict__ part, float* red) {
-  red[threadIdx.x] = part[threadIdx.x];
-  __syncthreads();
-  for (uint32_t off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  const float t = red[0];
-  __syncthreads();
-  return t;
+// The batch's total weight from the 256 partials, computed by every wave on its own: the
+// fixed-order tree p[t] + p[t + 128], then + [t + 64], + [t + 32], ... down to one value
+// (lane t pairs with lane t ^ off, which is t + off where the tree reads it).
+__device__ __forceinline__ float weight_total(const float* __restrict__ part) {
+  const uint32_t t = threadIdx.x & 63u;
+  float v = (part[t] + part[t + 128u]) + (part[t + 64u] + part[t + 192u]);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return __shfl(v, 0, 64);
 }
 
-// Fused training step.  One block (4 waves) walks `rows_per` consecutive samples in
-// chunks of 64 (16 per wave) with every activation in LDS: forward through the four
-// layers (A fragments from LDS tiles, B fragments = out-major weights from L2), the
-// vMF-mixture loss gradient with 4 lanes per sample (2 lobes each, quad reductions), then
-// layer by layer the chunk's weight gradient (wave w owns output rows 16w..16w+15 of
-// every layer's dW, accumulated across chunks in registers; the MFMA K dimension runs
-// over 32 samples) and the gated back-propagation with the in-major weights.  Each block
-// writes one partial gradient per layer for the fixed-order reduction and Adam.
+// Fused training step.  One block of 8 waves walks `rows_per` consecutive samples in
+// chunks of 64 with every activation in LDS; two waves per SIMD hide each other's LDS
+// and MFMA latency.  Wave w = 4h + v owns the 16 features 16v..16v+15 of every layer and
+// half h of the chunk: in the forward it computes those output columns for samples
+// 32h..32h+31 (A fragments from the row-major LDS tile, B fragments = its 16 rows of the
+// out-major weights, loaded once into registers); the vMF-mixture loss gradient runs on
+// 8 lanes per sample, one lobe each; then the wave accumulates rows 16v..16v+15, columns
+// 32h..32h+31 of every layer's weight gradient (across chunks, in registers; the MFMA K
+// dimension runs over 32 samples) and back-propagates the columns 16v..16v+15, samples
+// 32h..32h+31 of the layer below (its 16 rows of the in-major weights, also in registers),
+// gated by the ReLU of the layer's input.  Each block writes one partial gradient per
+// layer for the fixed-order reduction and Adam.  Each output element is one MFMA chain
+// over K = 0..31 then 32..63, whichever wave computes it.
 #define TF_LD 72  // LDS row stride (BF16 elements) of a 64-wide activation tile
+// The weight gradient's MFMA runs its K dimension over samples, so both its operands are
+// read from transposed copies of the tiles ([feature][sample], same stride): one 16-byte
+// LDS load per operand instead of eight 2-byte column reads.  The forward and the
+// back-propagation write each output tile in both layouts (an MFMA output lane holds four
+// consecutive rows of one column: one 8-byte store to the transposed copy).
+__device__ __forceinline__ void st4t(uint16_t* t, uint32_t col, uint32_t row0, const uint16_t v[4]) {
+  *(uint2*)(t + col * TF_LD + row0) =
+      make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+}
 __device__ __forceinline__ tg_bf16x8 lds8(const uint16_t* p) {
   return __builtin_bit_cast(tg_bf16x8, *(const uint4*)p);
 }
-__device__ __forceinline__ float quad_sum(float v) {
-  return (qbcast_f<0>(v) + qbcast_f<1>(v)) + (qbcast_f<2>(v) + qbcast_f<3>(v));
+// Sums (and maxima) over the 8 lanes of a sample, lane k holding lobe k: the pairs
+// (2p, 2p + 1) first, then (p0 + p1) + (p2 + p3) -- the order of a 4-lane quad sum over
+// lanes that hold two lobes each.  DPP exchanges: lane ^ 1, lane ^ 2, then the other quad
+// of the octet (each partial is uniform over its pair, then over its quad; + and max are
+// commutative, so every lane ends with the same value).
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
-__device__ __forceinline__ float quad_max(float v) {
-  return th::fmax_(th::fmax_(qbcast_f<0>(v), qbcast_f<1>(v)), th::fmax_(qbcast_f<2>(v), qbcast_f<3>(v)));
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float oct_sum(float v) {
+  const float s = v + dpp_xor1(v);
+  const float t = s + dpp_xor2(s);
+  return t + oswap_f(t);
+}
+__device__ __forceinline__ float oct_max(float v) {
+  const float m = th::fmax_(v, dpp_xor1(v));
+  const float t = th::fmax_(m, dpp_xor2(m));
+  return th::fmax_(t, oswap_f(t));
 }
 
-__global__ void __launch_bounds__(256) k_train_fused(const uint16_t* __restrict__ X,
-                                                     const float* __restrict__ dirs,
-                                                     const float* __restrict__ w, uint32_t count,
-                                                     uint32_t rows_per, const float* __restrict__ wpart,
-                                                     const uint16_t* __restrict__ woi,
-                                                     const uint16_t* __restrict__ wio,
-                                                     const float* __restrict__ bias,
-                                                     float* __restrict__ partials, float* __restrict__ loss) {
+#define TF_THREADS 512
+__global__ void __launch_bounds__(TF_THREADS) k_train_fused(const uint16_t* __restrict__ X,
+                                                            const float* __restrict__ dirs,
+                                                            const float* __restrict__ w, uint32_t count,
+                                                            uint32_t rows_per, const float* __restrict__ wpart,
+                                                            const uint16_t* __restrict__ woi,
+                                                            const uint16_t* __restrict__ wio,
+                                                            const float* __restrict__ bias,
+                                                            float* __restrict__ partials,
+                                                            float* __restrict__ loss_part) {
   __shared__ __align__(16) uint16_t tX[64 * TF_LD], tH1[64 * TF_LD], tH2[64 * TF_LD], tH3[64 * TF_LD];
   __shared__ __align__(16) uint16_t tDa[64 * TF_LD], tDb[64 * TF_LD];
+  __shared__ __align__(16) uint16_t uX[64 * TF_LD], uH1[64 * TF_LD], uH2[64 * TF_LD], uH3[64 * TF_LD];
+  __shared__ __align__(16) uint16_t uDa[64 * TF_LD], uDb[64 * TF_LD];  // transposed copies
   __shared__ __align__(16) float tO[64 * TG_OUT];
-  const uint32_t tid = threadIdx.x, wv = tid >> 6, l = tid & 63u, r = l & 15u, g = l >> 4;
+  __shared__ float lred[TF_THREADS / 64];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, wv = wave & 3u, h = wave >> 2;
+  const uint32_t l = tid & 63u, r = l & 15u, g = l >> 4;
   const uint32_t r0 = blockIdx.x * rows_per, r1 = min(r0 + rows_per, count);
-  const float S = weight_total(wpart, (float*)tDa);  // tDa is free until the backward pass
+  const uint32_t col = 16u * wv + r;  // this lane's feature column in the forward / dZ passes
+  const uint32_t rr = tid >> 3, lobe = tid & 7u;  // the loss gradient's sample and lobe
+  // global loads first, in the order they are consumed: the weight partial, this wave's
+  // weight rows, the first chunk's inputs
+  tg_bf16x8 wf[4][2], wb[3][2];
+#pragma unroll
+  for (int L = 0; L < 4; ++L) {
+    wf[L][0] = ld8(woi + L * 4096 + col * 64u + g * 8u);
+    wf[L][1] = ld8(woi + L * 4096 + col * 64u + 32u + g * 8u);
+  }
+#pragma unroll
+  for (int L = 1; L < 4; ++L) {
+    wb[L - 1][0] = ld8(wio + L * 4096 + col * 64u + g * 8u);
+    wb[L - 1][1] = ld8(wio + L * 4096 + col * 64u + 32u + g * 8u);
+  }
+  float bcol[4];
+#pragma unroll
+  for (int L = 0; L < 4; ++L) bcol[L] = bias[L * 64 + col];
+  auto load_x = [&](uint32_t c0) {
+    const uint32_t gr = c0 + (tid >> 3);
+    return gr < r1 ? ((const uint4*)(X + (size_t)gr * 64u))[tid & 7u] : make_uint4(0, 0, 0, 0);
+  };
+  auto load_dw = [&](uint32_t c0, float dw[4]) {
+    const uint32_t gr = c0 + rr;
+    if (gr < r1) {
+      dw[0] = dirs[3u * gr];
+      dw[1] = dirs[3u * gr + 1u];
+      dw[2] = dirs[3u * gr + 2u];
+      dw[3] = w[gr];
+    } else {
+      dw[0] = dw[1] = dw[3] = 0.0f;
+      dw[2] = 1.0f;
+    }
+  };
+  uint4 xv = load_x(r0);
+  float dw[4];
+  load_dw(r0, dw);
+  const float S = weight_total(wpart);
   const uint16_t* hin[4] = {tX, tH1, tH2, tH3};
-  tg_f32x4 acc[4][4];
+  uint16_t* hinT[4] = {uX, uH1, uH2, uH3};
+  tg_f32x4 acc[4][2];  // dW rows 16 wv .., columns 16 (2h + kk) ..
 #pragma unroll
   for (int L = 0; L < 4; ++L)
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc[L][kb] = tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  float dbias[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // column 16 wv + (l & 15), rows (l >> 4) * 16 ...
+    for (int kk = 0; kk < 2; ++kk) acc[L][kk] = tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float dbias[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // waves 0..3: column 16 wv + (l & 15), rows (l >> 4) * 16 ...
   float lsum = 0.0f;
   for (uint32_t c0 = r0; c0 < r1; c0 += 64u) {  // uniform per block
-    for (uint32_t i = tid; i < 64u * 8u; i += 256u) {
-      const uint32_t row = i >> 3, seg = i & 7u, gr = c0 + row;
-      uint4 v = gr < r1 ? ((const uint4*)(X + (size_t)gr * 64u))[seg] : make_uint4(0, 0, 0, 0);
-      *(uint4*)(tX + row * TF_LD + seg * 8u) = v;
+    {
+      const uint32_t row = tid >> 3, seg = tid & 7u;
+      *(uint4*)(tX + row * TF_LD + seg * 8u) = xv;
+      const uint32_t w4[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        uX[(seg * 8u + 2u * j) * TF_LD + row] = (uint16_t)(w4[j] & 0xffffu);
+        uX[(seg * 8u + 2u * j + 1u) * TF_LD + row] = (uint16_t)(w4[j] >> 16);
+      }
+    }
+    const float dcur_w[4] = {dw[0], dw[1], dw[2], dw[3]};
+    if (c0 + 64u < r1) {  // the next chunk's inputs are in flight during this one
+      xv = load_x(c0 + 64u);
+      load_dw(c0 + 64u, dw);
     }
     __syncthreads();
-    // ---- forward: wave wv owns rows 16 wv .. 16 wv + 15 of the chunk
-    const uint32_t arow = (16u * wv + r) * TF_LD;
+    // ---- forward: columns 16 wv .. 16 wv + 15 of every layer, samples 32 h .. 32 h + 31
 #pragma unroll
     for (int L = 0; L < 4; ++L) {
-      const uint16_t* in = hin[L];
-      const tg_bf16x8 a0 = lds8(in + arow + g * 8u), a1 = lds8(in + arow + 32u + g * 8u);
+      if (!(L == 3 && wv == 3u)) {  // the last layer has 40 outputs
+        const uint16_t* in = hin[L];
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        if (L == 3 && nb == 3) continue;  // the last layer has 40 outputs
-        const uint16_t* wr = woi + L * 4096 + (nb * 16 + r) * 64;
-        tg_f32x4 c = mfma(a0, ld8(wr + g * 8u), tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-        c = mfma(a1, ld8(wr + 32u + g * 8u), c);
-        const uint32_t col = nb * 16 + r;
-        const float bv = bias[L * 64 + col];
+        for (int mm = 0; mm < 2; ++mm) {
+          const uint32_t mb = 2u * h + mm;
+          const uint32_t arow = (16u * mb + r) * TF_LD;
+          tg_f32x4 c = mfma(lds8(in + arow + g * 8u), wf[L][0], tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+          c = mfma(lds8(in + arow + 32u + g * 8u), wf[L][1], c);
+          uint16_t hv[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t row = 16u * wv + 4u * g + i;
-          const float v = c[i] + bv;
-          if (L < 3)
-            ((uint16_t*)hin[L + 1])[row * TF_LD + col] = tg::f2bf(v > 0.0f ? v : 0.0f);
-          else if (col < (uint32_t)TG_OUT)
-            tO[row * TG_OUT + col] = v;
+          for (int i = 0; i < 4; ++i) {
+            const uint32_t row = 16u * mb + 4u * g + i;
+            const float v = c[i] + bcol[L];
+            hv[i] = tg::f2bf(v > 0.0f ? v : 0.0f);
+            if (L < 3)
+              ((uint16_t*)hin[L + 1])[row * TF_LD + col] = hv[i];
+            else if (col < (uint32_t)TG_OUT)
+              tO[row * TG_OUT + col] = v;
+          }
+          if (L < 3) st4t(hinT[L + 1], col, 16u * mb + 4u * g, hv);
         }
       }
       __syncthreads();
     }
-    // ---- loss gradient: 4 lanes per sample, lobes 2p and 2p + 1
+    // ---- loss gradient: 8 lanes per sample, lane k holds lobe k
     {
-      const uint32_t rr = 16u * wv + (l >> 2), p = l & 3u, gr = c0 + rr;
+      const uint32_t gr = c0 + rr;
       const bool live = gr < r1 && S > 0.0f;
       const float* o = tO + rr * TG_OUT;
-      const int k0 = 2 * (int)p;
-      const float lg0 = o[32 + k0], lg1 = o[33 + k0];
-      const float mx = quad_max(th::fmax_(lg0, lg1));
-      const float e0 = tg::fexp(lg0 - mx), e1 = tg::fexp(lg1 - mx);
-      const float esum = quad_sum(e0 + e1);
-      const float wk[2] = {e0 / esum, e1 / esum};
-      const v3 om = live ? th::mk(dirs[3u * gr], dirs[3u * gr + 1u], dirs[3u * gr + 2u]) : th::mk(0.0f, 0.0f, 1.0f);
-      v3 mu[2];
-      float len[2], kap[2], f[2];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int k = k0 + j;
-        const v3 rv = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
-        len[j] = th::fsqrt(th::dot(rv, rv) + 1e-8f);
-        mu[j] = th::divs(rv, len[j]);
-        kap[j] = th::fmin_(tg::softplus(o[24 + k]) + 0.01f, 1000.0f);
-        f[j] = tg::vmf_pdf(mu[j], kap[j], om);
-      }
-      const float q = th::fmax_(quad_sum(wk[0] * f[0] + wk[1] * f[1]), 1e-30f);
-      const float scale = live ? -w[gr] / S : 0.0f;
-      if (p == 0u && live) lsum += scale * tg::flog(q);
+      const int k = (int)lobe;
+      const float lg = o[32 + k];
+      const float mx = oct_max(lg);
+      const float e = tg::fexp(lg - mx);
+      const float esum = oct_sum(e);
+      const float wk = e / esum;
+      const v3 om = live ? th::mk(dcur_w[0], dcur_w[1], dcur_w[2]) : th::mk(0.0f, 0.0f, 1.0f);
+      const v3 rv = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
+      const float len = th::fsqrt(th::dot(rv, rv) + 1e-8f);
+      const v3 mu = th::divs(rv, len);
+      const float kap = th::fmin_(tg::softplus(o[24 + k]) + 0.01f, 1000.0f);
+      const float f = tg::vmf_pdf(mu, kap, om);
+      const float q = th::fmax_(oct_sum(wk * f), 1e-30f);
+      const float scale = live ? -dcur_w[3] / S : 0.0f;
+      if (lobe == 0u && live) lsum += scale * tg::flog(q);
       uint16_t* dz = tDa + rr * TF_LD;
+      auto put = [&](uint32_t c, uint16_t v) {  // both layouts of the output gradient
+        dz[c] = v;
+        uDa[c * TF_LD + rr] = v;
+      };
+      const float gk = wk * f / q;
+      const float c = th::dot(mu, om);
+      const v3 dr = th::muls(th::sub(om, th::muls(mu, c)), gk * kap / len);
+      const float e2 = tg::fexp(-2.0f * kap);
+      const float dk = gk * (1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c - 1.0f));
+      const float sig = kap >= 1000.0f ? 0.0f : 1.0f / (1.0f + tg::fexp(-o[24 + k]));
+      put(3 * k, tg::f2bf(scale * dr.x));
+      put(3 * k + 1, tg::f2bf(scale * dr.y));
+      put(3 * k + 2, tg::f2bf(scale * dr.z));
+      put(24 + k, tg::f2bf(scale * dk * sig));
+      put(32 + k, tg::f2bf(scale * (gk - wk)));
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int k = k0 + j;
-        const float gk = wk[j] * f[j] / q;
-        const float c = th::dot(mu[j], om);
-        const v3 dr = th::muls(th::sub(om, th::muls(mu[j], c)), gk * kap[j] / len[j]);
-        const float e2 = tg::fexp(-2.0f * kap[j]);
-        const float dk = gk * (1.0f / kap[j] - 2.0f * e2 / (1.0f - e2) + (c - 1.0f));
-        const float sig = kap[j] >= 1000.0f ? 0.0f : 1.0f / (1.0f + tg::fexp(-o[24 + k]));
-        dz[3 * k] = tg::f2bf(scale * dr.x);
-        dz[3 * k + 1] = tg::f2bf(scale * dr.y);
-        dz[3 * k + 2] = tg::f2bf(scale * dr.z);
-        dz[24 + k] = tg::f2bf(scale * dk * sig);
-        dz[32 + k] = tg::f2bf(scale * (gk - wk[j]));
-      }
-#pragma unroll
-      for (int j = 0; j < 6; ++j) dz[TG_OUT + 6 * p + j] = 0;
+      for (int j = 0; j < 3; ++j) put(TG_OUT + 3 * k + j, 0);
     }
     __syncthreads();
     // ---- backward, layer by layer
     uint16_t* dcur = tDa;
     uint16_t* dnext = tDb;
+    uint16_t* dcurT = uDa;
+    uint16_t* dnextT = uDb;
 #pragma unroll
     for (int L = 3; L >= 0; --L) {
-      // weight gradient: dW[n][k] += sum_rows dZ[row][n] hin[L][row][k], n = 16 wv + r
+      // weight gradient: dW[n][k] += sum_rows dZ[row][n] hin[L][row][k], n = 16 wv + r,
+      // k in columns 16 (2h + kk) ..; operands from the transposed tiles, K = 32 samples
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) {
-        uint16_t av[8];
+        const tg_bf16x8 A = lds8(dcurT + (16u * wv + r) * TF_LD + ks * 32 + g * 8);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) av[j] = dcur[(ks * 32 + g * 8 + j) * TF_LD + 16u * wv + r];
-        const tg_bf16x8 A = __builtin_bit_cast(tg_bf16x8, *(uint4*)av);
+        for (int kk = 0; kk < 2; ++kk)
+          acc[L][kk] = mfma(A, lds8(hinT[L] + ((2u * h + kk) * 16u + r) * TF_LD + ks * 32 + g * 8), acc[L][kk]);
+      }
+      if (h == 0u) {  // the bias gradient over this lane's 16 samples, in sample order
+        const uint4 b0 = *(const uint4*)(dcurT + (16u * wv + (l & 15u)) * TF_LD + (l >> 4) * 16);
+        const uint4 b1 = *(const uint4*)(dcurT + (16u * wv + (l & 15u)) * TF_LD + (l >> 4) * 16 + 8);
+        const uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-          uint16_t bv[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) bv[j] = hin[L][(ks * 32 + g * 8 + j) * TF_LD + kb * 16 + r];
-          acc[L][kb] = mfma(A, __builtin_bit_cast(tg_bf16x8, *(uint4*)bv), acc[L][kb]);
+        for (int j = 0; j < 8; ++j) {
+          dbias[L] += tg::bf2f((uint16_t)(bw[j] & 0xffffu));
+          dbias[L] += tg::bf2f((uint16_t)(bw[j] >> 16));
         }
       }
+      if (L > 0) {  // dZ of the layer below, columns 16 wv .. 16 wv + 15, samples 32 h ..:
+                    // (dZ W) gated by the ReLU of this layer's input
 #pragma unroll
-      for (int j = 0; j < 16; ++j) dbias[L] += tg::bf2f(dcur[((l >> 4) * 16 + j) * TF_LD + 16u * wv + (l & 15u)]);
-      if (L > 0) {  // dZ of the layer below: (dZ W) gated by the ReLU of this layer's input
-        const tg_bf16x8 a0 = lds8(dcur + arow + g * 8u), a1 = lds8(dcur + arow + 32u + g * 8u);
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-          const uint16_t* wr = wio + L * 4096 + (nb * 16 + r) * 64;
-          tg_f32x4 c = mfma(a0, ld8(wr + g * 8u), tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-          c = mfma(a1, ld8(wr + 32u + g * 8u), c);
-          const uint32_t col = nb * 16 + r;
+        for (int mm = 0; mm < 2; ++mm) {
+          const uint32_t mb = 2u * h + mm;
+          const uint32_t arow = (16u * mb + r) * TF_LD;
+          tg_f32x4 c = mfma(lds8(dcur + arow + g * 8u), wb[L - 1][0], tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+          c = mfma(lds8(dcur + arow + 32u + g * 8u), wb[L - 1][1], c);
+          const uint32_t row0 = 16u * mb + 4u * g;
+          const uint2 gate2 = *(const uint2*)(hinT[L] + col * TF_LD + row0);
+          const uint16_t gate[4] = {(uint16_t)(gate2.x & 0xffffu), (uint16_t)(gate2.x >> 16),
+                                    (uint16_t)(gate2.y & 0xffffu), (uint16_t)(gate2.y >> 16)};
+          uint16_t dv[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const uint32_t row = 16u * wv + 4u * g + i;
-            const uint16_t gate = hin[L][row * TF_LD + col];
-            dnext[row * TF_LD + col] = tg::f2bf((gate & 0x7fffu) ? c[i] : 0.0f);
+            dv[i] = tg::f2bf((gate[i] & 0x7fffu) ? c[i] : 0.0f);
+            dnext[(row0 + i) * TF_LD + col] = dv[i];
           }
+          st4t(dnextT, col, row0, dv);
         }
         __syncthreads();  // the whole chunk's dZ is written and the old one fully read
         uint16_t* t = dcur;
         dcur = dnext;
         dnext = t;
+        t = dcurT;
+        dcurT = dnextT;
+        dnextT = t;
       }
     }
     __syncthreads();  // tiles are reloaded by the next chunk
@@ -287,40 +373,53 @@ __global__ void __launch_bounds__(256) k_train_fused(const uint16_t* __restrict_
   for (int L = 0; L < 4; ++L) {
     float* out = partials + (size_t)L * 256u * TG_PART + (size_t)blockIdx.x * TG_PART;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
+    for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) out[(16u * wv + 4u * g + i) * 64u + kb * 16 + r] = acc[L][kb][i];
-    float b = dbias[L];
-    b += __shfl_xor(b, 16, 64);
-    b += __shfl_xor(b, 32, 64);
-    if (l < 16u) out[64 * 64 + 16u * wv + l] = b;
+      for (int i = 0; i < 4; ++i) out[(16u * wv + 4u * g + i) * 64u + (2u * h + kk) * 16u + r] = acc[L][kk][i];
+    if (h == 0u) {
+      float b = dbias[L];
+      b += __shfl_xor(b, 16, 64);
+      b += __shfl_xor(b, 32, 64);
+      if (l < 16u) out[64 * 64 + 16u * wv + l] = b;
+    }
   }
+  // the block's loss, waves in order (one partial per block; k_adam sums them in order)
   for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
-  if (l == 0u && lsum != 0.0f) atomicAdd(loss, lsum);
+  if (l == 0u) lred[wave] = lsum;
+  __syncthreads();
+  if (tid == 0u) {
+    float t = 0.0f;
+#pragma unroll
+    for (int i = 0; i < TF_THREADS / 64; ++i) t += lred[i];
+    loss_part[blockIdx.x] = t;
+  }
 }
 
-// First stage of the fixed-order sum of the per-block weight-gradient partials: group j
-// of 16 sums blocks j, j + 16, j + 32, ... of every layer element.
+// The gradient of a parameter is the fixed-order sum of its per-block partials: group j
+// of 16 sums blocks j, j + 16, j + 32, ... in order, then the group sums are added in
+// order (at most 256 blocks).
 #define TG_GROUPS 16
-__global__ void __launch_bounds__(256) k_reduce_partials(const float* __restrict__ partials, uint32_t nblocks,
-                                                         float* __restrict__ groups) {
-  const uint32_t e = blockIdx.x * 256u + threadIdx.x, j = blockIdx.y;
-  if (e >= 4u * TG_PART) return;
-  const uint32_t l = e / TG_PART, off = e % TG_PART;
-  const float* base = partials + (size_t)l * 256u * TG_PART + off;
-  float s = 0.0f;
-  for (uint32_t b = j; b < nblocks; b += TG_GROUPS) s += base[(size_t)b * TG_PART];
-  groups[(size_t)j * 4u * TG_PART + e] = s;
-}
 
-// Adam on the FP32 master parameters (gradient = the sum of the TG_GROUPS group sums,
-// or only the gradient when lr == 0), then both padded BF16 copies and biases.
-__global__ void k_adam(float* __restrict__ master, float* __restrict__ m1, float* __restrict__ m2,
-                       const float* __restrict__ groups, uint32_t ngroups, float lr, float b1,
+// Adam on the FP32 master parameters (gradient = the fixed-order sum of the `nblocks`
+// block partials, or only the gradient when lr == 0), then both padded BF16 copies and
+// biases.  A block takes 64 consecutive parameters; wave q sums groups 4q..4q+3 of each
+// (64 independent loads per lane, partials TG_PART floats apart), wave 0 adds the 16
+// group sums in order and applies the update.
+__global__ void __launch_bounds__(256) k_adam(float* __restrict__ master, float* __restrict__ m1,
+                       float* __restrict__ m2, const float* __restrict__ partials, float lr, float b1,
                        float b2, float eps, float bc1, float bc2, float* __restrict__ grad_out,
-                       uint16_t* __restrict__ woi, uint16_t* __restrict__ wio, float* __restrict__ bias) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= TG_NPARAMS) return;
+                       uint16_t* __restrict__ woi, uint16_t* __restrict__ wio, float* __restrict__ bias,
+                       const float* __restrict__ loss_part, uint32_t nblocks, float* __restrict__ loss) {
+  __shared__ float gs[TG_GROUPS][64];
+  const uint32_t lane = threadIdx.x & 63u, q = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 64u + lane;
+  if (loss != nullptr && blockIdx.x == 0 && q == 0u) {  // the step's loss: fixed-order sum of the block partials
+    float t = 0.0f;
+    for (uint32_t b = lane; b < nblocks; b += 64u) t += loss_part[b];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0u) *loss = t;
+  }
+  const bool valid = i < TG_NPARAMS;
   int l = 0;
   while (l < 3 && (int)i >= layer_dims(l + 1).woff) ++l;
   const LayerDims D = layer_dims(l);
@@ -332,9 +431,30 @@ __global__ void k_adam(float* __restrict__ master, float* __restrict__ m1, float
     n = ((int)i - D.woff) / D.kin;
     k = ((int)i - D.woff) % D.kin;
   }
-  const size_t e = (size_t)l * TG_PART + (is_bias ? (size_t)64 * 64 + n : (size_t)n * 64 + k);
+  if (valid) {
+    const float* base =
+        partials + (size_t)l * 256u * TG_PART + (is_bias ? (size_t)64 * 64 + n : (size_t)n * 64 + k);
+    float v[4][256 / TG_GROUPS];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (uint32_t t = 0; t < 256 / TG_GROUPS; ++t) {
+        const uint32_t b = 4u * q + jj + t * TG_GROUPS;
+        v[jj][t] = b < nblocks ? base[(size_t)b * TG_PART] : 0.0f;
+      }
+#pragma unroll
+    for (uint32_t jj = 0; jj < 4; ++jj) {
+      float s = 0.0f;
+#pragma unroll
+      for (uint32_t t = 0; t < 256 / TG_GROUPS; ++t)
+        if (4u * q + jj + t * TG_GROUPS < nblocks) s += v[jj][t];
+      gs[4u * q + jj][lane] = s;
+    }
+  }
+  __syncthreads();
+  if (!valid || q != 0u) return;
   float gsum = 0.0f;
-  for (uint32_t j = 0; j < ngroups; ++j) gsum += groups[(size_t)j * 4u * TG_PART + e];
+  for (uint32_t j = 0; j < TG_GROUPS && j < nblocks; ++j) gsum += gs[j][lane];
   if (grad_out) grad_out[i] = gsum;
   float p = master[i];
   if (lr > 0.0f) {
@@ -355,12 +475,21 @@ __global__ void k_adam(float* __restrict__ master, float* __restrict__ m1, float
   }
 }
 
+// BF16 encoding of host-side features: one thread per 8-element segment of a padded row
+// (one 16-byte store); rows from `count` on and columns from TG_IN on are zero.
 __global__ void k_encode_host(const float* __restrict__ feats, uint32_t count, uint32_t rows,
                               uint16_t* __restrict__ X) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, b = t >> 3, seg = t & 7u;
   if (b >= rows) return;
-  for (int j = 0; j < TG_INP; ++j)
-    X[(size_t)b * TG_INP + j] = (b < count && j < TG_IN) ? tg::f2bf(feats[(size_t)b * TG_IN + j]) : 0;
+  uint32_t v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t c = seg * 8u + 2u * j;
+    const uint16_t lo = (b < count && c < TG_IN) ? tg::f2bf(feats[(size_t)b * TG_IN + c]) : 0;
+    const uint16_t hi = (b < count && c + 1u < TG_IN) ? tg::f2bf(feats[(size_t)b * TG_IN + c + 1u]) : 0;
+    v[j] = (uint32_t)lo | ((uint32_t)hi << 16);
+  }
+  *(uint4*)(X + (size_t)b * TG_INP + seg * 8u) = make_uint4(v[0], v[1], v[2], v[3]);
 }
 
 __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
@@ -378,11 +507,16 @@ __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
 
 // Fused inference over a queue of deferred diffuse vertices, one wave per 16-row tile
 // and persistent over the queue.  Row i of the queue is the vertex (p, n, wo) stored in
-// queue order (vp, vn, vw).  Lane (r, g) encodes features 12g..12g+11 of row r into an
-// LDS tile (each sin/cos pair once); the four layers then run on MFMA with every weight
+// queue order (vp, vn, vw).  Lane (r, g) encodes a quarter of row r's position sin/cos
+// pairs and a quarter of its direction bins into an LDS tile; the four layers then run on MFMA with every weight
 // fragment held in registers (loaded once per wave) and activations passed through the
 // same tile (BF16, row stride 72 elements = 144 bytes, 16-byte aligned rows); each row's
 // 40 outputs are written in FP32.
+#define TG_PAIRS_PER_GROUP (3 * TG_POS_OCT / 4)  // position sin/cos pairs per lane group
+#define TG_BINS_PER_GROUP (6 * TG_DIR_BINS / 4)  // direction bins per lane group
+static_assert(TG_PAIRS_PER_GROUP * 4 == 3 * TG_POS_OCT && TG_BINS_PER_GROUP * 4 == 6 * TG_DIR_BINS &&
+                  TG_BINS_PER_GROUP % 2 == 0,
+              "the encoding splits evenly over four lane groups");
 __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__ woi,
                                                     const float* __restrict__ bias,
                                                     const uint32_t* __restrict__ count_ptr,
@@ -428,20 +562,22 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
       n = th::mk(b.x, b.y, b.z);
       wo = th::mk(c.x, c.y, c.z);
     }
+    // every lane group does the same work: a quarter of the position sin/cos pairs and a
+    // quarter of the direction bins (no divergence between the two kinds of feature)
 #pragma unroll
-    for (int jj = 0; jj < 12; jj += 2) {
-      const int j = 12 * (int)g + jj;
-      float f0, f1;
-      if (j < 6 * TG_POS_OCT) {
-        tg::encode_pos_pair(j, p01, &f0, &f1);
-      } else {
-        f0 = tg::encode_feature(j, p01, n, wo);
-        f1 = tg::encode_feature(j + 1, p01, n, wo);
-      }
-      *(uint32_t*)(T + r * 72 + j) = (uint32_t)tg::f2bf(f0) | ((uint32_t)tg::f2bf(f1) << 16);
+    for (int t = 0; t < TG_PAIRS_PER_GROUP; ++t) {
+      const int j = 2 * (TG_PAIRS_PER_GROUP * (int)g + t);
+      float s, c;
+      tg::encode_pos_pair(j, p01, &s, &c);
+      *(uint32_t*)(T + r * 72 + j) = tg::f2bf2(s, c);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) T[r * 72 + 48 + 4 * g + j] = 0;
+    for (int t = 0; t < TG_BINS_PER_GROUP; t += 2) {
+      const int j = 6 * TG_POS_OCT + TG_BINS_PER_GROUP * (int)g + t;
+      *(uint32_t*)(T + r * 72 + j) =
+          tg::f2bf2(tg::encode_feature(j, p01, n, wo), tg::encode_feature(j + 1, p01, n, wo));
+    }
+    *(uint2*)(T + r * 72 + 48 + 4 * g) = make_uint2(0u, 0u);
     __syncthreads();
 #pragma unroll
     for (int L = 0; L < 4; ++L) {
@@ -498,12 +634,12 @@ struct th_guide {
   uint32_t cap = 0;
   uint16_t *X0 = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr;
   float *O = nullptr, *feat = nullptr, *dirs = nullptr, *w = nullptr, *partials = nullptr, *scal = nullptr;
-  float *groups = nullptr, *wpart = nullptr;
+  float *wpart = nullptr, *lpart = nullptr;
   uint32_t nblocks = 0;
   std::vector<void*> ws;
 };
 
-static const uint32_t kWgradBlocks = 256;  // partial slots per layer (k_reduce_partials assumes 256)
+static const uint32_t kWgradBlocks = 256;  // partial slots per layer (k_adam sums at most 256)
 
 static int32_t guide_reserve(th_guide* g, uint32_t rows) {
   if (g->cap >= rows) return TH_OK;
@@ -526,8 +662,8 @@ static int32_t guide_reserve(th_guide* g, uint32_t rows) {
       (rc = al((void**)&g->feat, (size_t)cap * TG_IN * 4)) || (rc = al((void**)&g->dirs, (size_t)cap * 12)) ||
       (rc = al((void**)&g->w, (size_t)cap * 4)) ||
       (rc = al((void**)&g->partials, (size_t)kWgradBlocks * 4 * TG_PART * 4)) ||
-      (rc = al((void**)&g->groups, (size_t)TG_GROUPS * 4 * TG_PART * 4)) ||
-      (rc = al((void**)&g->wpart, 256 * 4)) || (rc = al((void**)&g->scal, 64)))
+      (rc = al((void**)&g->wpart, 256 * 4)) || (rc = al((void**)&g->lpart, kWgradBlocks * 4)) ||
+      (rc = al((void**)&g->scal, 64)))
     return rc;
   g->cap = cap;
   return TH_OK;
@@ -561,8 +697,8 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
   const uint32_t rows_per = round_up((count + kWgradBlocks - 1) / kWgradBlocks, 64);
   const uint32_t nb = (count + rows_per - 1) / rows_per;
   float* P = g->partials;
-  hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(256), 0, st, X, dirs, w, count, rows_per, g->wpart, g->d_woi,
-                     g->d_wio, g->d_bias, P, g->scal + 1);
+  hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(TF_THREADS), 0, st, X, dirs, w, count, rows_per, g->wpart, g->d_woi,
+                     g->d_wio, g->d_bias, P, g->lpart);
   TH_CHECK(hipGetLastError());
   float bc1 = 1.0f, bc2 = 1.0f;
   if (lr > 0.0f) {
@@ -571,11 +707,9 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
     bc2 = 1.0f - powf(g->beta2, (float)g->step);
   }
   // partials of layer l: kWgradBlocks slots of TG_PART floats; the first nb are summed
-  hipLaunchKernelGGL(k_reduce_partials, dim3((4 * TG_PART + 255) / 256, TG_GROUPS), dim3(256), 0, st, P, nb,
-                     g->groups);
-  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 255) / 256), dim3(256), 0, st, g->d_master,
-                     g->d_m1, g->d_m2, g->groups, std::min<uint32_t>(nb, TG_GROUPS), lr, g->beta1, g->beta2,
-                     g->eps, bc1, bc2, g->d_grad, g->d_woi, g->d_wio, g->d_bias);
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, st, g->d_master,
+                     g->d_m1, g->d_m2, P, lr, g->beta1, g->beta2,
+                     g->eps, bc1, bc2, g->d_grad, g->d_woi, g->d_wio, g->d_bias, g->lpart, nb, g->scal + 1);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -644,9 +778,9 @@ static void host_init_params(uint32_t seed, std::vector<float>& p) {
 }
 
 static int32_t guide_refresh(th_guide* g) {  // bf16 copies from the master weights
-  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 255) / 256), dim3(256), 0, g->ctx->stream, g->d_master,
-                     g->d_m1, g->d_m2, g->groups, 0u, 0.0f, g->beta1, g->beta2, g->eps, 1.0f, 1.0f,
-                     (float*)nullptr, g->d_woi, g->d_wio, g->d_bias);
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, g->ctx->stream, g->d_master,
+                     g->d_m1, g->d_m2, g->partials, 0.0f, g->beta1, g->beta2, g->eps, 1.0f, 1.0f,
+                     (float*)nullptr, g->d_woi, g->d_wio, g->d_bias, (const float*)nullptr, 0u, (float*)nullptr);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -734,7 +868,7 @@ int32_t th_guide_forward(th_guide* g, const float* feats, uint32_t count, float*
   hipStream_t st = g->ctx->stream;
   uint32_t rows = round_up(count, 32);
   TH_CHECK(hipMemcpyAsync(g->feat, feats, (size_t)count * TG_IN * 4, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_encode_host, dim3((rows + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
+  hipLaunchKernelGGL(k_encode_host, dim3((rows * 8u + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
   if ((rc = th_guide_forward_rows(g, g->X0, nullptr, rows, st))) return rc;
   std::vector<float> o((size_t)rows * TG_OUTP);
   TH_CHECK(hipMemcpyAsync(o.data(), g->O, o.size() * 4, hipMemcpyDeviceToHost, st));
@@ -754,7 +888,7 @@ int32_t th_guide_train_batch(th_guide* g, const float* feats, const float* dirs,
   TH_CHECK(hipMemcpyAsync(g->feat, feats, (size_t)count * TG_IN * 4, hipMemcpyHostToDevice, st));
   TH_CHECK(hipMemcpyAsync(g->dirs, dirs, (size_t)count * 12, hipMemcpyHostToDevice, st));
   TH_CHECK(hipMemcpyAsync(g->w, weights, (size_t)count * 4, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_encode_host, dim3((rows + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
+  hipLaunchKernelGGL(k_encode_host, dim3((rows * 8u + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
   if ((rc = guide_step(g, g->X0, g->dirs, g->w, count, update ? g->lr : 0.0f))) return rc;
   if (loss) TH_CHECK(hipMemcpyAsync(loss, g->scal + 1, 4, hipMemcpyDeviceToHost, st));
   if (grad) TH_CHECK(hipMemcpyAsync(grad, g->d_grad, TG_NPARAMS * 4, hipMemcpyDeviceToHost, st));
