@@ -552,6 +552,12 @@ int32_t th_render_guided(th_context* ctx, th_scene* scene, th_guide* guide,
                          const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
                          const th_guided_options* options, th_guided_stats* stats);
 int32_t th_guide_mfma_selftest(th_context* ctx, const float* a, const float* b, float* c);
+/* The device's float -> BF16 rounding (the gfx950 conversion instruction, used by every
+ * network kernel) against integer round-to-nearest-even over all 2^32 float patterns:
+ * *mismatches counts non-NaN inputs that differ (expected 0), *nan_mismatches (may be
+ * null) NaN inputs whose bits differ.  No counterpart in the reference (its renderer has
+ * no network); a check of the rounding the guide's oracle emulates. */
+int32_t th_bf16_selftest(th_context* ctx, uint64_t* mismatches, uint64_t* nan_mismatches);
 
 /* Linear HDR float4 -> RGBA8 with exposure, ACES fitted and sRGB OETF
  * (image_output.rs:59-92, color.rs:23-75).  Host implementation. */

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import guide as G
-from toaster_amd.guide import Guide, mfma_selftest
+from toaster_amd.guide import Guide, bf16_selftest, mfma_selftest
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +27,15 @@ def test_mfma_fragment_mapping(gpu_ctx):
     c = mfma_selftest(gpu_ctx, a, b)
     ref = G.bf16(a) @ G.bf16(b)
     assert np.allclose(c, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_bf16_rounding_matches_oracle(gpu_ctx):
+    """The device's float -> BF16 conversion (the gfx950 instruction every network kernel
+    uses) gives the oracle's integer round-to-nearest-even (oracle/guide.py bf16) for all
+    2^32 float patterns except NaNs, denormals and infinities included."""
+    print("case bf16 selftest over 2^32 patterns", flush=True)
+    bad, _ = bf16_selftest(gpu_ctx)
+    assert bad == 0
 
 
 def _inputs(rng, n):

@@ -492,6 +492,28 @@ __global__ void k_encode_host(const float* __restrict__ feats, uint32_t count, u
   *(uint4*)(X + (size_t)b * TG_INP + seg * 8u) = make_uint4(v[0], v[1], v[2], v[3]);
 }
 
+// tg::f2bf on the device (the conversion instruction) against the integer
+// round-to-nearest-even over every 32-bit pattern: out[0] counts non-NaN mismatches,
+// out[1] NaN inputs whose bits differ (the instruction returns a quiet NaN).
+__global__ void __launch_bounds__(256) k_bf16_selftest(unsigned long long* __restrict__ out) {
+  unsigned long long bad = 0, bad_nan = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+    const uint32_t u = (uint32_t)i;
+    uint32_t ref;
+    if ((u & 0x7f800000u) == 0x7f800000u)
+      ref = u >> 16;
+    else
+      ref = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    if (tg::f2bf(__uint_as_float(u)) != (uint16_t)ref) {
+      if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) ++bad_nan;
+      else ++bad;
+    }
+  }
+  if (bad) atomicAdd(&out[0], bad);
+  if (bad_nan) atomicAdd(&out[1], bad_nan);
+}
+
 __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
   // A 16x32 and B 32x16 row-major FP32 (rounded to BF16 here); C 16x16
   uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
@@ -915,6 +937,26 @@ int32_t th_guide_mfma_selftest(th_context* ctx, const float* A, const float* B, 
   (void)hipFree(dB);
   (void)hipFree(dC);
   if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("mfma selftest: ") + hipGetErrorString(e));
+  return TH_OK;
+}
+
+int32_t th_bf16_selftest(th_context* ctx, uint64_t* mismatches, uint64_t* nan_mismatches) {
+  if (!ctx || !mismatches) return th_set_error(TH_E_INVALID, "null argument");
+  TH_CHECK(hipSetDevice(ctx->device));
+  unsigned long long* d = nullptr;
+  unsigned long long h[2] = {0, 0};
+  hipError_t e = hipMalloc((void**)&d, sizeof(h));
+  if (e == hipSuccess) e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_bf16_selftest, dim3(8192), dim3(256), 0, ctx->stream, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("bf16 selftest: ") + hipGetErrorString(e));
+  *mismatches = h[0];
+  if (nan_mismatches) *nan_mismatches = h[1];
   return TH_OK;
 }
 

@@ -85,6 +85,14 @@ def mfma_selftest(ctx: GpuContext, a, b):
     return c
 
 
+def bf16_selftest(ctx: GpuContext):
+    """(non-NaN mismatches, NaN mismatches) of the device's float -> BF16 rounding against
+    integer round-to-nearest-even over all 2^32 float patterns."""
+    m, mn = C.c_uint64(), C.c_uint64()
+    A.check(A.lib().th_bf16_selftest(ctx.handle, C.byref(m), C.byref(mn)))
+    return m.value, mn.value
+
+
 # Defaults measured on the indirect-lit terrain (1280x720, 262,144 triangles): guide
 # probability 0.8, environment light by the continuation alone at diffuse vertices, Adam
 # steps of 2^15 samples, a warm-up of 768 steps at 16 steps per pass (48 passes) with
