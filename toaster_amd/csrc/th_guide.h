@@ -173,8 +173,10 @@ TH_HD float fsqrt_fast(float x) {
 
 TH_HD void decode(const float* o, Mixture* m) {
   float mx = -3.0e38f;
+#pragma unroll
   for (int k = 0; k < TG_LOBES; ++k) mx = th::fmax_(mx, o[32 + k]);
   float sum = 0.0f;
+#pragma unroll
   for (int k = 0; k < TG_LOBES; ++k) {
     v3 r = th::mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
     m->mu[k] = th::smul(frsq(th::dot(r, r) + 1e-8f), r);
@@ -184,6 +186,7 @@ TH_HD void decode(const float* o, Mixture* m) {
   }
   const float INV_2PI = 0.15915494f;
   const float rs = frcp(sum);
+#pragma unroll
   for (int k = 0; k < TG_LOBES; ++k) {
     m->w[k] = m->w[k] * rs;
     m->wn[k] = m->w[k] * (m->kappa[k] * INV_2PI * frcp(1.0f - fexp(-2.0f * m->kappa[k])));
@@ -198,6 +201,7 @@ TH_HD float vmf_pdf(v3 mu, float kappa, v3 w) {
 
 TH_HD float mixture_pdf(const Mixture& m, v3 w) {
   float p = 0.0f;
+#pragma unroll
   for (int k = 0; k < TG_LOBES; ++k) p += m.wn[k] * fexp(m.kappa[k] * (th::dot(m.mu[k], w) - 1.0f));
   return p;
 }
@@ -222,14 +226,33 @@ TH_HD v3 vmf_sample(v3 mu, float kappa, float u1, float u2) {
   return th::add(th::smul(w, mu), th::add(th::smul(r * c, t), th::smul(r * s, b)));
 }
 
+// Lobe k is the first whose running weight sum exceeds u0 (the last when none does).
+// The index is found first, then the lobe gathered with value selects at compile-time
+// positions, so the decoded mixture stays in registers (a run-time index into it, or a
+// select between lobe addresses, would place it in scratch memory).
 TH_HD v3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
-  int k = 0;
+  int ks = 0;
   float c = m.w[0];
-  while (k < TG_LOBES - 1 && u0 >= c) {
-    ++k;
-    c += m.w[k];
+  bool more = u0 >= c;
+#pragma unroll
+  for (int k = 1; k < TG_LOBES; ++k) {
+    if (more) {
+      ks = k;
+      c += m.w[k];
+      more = u0 >= c;
+    }
   }
-  return vmf_sample(m.mu[k], m.kappa[k], u1, u2);
+  v3 mu = m.mu[0];
+  float kappa = m.kappa[0];
+#pragma unroll
+  for (int k = 1; k < TG_LOBES; ++k) {
+    const bool s = ks == k;
+    mu.x = s ? m.mu[k].x : mu.x;
+    mu.y = s ? m.mu[k].y : mu.y;
+    mu.z = s ? m.mu[k].z : mu.z;
+    kappa = s ? m.kappa[k] : kappa;
+  }
+  return vmf_sample(mu, kappa, u1, u2);
 }
 
 }  // namespace tg

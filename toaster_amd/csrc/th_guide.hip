@@ -542,13 +542,20 @@ static_assert(TG_PAIRS_PER_GROUP * 4 == 3 * TG_POS_OCT && TG_BINS_PER_GROUP * 4 
 __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__ woi,
                                                     const float* __restrict__ bias,
                                                     const uint32_t* __restrict__ count_ptr,
+                                                    uint32_t count_stride, uint32_t nseg, uint32_t seg_cap,
                                                     const float4* __restrict__ vp,
                                                     const float4* __restrict__ vn,
                                                     const float4* __restrict__ vw, float4 bmin,
                                                     float4 bscale, float* __restrict__ O) {
   __shared__ __align__(16) uint16_t T[16 * 72];
-  const uint32_t count = *count_ptr;
-  if (blockIdx.x * 16u >= count) return;  // one wave per block: uniform exit
+  // block b walks queue segment b % nseg (the grid is a multiple of nseg)
+  const uint32_t sg = blockIdx.x % nseg, jb = blockIdx.x / nseg, nbs = gridDim.x / nseg;
+  const uint32_t count = count_ptr[(size_t)sg * count_stride];
+  if (jb * 16u >= count) return;  // one wave per block: uniform exit
+  vp += (size_t)sg * seg_cap;
+  vn += (size_t)sg * seg_cap;
+  vw += (size_t)sg * seg_cap;
+  O += (size_t)sg * seg_cap * TG_OUT;
   const uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
   tg_bf16x8 W[4][4][2];
   float bv[4][4];
@@ -562,17 +569,17 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
     }
   // the next tile's vertex rows are loaded while the current tile runs its layers
   const float4 zero4 = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
-  uint32_t base = blockIdx.x * 16u;
+  uint32_t base = jb * 16u;
   float4 na = zero4, nbv = zero4, nc = zero4;
   if (base + r < count) {
     na = vp[base + r];
     nbv = vn[base + r];
     nc = vw[base + r];
   }
-  for (; base < count; base += gridDim.x * 16u) {
+  for (; base < count; base += nbs * 16u) {
     const uint32_t row = base + r;
     const float4 a = na, b = nbv, c = nc;
-    const uint32_t nrow = row + gridDim.x * 16u;
+    const uint32_t nrow = row + nbs * 16u;
     if (nrow < count) {
       na = vp[nrow];
       nbv = vn[nrow];
@@ -737,20 +744,25 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
 }
 
 // ====================================================================== renderer interface
-int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t max_rows, const float4* vp,
-                    const float4* vn, const float4* vw, const float* bmin, const float* bmax, float* O,
-                    hipStream_t st) {
+int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride, uint32_t nseg,
+                    uint32_t seg_cap, const float4* vp, const float4* vn, const float4* vw,
+                    const float* bmin, const float* bmax, float* O, hipStream_t st) {
+  if (nseg == 0u) return th_set_error(TH_E_INVALID, "guide queue without segments");
   // persistent: as many one-wave blocks as stay resident (2 per SIMD at this kernel's
-  // register count), so each wave loads the weights once and walks many tiles
-  uint32_t grid = std::min<uint32_t>((max_rows + 15u) / 16u, TG_INFER_WAVES_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1));
+  // register count), so each wave loads the weights once and walks many tiles; a
+  // multiple of nseg, so every segment has the same number of blocks
+  const uint32_t tiles = (seg_cap + 15u) / 16u;
+  uint32_t per_seg = std::min<uint32_t>(
+      tiles, (TG_INFER_WAVES_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1) + nseg - 1u) / nseg);
+  const uint32_t grid = per_seg * nseg;
   if (grid == 0) return TH_OK;
   float4 lo = make_float4(bmin[0], bmin[1], bmin[2], 0.0f), sc;
   float e[3];
   for (int a = 0; a < 3; ++a) e[a] = bmax[a] - bmin[a];
   sc = make_float4(e[0] > 0.0f ? 1.0f / e[0] : 0.0f, e[1] > 0.0f ? 1.0f / e[1] : 0.0f,
                    e[2] > 0.0f ? 1.0f / e[2] : 0.0f, 0.0f);
-  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, st, g->d_woi, g->d_bias, d_count, vp, vn, vw,
-                     lo, sc, O);
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, st, g->d_woi, g->d_bias, d_count, count_stride,
+                     nseg, seg_cap, vp, vn, vw, lo, sc, O);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }

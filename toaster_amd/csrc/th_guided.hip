@@ -51,12 +51,15 @@ struct GdBufs {
   float4 *vp, *vn, *vw;  // guide-queue row: (p, albedo.x), (n, albedo.y), (wo, albedo.z)
   float4 *sho, *sh0, *c0, *sh1, *c1;
   uint32_t* vis;
-  uint32_t *q0, *q1, *qs, *qg;
-  uint32_t* cnt;  // [0] [1] live, [2] shadow, [3] guide queue, [5] diffuse total,
-                  // [8 + c] records of chunk c
+  uint32_t *q0, *q1, *qs, *qg;  // queues in GD_NSEG segments (see GD_QC)
+  uint32_t* cnt;  // GD_C: [0] paths at the first bounce, [5] diffuse total, [6] rows the
+                  // network evaluated, [8 + c] records of chunk c; GD_QC: queue segments
+  uint32_t seg_cap;  // paths per segment of q0, q1, qg (and its rows); qs holds twice that
   float* O;       // guide outputs, TG_OUT per guide-queue row
   uint16_t* rX;   // training records: BF16 features, 64 wide (zero beyond 48)
   float *rdir, *rw;
+  uint16_t* tX;   // the records of each chunk made contiguous for training (rows of rX,
+  float* tdir;    // rdir; rw is written at the same contiguous rows)
   float4 *rbeta, *rL;
   uint32_t* rpath;
   uint32_t rcap;  // chunks x chunk capacity
@@ -64,6 +67,23 @@ struct GdBufs {
 
 #define GD_MAX_CHUNKS 16u
 #define GD_C(c) ((c) * 32u)  // queue counters 128 bytes apart: no two share a cache line
+// Queues in segments.  A wave that appends to a queue takes its slots with one atomic on
+// a counter, and with one counter for the whole queue those same-address atomics
+// serialise in the memory system (the first bounce's shading launch on a 1280x720 frame
+// took 173 us with them, 24 us with the slots taken without atomics).  Each
+// queue is therefore split into GD_NSEG segments with a counter each: block b appends to
+// and consumes from segment b % GD_NSEG (grids are multiples of GD_NSEG), so a path stays
+// in one segment for the whole pass, a segment never holds more than seg_cap paths, and
+// its state is read back by the same XCD (blocks are dealt to the 8 XCDs round-robin).
+// Queue q: 0 and 1 the live queues, 2 the shadow-ray queue, 3 the guide queue.
+#ifndef GD_NSEG
+#define GD_NSEG 16u
+#endif
+#define GD_QC(q, sg) GD_C(32u + (q) * GD_NSEG + (sg))
+// Training records of chunk c from segment sg: their own counter (records of one chunk
+// from every segment on one counter serialise the same way).
+#define GD_RC(c, sg) GD_C(32u + 4u * GD_NSEG + (c) * GD_NSEG + (sg))
+#define GD_CNT_LINES (32u + 4u * GD_NSEG + GD_MAX_CHUNKS * GD_NSEG)
 
 struct GdArgs {
   uint32_t n;           // paths per pass (pixels)
@@ -76,7 +96,8 @@ struct GdArgs {
   uint32_t rec_all;     // record every eligible vertex
   uint32_t env_nee;     // environment sample + shadow ray at diffuse vertices
   uint32_t nchunks;     // records are spread uniformly at random over this many chunks
-  uint32_t chunk_cap;   // capacity of one chunk (chunk c holds rows c * chunk_cap ...)
+  uint32_t chunk_cap;   // capacity of one chunk (chunk c holds rows c * chunk_cap ...), in
+                        // GD_NSEG pieces of chunk_cap / GD_NSEG, one per queue segment
   float bmin[3], bscale[3];
 };
 
@@ -116,6 +137,7 @@ __device__ __forceinline__ void gd_push2(bool la, uint32_t a, bool lb, uint32_t 
 __global__ void gd_reset(GdBufs B, uint32_t all) {
   const uint32_t t = threadIdx.x;
   if (t < 4u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[GD_C(t)] = 0u;
+  for (uint32_t c = t; c < (4u + GD_MAX_CHUNKS) * GD_NSEG; c += blockDim.x) B.cnt[GD_C(32u + c)] = 0u;
 }
 
 // ---------------------------------------------------------------- generate
@@ -144,20 +166,26 @@ __global__ void __launch_bounds__(256) gd_generate(GdBufs B, GdArgs A, TH_GpuRen
 }
 
 // ---------------------------------------------------------------- extend / shadow
-// `first`: the first bounce, whose queue is the identity.
+// `first`: the first bounce, whose queue is the identity (every path, in path order).
+// Later bounces: block b takes rays of live-queue segment b % GD_NSEG, 16 per wave.
 __global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRenderParams P, uint32_t li,
                                                 uint32_t first) {
   extern __shared__ uint2 lds_stack[];
-  const uint32_t* q = li ? B.q1 : B.q0;
-  const uint32_t n = B.cnt[GD_C(li)];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {  // queues filled later in this iteration
-    B.cnt[GD_C(2)] = 0;
-    B.cnt[GD_C(3)] = 0;
-    B.cnt[GD_C(li ^ 1u)] = 0;
-  }
+  const uint32_t sg = blockIdx.x % GD_NSEG;
+  const uint32_t* q = (li ? B.q1 : B.q0) + (size_t)sg * B.seg_cap;
+  const uint32_t n = first ? B.cnt[GD_C(0)] : B.cnt[GD_QC(li, sg)];
+  if (blockIdx.x == 0)  // queues filled later in this iteration
+    for (uint32_t c = threadIdx.x; c < GD_NSEG; c += blockDim.x) {
+      B.cnt[GD_QC(2u, c)] = 0;
+      B.cnt[GD_QC(3u, c)] = 0;
+      B.cnt[GD_QC(li ^ 1u, c)] = 0;
+    }
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
-  const uint32_t stride = gridDim.x * blockDim.x / 4u;
-  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / 4u;; i += stride) {
+  const uint32_t rays = blockDim.x / 4u;
+  const uint32_t start = first ? (blockIdx.x * blockDim.x + threadIdx.x) / 4u
+                               : (blockIdx.x / GD_NSEG) * rays + threadIdx.x / 4u;
+  const uint32_t stride = first ? gridDim.x * rays : (gridDim.x / GD_NSEG) * rays;
+  for (uint32_t i = start;; i += stride) {
     bool act = i < n;  // uniform per quad
     if (!__any(act)) break;
     if (act) {
@@ -177,16 +205,20 @@ __global__ void __launch_bounds__(64) gd_extend(GdBufs B, DevScene S, TH_GpuRend
   }
 }
 
+// Shadow rays of shadow-queue segment b % GD_NSEG, 16 per wave.
 __global__ void __launch_bounds__(64) gd_shadow(GdBufs B, DevScene S, TH_GpuRenderParams P) {
   extern __shared__ uint2 lds_stack[];
-  const uint32_t n = B.cnt[GD_C(2)];
+  const uint32_t sg = blockIdx.x % GD_NSEG;
+  const uint32_t* qs = B.qs + (size_t)sg * 2u * B.seg_cap;
+  const uint32_t n = B.cnt[GD_QC(2u, sg)];
   Stack st{lds_stack + (threadIdx.x >> 2), (int)(blockDim.x >> 2), 0};
-  const uint32_t stride = gridDim.x * blockDim.x / 4u;
-  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / 4u;; i += stride) {
+  const uint32_t rays = blockDim.x / 4u;
+  const uint32_t stride = (gridDim.x / GD_NSEG) * rays;
+  for (uint32_t i = (blockIdx.x / GD_NSEG) * rays + threadIdx.x / 4u;; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
     if (act) {
-      uint32_t e = B.qs[i], k = e >> 1;
+      uint32_t e = qs[i], k = e >> 1;
       float4 so = B.sho[k], sd = (e & 1u) ? B.sh1[k] : B.sh0[k];
       v3 o = th::mk(so.x, so.y, so.z), d = th::mk(sd.x, sd.y, sd.z);
       float limit = sd.w, t;
@@ -215,13 +247,16 @@ __global__ void __launch_bounds__(64) gd_shadow(GdBufs B, DevScene S, TH_GpuRend
 // estimator credits that share of the next bounce, so every variant here estimates the
 // same quantity as the reference (without it, an estimator that drops the environment
 // sample, or weighs it against the mixture, converges to a slightly different image).
+// MIX: `m` holds the vertex's decoded mixture (a compile-time flag rather than a null
+// test: comparing the address of a private variable with null keeps it out of registers).
+template <bool MIX>
 __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, false>& it,
                                               const TH_GpuRenderParams& P, uint32_t k, const v3& p,
                                               const v3& n, const v3& albedo, const tg::Mixture* m,
                                               float alpha, bool env_nee, bool last, v3& throughput,
                                               uint32_t& rng, v3& nd, float& pdf, bool& sh_light,
                                               bool& sh_env) {
-  const bool guided = m != nullptr && alpha > 0.0f;
+  const bool guided = MIX && alpha > 0.0f;
   {
     v3 so, dir, term;
     float limit;
@@ -301,13 +336,21 @@ __device__ __forceinline__ void apply_pending(const GdBufs& B, uint32_t k, uint3
 __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
                                                  const float* __restrict__ srgb_lut, uint32_t li) {
   const bool first = A.bounce == 0u;  // identity queue
-  const uint32_t* q = li ? B.q1 : B.q0;
-  const uint32_t n = B.cnt[GD_C(li)];
+  // block b reads and appends to segment b % GD_NSEG; on the identity queue its paths k
+  // are those with (k / 64) % GD_NSEG == b % GD_NSEG (the grid is a multiple of GD_NSEG)
+  const uint32_t sg = blockIdx.x % GD_NSEG;
+  const uint32_t* q = (li ? B.q1 : B.q0) + (size_t)sg * B.seg_cap;
+  const uint32_t n = first ? B.cnt[GD_C(0)] : B.cnt[GD_QC(li, sg)];
+  uint32_t* qlive = (li ? B.q0 : B.q1) + (size_t)sg * B.seg_cap;
+  uint32_t* qsh = B.qs + (size_t)sg * 2u * B.seg_cap;
+  const uint32_t gbase = sg * B.seg_cap;  // this segment's guide-queue rows
   Stack st{nullptr, 0, 0};
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
+  const uint32_t start = first ? blockIdx.x * blockDim.x + threadIdx.x
+                               : (blockIdx.x / GD_NSEG) * blockDim.x + threadIdx.x;
+  const uint32_t stride = first ? gridDim.x * blockDim.x : (gridDim.x / GD_NSEG) * blockDim.x;
+  for (uint32_t i = start;; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
     bool sh_light = false, sh_env = false, park = false, live = false;
@@ -358,7 +401,7 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
             park = true;
           } else {
             float pdf;
-            shade_diffuse(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, A.env_nee != 0u,
+            shade_diffuse<false>(B, it, P, k, rec.p, rec.n, albedo, nullptr, 0.0f, A.env_nee != 0u,
                           A.bounce + 1u >= P.max_bounces, throughput, rng, nd, pdf, sh_light, sh_env);
             flags = GD_PREV_DIFFUSE;
             o_next = make_float4(rec.p.x, rec.p.y, rec.p.z, pdf);
@@ -414,9 +457,9 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
       }
       B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
     }
-    gd_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
-    gd_push(live, k, li ? B.q0 : B.q1, &B.cnt[GD_C(li ^ 1u)]);
-    const uint32_t gi = gd_push(park, k, B.qg, &B.cnt[GD_C(3)]);
+    gd_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, qsh, &B.cnt[GD_QC(2u, sg)]);
+    gd_push(live, k, qlive, &B.cnt[GD_QC(li ^ 1u, sg)]);
+    const uint32_t gi = gbase + gd_push(park, k, B.qg + gbase, &B.cnt[GD_QC(3u, sg)]);
     if (park) {  // vertex data in queue order: contiguous rows for inference and shade B
       B.vp[gi] = pk_p;
       B.vn[gi] = pk_n;
@@ -428,8 +471,13 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
 // ---------------------------------------------------------------- shade B
 __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S, TH_GpuRenderParams P,
                                                  const float* __restrict__ srgb_lut, uint32_t li) {
-  const uint32_t n = B.cnt[GD_C(3)];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  // block b takes guide-queue segment b % GD_NSEG and appends to that segment's queues
+  const uint32_t sg = blockIdx.x % GD_NSEG, jb = blockIdx.x / GD_NSEG;
+  const uint32_t n = B.cnt[GD_QC(3u, sg)];
+  const uint32_t gbase = sg * B.seg_cap;
+  uint32_t* qlive = (li ? B.q0 : B.q1) + (size_t)sg * B.seg_cap;
+  uint32_t* qsh = B.qs + (size_t)sg * 2u * B.seg_cap;
+  if (jb == 0 && threadIdx.x == 0) {
     atomicAdd(&B.cnt[GD_C(5)], n);
     if (A.alpha > 0.0f) atomicAdd(&B.cnt[GD_C(6)], n);  // rows the network evaluated
   }
@@ -437,10 +485,11 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
   const bool use_guide = A.alpha > 0.0f;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;; i += stride) {
-    bool act = i < n;
+  const uint32_t stride = (gridDim.x / GD_NSEG) * blockDim.x;
+  for (uint32_t t = jb * blockDim.x + threadIdx.x;; t += stride) {
+    bool act = t < n;
     if (!__any(act)) break;
+    const uint32_t i = gbase + t;  // guide-queue row
     bool sh_light = false, sh_env = false, rec_ok = false, live = false;
     uint32_t k = 0;
     v3 p, nrm, wo, nd, beta;
@@ -469,7 +518,8 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
         tg::decode(o, &mix);
       }
       float pdf;
-      bool cont = shade_diffuse(B, it, P, k, p, nrm, albedo, use_guide ? &mix : nullptr, A.alpha,
+      // alpha = 0 (use_guide false) leaves `mix` unread
+      bool cont = shade_diffuse<true>(B, it, P, k, p, nrm, albedo, &mix, A.alpha,
                                 A.env_nee != 0u, A.bounce + 1u >= P.max_bounces, throughput, rng, nd, pdf,
                                 sh_light, sh_env);
       bool done = !cont || A.bounce + 1u >= P.max_bounces;
@@ -486,33 +536,38 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       B.rng[k] = rng;
       live = !done;
     }
-    gd_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, B.qs, &B.cnt[GD_C(2)]);
-    gd_push(live, k, li ? B.q0 : B.q1, &B.cnt[GD_C(li ^ 1u)]);
+    gd_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, qsh, &B.cnt[GD_QC(2u, sg)]);
+    gd_push(live, k, qlive, &B.cnt[GD_QC(li ^ 1u, sg)]);
     // chunk drawn at random so every training step sees an unbiased subset; slots are
-    // handed out with one atomic per (wave, chunk)
+    // handed out with one atomic per (wave, chunk), all of a wave's chunks at once: lane c
+    // takes chunk c's slots (a chain of atomics, each waiting for the last, had cost a
+    // wave one memory round trip per chunk)
     uint32_t r = 0xffffffffu;
     const uint32_t myc =
         rec_ok ? th::pcg_hash(k ^ (A.bounce << 27) ^ (A.smp * 0x2545F491u)) % A.nchunks : 0xffffffffu;
-    const uint32_t lane = threadIdx.x & 63u;
-    for (uint64_t pending = __ballot(rec_ok); pending != 0;) {
-      const int leader = __ffsll((unsigned long long)pending) - 1;
-      const uint32_t cc = __shfl(myc, leader, 64);
-      const uint64_t m = __ballot(myc == cc);
-      uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(&B.cnt[GD_C(8u + cc)], (uint32_t)__popcll(m));
-      base = __shfl(base, leader, 64);
-      if (myc == cc) {
-        const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (idx < A.chunk_cap) r = cc * A.chunk_cap + idx;
+    if (__ballot(rec_ok) != 0) {
+      const uint32_t lane = threadIdx.x & 63u;
+      uint64_t mymask = 0, lanemask = 0;  // lanes of this lane's chunk / of chunk `lane`
+      for (uint32_t c = 0; c < A.nchunks; ++c) {
+        const uint64_t m = __ballot(myc == c);
+        if (myc == c) mymask = m;
+        if (lane == c) lanemask = m;
       }
-      pending &= ~m;
+      uint32_t base = 0;
+      if (lanemask != 0) base = atomicAdd(&B.cnt[GD_RC(lane, sg)], (uint32_t)__popcll(lanemask));
+      const uint32_t cb = __shfl(base, rec_ok ? (int)myc : (int)lane, 64);
+      if (rec_ok) {
+        const uint32_t piece = A.chunk_cap / GD_NSEG;  // this segment's piece of the chunk
+        const uint32_t idx = cb + (uint32_t)__popcll(mymask & ((1ull << lane) - 1ull));
+        if (idx < piece) r = myc * A.chunk_cap + sg * piece + idx;
+      }
     }
     if (r != 0xffffffffu) {
       B.rpath[r] = k;
       v3 p01 = th::mk((p.x - A.bmin[0]) * A.bscale[0], (p.y - A.bmin[1]) * A.bscale[1],
                       (p.z - A.bmin[2]) * A.bscale[2]);
-      float feat[TG_IN];
-      tg::encode_row(p01, nrm, wo, feat);
+      // the features of tg::encode_row, two at a time at compile-time positions (no
+      // per-lane array, which would live in scratch memory)
       uint4* x = (uint4*)(B.rX + (size_t)r * 64u);  // 128-byte row, eight 16-byte stores
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -520,9 +575,15 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int j0 = 8 * q + 2 * t;
-          const uint32_t lo = j0 < TG_IN ? tg::f2bf(feat[j0 < TG_IN ? j0 : 0]) : 0u;
-          const uint32_t hi = j0 + 1 < TG_IN ? tg::f2bf(feat[j0 + 1 < TG_IN ? j0 + 1 : 0]) : 0u;
-          h[t] = lo | (hi << 16);
+          if (j0 < 6 * TG_POS_OCT) {
+            float fs, fc;
+            tg::encode_pos_pair(j0, p01, &fs, &fc);
+            h[t] = tg::f2bf2(fs, fc);
+          } else if (j0 < TG_IN) {
+            h[t] = tg::f2bf2(tg::encode_feature(j0, p01, nrm, wo), tg::encode_feature(j0 + 1, p01, nrm, wo));
+          } else {
+            h[t] = 0u;
+          }
         }
         x[q] = make_uint4(h[0], h[1], h[2], h[3]);
       }
@@ -552,17 +613,43 @@ __global__ void __launch_bounds__(256) gd_retire(GdBufs B, uint32_t n) {
 
 // ---------------------------------------------------------------- records, resolve
 // Weight of each record of the pass: lum((L - L_v) / beta_v), zero where not finite.
+// The chunk's records, in GD_NSEG pieces (one per queue segment), are gathered into
+// contiguous rows of tX, tdir and rw (piece 0's first, then piece 1's, ...), and the
+// chunk's total is left in GD_C(8 + c) for the host.  One thread per destination row.
 __global__ void __launch_bounds__(256) gd_weights(GdBufs B, uint32_t nchunks, uint32_t chunk_cap) {
+  __shared__ uint32_t pre[GD_MAX_CHUNKS][GD_NSEG + 1];  // pieces' first rows; [GD_NSEG]: total
+  const uint32_t piece = chunk_cap / GD_NSEG;
+  for (uint32_t c = threadIdx.x; c < nchunks; c += blockDim.x) {
+    uint32_t acc = 0;
+    for (uint32_t j = 0; j < GD_NSEG; ++j) {
+      pre[c][j] = acc;
+      acc += min(B.cnt[GD_RC(c, j)], piece);
+    }
+    pre[c][GD_NSEG] = acc;
+    if (blockIdx.x == 0) B.cnt[GD_C(8u + c)] = acc;
+  }
+  __syncthreads();
   const uint32_t n = nchunks * chunk_cap;
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    const uint32_t c = r / chunk_cap;
-    if (r - c * chunk_cap >= min(B.cnt[GD_C(8u + c)], chunk_cap)) continue;
+  for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < n; d += gridDim.x * blockDim.x) {
+    const uint32_t c = d / chunk_cap, o = d - c * chunk_cap;
+    if (o >= pre[c][GD_NSEG]) continue;
+    uint32_t sg = 0;  // the piece holding row o: the last whose first row is <= o
+    for (uint32_t j = 1; j < GD_NSEG; ++j)
+      if (pre[c][j] <= o) sg = j;
+    const uint32_t r = c * chunk_cap + sg * piece + (o - pre[c][sg]);
     float4 L = B.rad[B.rpath[r]], Lv = B.rL[r], b = B.rbeta[r];
     float v0 = b.x > 0.0f ? (L.x - Lv.x) / b.x : 0.0f;
     float v1 = b.y > 0.0f ? (L.y - Lv.y) / b.y : 0.0f;
     float v2 = b.z > 0.0f ? (L.z - Lv.z) / b.z : 0.0f;
     float w = 0.2126f * v0 + 0.7152f * v1 + 0.0722f * v2;
-    B.rw[r] = (w >= 0.0f && w < 3.0e38f) ? w : 0.0f;
+    B.rw[d] = (w >= 0.0f && w < 3.0e38f) ? w : 0.0f;
+    const uint4* src = (const uint4*)(B.rX + (size_t)r * 64u);
+    uint4* to = (uint4*)(B.tX + (size_t)d * 64u);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) to[q] = src[q];
+    B.tdir[3u * d] = B.rdir[3u * r];
+    B.tdir[3u * d + 1u] = B.rdir[3u * r + 1u];
+    B.tdir[3u * d + 2u] = B.rdir[3u * r + 2u];
   }
 }
 
@@ -607,6 +694,10 @@ static int32_t gd_alloc(th_guided_ws* w, void** p, size_t bytes) {
   return TH_OK;
 }
 
+// Paths per queue segment for a pass over n paths: the first bounce's identity queue hands
+// segment s the paths k with (k / 64) % GD_NSEG == s, at most this many.
+static uint32_t gd_seg_cap(uint32_t n) { return (n + 64u * GD_NSEG - 1u) / (64u * GD_NSEG) * 64u; }
+
 static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
   if (s->gw && s->gw->capacity >= n && s->gw->rcap >= rcap) return TH_OK;
   TH_CHECK(hipStreamSynchronize(s->ctx->stream));
@@ -617,22 +708,26 @@ static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
   w->rcap = rcap;
   GdBufs& b = w->b;
   const size_t f4 = (size_t)n * sizeof(float4), u = (size_t)n * sizeof(uint32_t);
-  const size_t rows = (size_t)((n + 63u) / 64u) * 64u;
+  const size_t rows = (size_t)GD_NSEG * gd_seg_cap(n);  // queue positions (a multiple of 64)
+  const size_t qf4 = rows * sizeof(float4), qu = rows * sizeof(uint32_t);
   int32_t rc;
   if ((rc = gd_alloc(w, (void**)&b.o, f4)) || (rc = gd_alloc(w, (void**)&b.d, f4)) ||
       (rc = gd_alloc(w, (void**)&b.thr, f4)) || (rc = gd_alloc(w, (void**)&b.rad, f4)) ||
       (rc = gd_alloc(w, (void**)&b.acc, f4)) || (rc = gd_alloc(w, (void**)&b.rng, u)) ||
       (rc = gd_alloc(w, (void**)&b.hit, (size_t)n * sizeof(uint2))) ||
-      (rc = gd_alloc(w, (void**)&b.vp, f4)) || (rc = gd_alloc(w, (void**)&b.vn, f4)) ||
-      (rc = gd_alloc(w, (void**)&b.vw, f4)) || (rc = gd_alloc(w, (void**)&b.sho, f4)) ||
+      (rc = gd_alloc(w, (void**)&b.vp, qf4)) || (rc = gd_alloc(w, (void**)&b.vn, qf4)) ||
+      (rc = gd_alloc(w, (void**)&b.vw, qf4)) || (rc = gd_alloc(w, (void**)&b.sho, f4)) ||
       (rc = gd_alloc(w, (void**)&b.sh0, f4)) || (rc = gd_alloc(w, (void**)&b.c0, f4)) ||
       (rc = gd_alloc(w, (void**)&b.sh1, f4)) || (rc = gd_alloc(w, (void**)&b.c1, f4)) ||
-      (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, u)) ||
-      (rc = gd_alloc(w, (void**)&b.q1, u)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * u)) ||
-      (rc = gd_alloc(w, (void**)&b.qg, u)) || (rc = gd_alloc(w, (void**)&b.cnt, GD_C(32) * sizeof(uint32_t))) ||
+      (rc = gd_alloc(w, (void**)&b.vis, 2 * u)) || (rc = gd_alloc(w, (void**)&b.q0, qu)) ||
+      (rc = gd_alloc(w, (void**)&b.q1, qu)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * qu)) ||
+      (rc = gd_alloc(w, (void**)&b.qg, qu)) ||
+      (rc = gd_alloc(w, (void**)&b.cnt, GD_C(GD_CNT_LINES) * sizeof(uint32_t))) ||
       (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.rX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
       (rc = gd_alloc(w, (void**)&b.rdir, (size_t)rcap * 3 * sizeof(float))) ||
+      (rc = gd_alloc(w, (void**)&b.tX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
+      (rc = gd_alloc(w, (void**)&b.tdir, (size_t)rcap * 3 * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.rw, (size_t)rcap * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.rbeta, (size_t)rcap * sizeof(float4))) ||
       (rc = gd_alloc(w, (void**)&b.rL, (size_t)rcap * sizeof(float4))) ||
@@ -669,21 +764,25 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   const bool train = opt->train != 0u;
   const uint32_t max_chunks = std::max(steps, wrate);
   if (max_chunks > GD_MAX_CHUNKS) return th_set_error(TH_E_INVALID, "at most 16 steps per pass");
-  // one chunk per step: capacity 1.5 x batch absorbs the spread of the random split
-  const uint32_t chunk_cap = (batch + batch / 2u + 31u) / 32u * 32u;
+  // one chunk per step: capacity 1.5 x batch absorbs the spread of the random split (and of
+  // the segments' shares: GD_NSEG pieces of a multiple of 32 records each)
+  const uint32_t chunk_cap = (batch + batch / 2u + 32u * GD_NSEG - 1u) / (32u * GD_NSEG) * (32u * GD_NSEG);
   if (train && (uint64_t)chunk_cap * max_chunks > (1u << 26)) return th_set_error(TH_E_INVALID, "batch x steps too large");
   const uint32_t rcap = train ? chunk_cap * max_chunks : 32u;
   TH_CHECK(hipSetDevice(ctx->device));
   if ((rc = gd_reserve(s, npix, rcap))) return rc;
   th_guided_ws* w = s->gw;
   GdBufs B = w->b;
+  B.seg_cap = gd_seg_cap(npix);
   DevScene S = dev_scene(s);
   const uint32_t block = 64;
   const uint32_t depth = 3u * s->depth4 + 2u;
   const size_t lds = (size_t)depth * (block / 4) * sizeof(uint2);
-  const uint32_t grid_trace = std::min<uint32_t>((npix * 4u + block - 1) / block, 8192u);
-  const uint32_t grid_scalar = std::min<uint32_t>((npix + block - 1) / block, 4096u);
-  const uint32_t grid_shadow = std::min<uint32_t>((2u * npix * 4u + block - 1) / block, 8192u);
+  // every grid a multiple of GD_NSEG: block b serves queue segment b % GD_NSEG
+  auto segs = [](uint32_t g) { return (g + GD_NSEG - 1u) / GD_NSEG * GD_NSEG; };
+  const uint32_t grid_trace = segs(std::min<uint32_t>((npix * 4u + block - 1) / block, 8192u));
+  const uint32_t grid_scalar = segs(std::min<uint32_t>((npix + block - 1) / block, 4096u));
+  const uint32_t grid_shadow = segs(std::min<uint32_t>((2u * npix * 4u + block - 1) / block, 8192u));
   GdArgs A;
   memset(&A, 0, sizeof A);
   A.n = npix;
@@ -759,7 +858,8 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
       if (A.defer) {
         if (A.alpha > 0.0f) {
           mark(ev_infer);
-          rc = guide_infer(g, B.cnt + GD_C(3), npix, B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm);
+          rc = guide_infer(g, B.cnt + GD_QC(3u, 0u), GD_QC(3u, 1u) - GD_QC(3u, 0u), GD_NSEG, B.seg_cap,
+                           B.vp, B.vn, B.vw, s->root_min, s->root_max, B.O, sm);
           mark(ev_infer);
           if (rc) {
             (void)end_timed(ctx, evp);
@@ -789,7 +889,7 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
         if (cnt < 1024u) continue;  // too few samples for a step
         const size_t off = (size_t)j * chunk_cap;
         mark(ev_train);
-        rc = guide_train(g, B.rX + off * 64u, B.rdir + off * 3u, B.rw + off, cnt);
+        rc = guide_train(g, B.tX + off * 64u, B.tdir + off * 3u, B.rw + off, cnt);
         mark(ev_train);
         if (rc) {
           (void)end_timed(ctx, evp);
