@@ -7,7 +7,10 @@
 // hit, 4 lanes per ray) -> finish (adds visible direct light in the reference's order,
 // completes samples, regenerates the next camera path of the same pixel from the same
 // random stream, and compacts live paths into the next queue with a wave ballot plus
-// one atomic per wave).  A path slot owns one pixel (reference streams: all its samples
+// one atomic per wave).  Queues are split into WF_NSEG segments with a counter each:
+// block b reads and appends to segment b % WF_NSEG, so a slot stays in its segment (the
+// slots k with (k / 64) % WF_NSEG equal to it) and the append atomics spread over as
+// many addresses (with one counter per queue they serialise in the memory system).  A path slot owns one pixel (reference streams: all its samples
 // in stream order) or one of G sample groups of a pixel (per-sample streams, combined
 // in a fixed tree order by the resolve kernel).  Every value a stage hands to the next
 // goes through HBM in structure-of-arrays form; the deferred direct-light terms are the
@@ -23,6 +26,9 @@ namespace {
 #define WF_PEND_LIGHT 8u
 #define WF_PEND_ENV 16u
 #define WF_C(c) ((c) * 32u)  // queue counters 128 bytes apart: no two share a cache line
+#define WF_NSEG 16u
+#define WF_QC(q, sg) WF_C(4u + (q) * WF_NSEG + (sg))  // queue q: 0, 1 live, 2 shadow
+#define WF_CNT_LINES (4u + 3u * WF_NSEG)
 
 struct WfBufs {
   float4 *o, *d, *thr, *rad, *acc;  // o.w = prev bsdf pdf, d.w = flag bits
@@ -30,8 +36,9 @@ struct WfBufs {
   uint2* hit;                       // t bits, slot
   float4 *sho, *sh0, *c0, *sh1, *c1;
   uint32_t* vis;                     // 2 per slot
-  uint32_t *q0, *q1, *qs;            // live queues (ping-pong), shadow queue
-  uint32_t* cnt;                     // [0] live0 [1] live1 [2] shadow
+  uint32_t *q0, *q1, *qs;            // live queues (ping-pong), shadow queue; in segments
+  uint32_t* cnt;                     // WF_QC counters
+  uint32_t seg_cap;                  // slots per segment of q0, q1; qs holds twice that
 };
 
 struct WfArgs {
@@ -76,17 +83,30 @@ __device__ __forceinline__ void wf_push(bool live, uint32_t value, uint32_t* que
 }
 
 __global__ void wf_reset(WfBufs B) {
-  if (threadIdx.x < 4u) B.cnt[WF_C(threadIdx.x)] = 0u;
+  for (uint32_t c = threadIdx.x; c < WF_CNT_LINES; c += blockDim.x) B.cnt[WF_C(c)] = 0u;
+}
+
+// Appends: `la` lanes' values a, then `lb` lanes' values b, with one atomic.
+__device__ __forceinline__ void wf_push2(bool la, uint32_t a, bool lb, uint32_t b, uint32_t* queue,
+                                         uint32_t* count) {
+  uint64_t ma = __ballot(la), mb = __ballot(lb);
+  if ((ma | mb) == 0) return;
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t first = (uint32_t)__ffsll((unsigned long long)(ma | mb)) - 1u;
+  uint32_t na = (uint32_t)__popcll(ma);
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(count, na + (uint32_t)__popcll(mb));
+  base = __shfl(base, (int)first, 64);
+  uint64_t lt = (1ull << lane) - 1ull;
+  if (la) queue[base + (uint32_t)__popcll(ma & lt)] = a;
+  if (lb) queue[base + na + (uint32_t)__popcll(mb & lt)] = b;
 }
 
 // ---------------------------------------------------------------- generate
 __global__ void __launch_bounds__(256) wf_generate(WfBufs B, WfArgs A, TH_GpuRenderParams P,
                                                    TH_GpuCamera cam) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0) {
-    B.cnt[WF_C(1)] = 0;
-    B.cnt[WF_C(2)] = 0;
-  }
+  const uint32_t sg = (k >> 6) % WF_NSEG;  // this wave's segment (wf_reset cleared the counters)
   bool live = false;
   if (k < A.n_slots) {
     uint32_t pixel = k / A.G, j = k % A.G;
@@ -99,7 +119,7 @@ __global__ void __launch_bounds__(256) wf_generate(WfBufs B, WfArgs A, TH_GpuRen
       B.st[k] = make_uint4(0u, smp, 0u, pixel);
     }
   }
-  wf_push(live, k, B.q0, &B.cnt[WF_C(0)]);
+  wf_push(live, k, B.q0 + (size_t)sg * B.seg_cap, &B.cnt[WF_QC(0u, sg)]);
 }
 
 // ---------------------------------------------------------------- extend
@@ -107,14 +127,17 @@ template <int TMODE>
 __global__ void __launch_bounds__(64) wf_extend(WfBufs B, DevScene S, TH_GpuRenderParams P,
                                                 uint32_t live_idx, uint32_t per_quad) {
   extern __shared__ uint2 lds_stack[];
-  const uint32_t* q = live_idx ? B.q1 : B.q0;
-  uint32_t n = B.cnt[WF_C(live_idx)];
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[WF_C(2)] = 0;  // shadow queue of this iteration
+  const uint32_t sg = blockIdx.x % WF_NSEG;
+  const uint32_t* q = (live_idx ? B.q1 : B.q0) + (size_t)sg * B.seg_cap;
+  uint32_t n = B.cnt[WF_QC(live_idx, sg)];
+  if (blockIdx.x == 0)  // shadow queue of this iteration
+    for (uint32_t c = threadIdx.x; c < WF_NSEG; c += blockDim.x) B.cnt[WF_QC(2u, c)] = 0;
   uint32_t lanes_per = per_quad ? 4u : 1u;
   Stack st{lds_stack + (per_quad ? (threadIdx.x >> 2) : threadIdx.x),
            (int)(per_quad ? (blockDim.x >> 2) : blockDim.x), 0};
-  uint32_t stride = gridDim.x * blockDim.x / lanes_per;
-  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / lanes_per; ; i += stride) {
+  const uint32_t rays = blockDim.x / lanes_per;
+  uint32_t stride = (gridDim.x / WF_NSEG) * rays;
+  for (uint32_t i = (blockIdx.x / WF_NSEG) * rays + threadIdx.x / lanes_per; ; i += stride) {
     // the loop bound is uniform per quad, so quads stay converged for traverse_quad
     bool act = i < n;
     if (!__any(act)) break;
@@ -140,14 +163,17 @@ __global__ void __launch_bounds__(64) wf_extend(WfBufs B, DevScene S, TH_GpuRend
 // One iteration of the reference loop body (wgsl:817-911) for the hit found by extend.
 __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRenderParams P,
                                                const float* __restrict__ srgb_lut, uint32_t live_idx) {
-  const uint32_t* q = live_idx ? B.q1 : B.q0;
-  uint32_t n = B.cnt[WF_C(live_idx)];
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.cnt[WF_C(live_idx ^ 1u)] = 0;  // next live queue
+  const uint32_t sg = blockIdx.x % WF_NSEG;
+  const uint32_t* q = (live_idx ? B.q1 : B.q0) + (size_t)sg * B.seg_cap;
+  uint32_t n = B.cnt[WF_QC(live_idx, sg)];
+  if (blockIdx.x == 0)  // next live queue
+    for (uint32_t c = threadIdx.x; c < WF_NSEG; c += blockDim.x) B.cnt[WF_QC(live_idx ^ 1u, c)] = 0;
+  uint32_t* qs = B.qs + (size_t)sg * 2u * B.seg_cap;
   Stack st{nullptr, 0, 0};
   LaneCounters<false> lc;
   Integrator<1, false> it{S, P, srgb_lut, st, lc};
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; ; i += stride) {
+  uint32_t stride = (gridDim.x / WF_NSEG) * blockDim.x;
+  for (uint32_t i = (blockIdx.x / WF_NSEG) * blockDim.x + threadIdx.x; ; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
     bool sh_light = false, sh_env = false;
@@ -263,8 +289,7 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
       B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
       B.st[k] = make_uint4(rng, s4.y, bounce, s4.w);
     }
-    wf_push(sh_light, 2u * k, B.qs, &B.cnt[WF_C(2)]);
-    wf_push(sh_env, 2u * k + 1u, B.qs, &B.cnt[WF_C(2)]);
+    wf_push2(sh_light, 2u * k, sh_env, 2u * k + 1u, qs, &B.cnt[WF_QC(2u, sg)]);
   }
 }
 
@@ -273,16 +298,19 @@ template <int TMODE>
 __global__ void __launch_bounds__(64) wf_shadow(WfBufs B, DevScene S, TH_GpuRenderParams P,
                                                 uint32_t per_quad) {
   extern __shared__ uint2 lds_stack[];
-  uint32_t n = B.cnt[WF_C(2)];
+  const uint32_t sg = blockIdx.x % WF_NSEG;
+  const uint32_t* qs = B.qs + (size_t)sg * 2u * B.seg_cap;
+  uint32_t n = B.cnt[WF_QC(2u, sg)];
   uint32_t lanes_per = per_quad ? 4u : 1u;
   Stack st{lds_stack + (per_quad ? (threadIdx.x >> 2) : threadIdx.x),
            (int)(per_quad ? (blockDim.x >> 2) : blockDim.x), 0};
-  uint32_t stride = gridDim.x * blockDim.x / lanes_per;
-  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / lanes_per; ; i += stride) {
+  const uint32_t rays = blockDim.x / lanes_per;
+  uint32_t stride = (gridDim.x / WF_NSEG) * rays;
+  for (uint32_t i = (blockIdx.x / WF_NSEG) * rays + threadIdx.x / lanes_per; ; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
     if (act) {
-      uint32_t e = B.qs[i], k = e >> 1;
+      uint32_t e = qs[i], k = e >> 1;
       float4 so = B.sho[k], sd = (e & 1u) ? B.sh1[k] : B.sh0[k];
       v3 o = th::mk(so.x, so.y, so.z), d = th::mk(sd.x, sd.y, sd.z);
       float limit = sd.w, t;
@@ -304,11 +332,12 @@ __global__ void __launch_bounds__(64) wf_shadow(WfBufs B, DevScene S, TH_GpuRend
 // ---------------------------------------------------------------- finish
 __global__ void __launch_bounds__(64) wf_finish(WfBufs B, WfArgs A, TH_GpuRenderParams P,
                                                 TH_GpuCamera cam, uint32_t live_idx) {
-  const uint32_t* q = live_idx ? B.q1 : B.q0;
-  uint32_t* qn = live_idx ? B.q0 : B.q1;
-  uint32_t n = B.cnt[WF_C(live_idx)];
-  uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; ; i += stride) {
+  const uint32_t sg = blockIdx.x % WF_NSEG;
+  const uint32_t* q = (live_idx ? B.q1 : B.q0) + (size_t)sg * B.seg_cap;
+  uint32_t* qn = (live_idx ? B.q0 : B.q1) + (size_t)sg * B.seg_cap;
+  uint32_t n = B.cnt[WF_QC(live_idx, sg)];
+  uint32_t stride = (gridDim.x / WF_NSEG) * blockDim.x;
+  for (uint32_t i = (blockIdx.x / WF_NSEG) * blockDim.x + threadIdx.x; ; i += stride) {
     bool act = i < n;
     if (!__any(act)) break;
     bool live = false;
@@ -355,7 +384,7 @@ __global__ void __launch_bounds__(64) wf_finish(WfBufs B, WfArgs A, TH_GpuRender
         live = true;
       }
     }
-    wf_push(live, k, qn, &B.cnt[WF_C(live_idx ^ 1u)]);
+    wf_push(live, k, qn, &B.cnt[WF_QC(live_idx ^ 1u, sg)]);
   }
 }
 
@@ -401,7 +430,8 @@ struct th_wavefront {
 // host code never uses the array's name as a device address.
 #define WF_STATIC_SLOTS 65536u
 #define WF_SLOT_BYTES (10u * 16u + 16u + 8u + 8u + 4u + 4u + 8u)  // o..c1, st, hit, vis, q0, q1, qs
-#define WF_STATIC_BYTES ((size_t)WF_STATIC_SLOTS * WF_SLOT_BYTES + WF_C(16) * sizeof(uint32_t))
+#define WF_STATIC_BYTES ((size_t)WF_STATIC_SLOTS * WF_SLOT_BYTES + WF_C(WF_CNT_LINES) * sizeof(uint32_t))
+static_assert(WF_STATIC_SLOTS % (64u * WF_NSEG) == 0u, "the static queues hold whole segments");
 static __device__ __align__(256) unsigned char scratch[WF_STATIC_BYTES];
 
 int32_t th_wavefront_scratch_init(th_context* ctx) {
@@ -440,7 +470,7 @@ static WfBufs wf_static_bufs(void* base) {
   b.q0 = (uint32_t*)take(n * 4);
   b.q1 = (uint32_t*)take(n * 4);
   b.qs = (uint32_t*)take(n * 8);
-  b.cnt = (uint32_t*)take(WF_C(16) * sizeof(uint32_t));
+  b.cnt = (uint32_t*)take(WF_C(WF_CNT_LINES) * sizeof(uint32_t));
   return b;
 }
 static_assert(WF_STATIC_BYTES % 256u == 0u, "scratch layout assumes 256-byte multiples");
@@ -457,6 +487,10 @@ static int32_t wf_alloc(th_wavefront* wf, void** p, size_t bytes) {
   return TH_OK;
 }
 
+// Slots per queue segment for n slots: segment s holds the slots k with
+// (k / 64) % WF_NSEG == s, at most this many.
+static uint32_t wf_seg_cap(uint32_t n) { return (n + 64u * WF_NSEG - 1u) / (64u * WF_NSEG) * 64u; }
+
 static int32_t wf_reserve(th_scene* s, uint32_t n) {
   if (s->wf && s->wf->capacity >= n) return TH_OK;
   th_wavefront_free(s->wf);
@@ -465,6 +499,7 @@ static int32_t wf_reserve(th_scene* s, uint32_t n) {
   wf->capacity = n;
   WfBufs& b = wf->b;
   size_t f4 = (size_t)n * sizeof(float4);
+  const size_t qn = (size_t)WF_NSEG * wf_seg_cap(n);  // queue entries (whole segments)
   int32_t rc;
   if ((rc = wf_alloc(wf, (void**)&b.o, f4)) || (rc = wf_alloc(wf, (void**)&b.d, f4)) ||
       (rc = wf_alloc(wf, (void**)&b.thr, f4)) || (rc = wf_alloc(wf, (void**)&b.rad, f4)) ||
@@ -474,10 +509,10 @@ static int32_t wf_reserve(th_scene* s, uint32_t n) {
       (rc = wf_alloc(wf, (void**)&b.c0, f4)) || (rc = wf_alloc(wf, (void**)&b.sh1, f4)) ||
       (rc = wf_alloc(wf, (void**)&b.c1, f4)) ||
       (rc = wf_alloc(wf, (void**)&b.vis, 2 * (size_t)n * sizeof(uint32_t))) ||
-      (rc = wf_alloc(wf, (void**)&b.q0, (size_t)n * sizeof(uint32_t))) ||
-      (rc = wf_alloc(wf, (void**)&b.q1, (size_t)n * sizeof(uint32_t))) ||
-      (rc = wf_alloc(wf, (void**)&b.qs, 2 * (size_t)n * sizeof(uint32_t))) ||
-      (rc = wf_alloc(wf, (void**)&b.cnt, WF_C(16) * sizeof(uint32_t)))) {
+      (rc = wf_alloc(wf, (void**)&b.q0, qn * sizeof(uint32_t))) ||
+      (rc = wf_alloc(wf, (void**)&b.q1, qn * sizeof(uint32_t))) ||
+      (rc = wf_alloc(wf, (void**)&b.qs, 2 * qn * sizeof(uint32_t))) ||
+      (rc = wf_alloc(wf, (void**)&b.cnt, WF_C(WF_CNT_LINES) * sizeof(uint32_t)))) {
     th_wavefront_free(wf);
     return rc;
   }
@@ -537,18 +572,22 @@ extern "C" int32_t th_render_wavefront_static(th_context* ctx, th_scene* s, cons
 }
 
 static int32_t wf_launch(th_context* ctx, th_scene* s, const TH_GpuRenderParams* p, const TH_GpuCamera* cam,
-                         const th_render_options* o, const WfBufs& B, uint32_t n, uint32_t G) {
+                         const th_render_options* o, const WfBufs& Bq, uint32_t n, uint32_t G) {
   uint32_t npix = p->width * p->height;
+  WfBufs B = Bq;
+  B.seg_cap = wf_seg_cap(n);
   WfArgs A{n, G, o->rng_mode == TH_RNG_PER_SAMPLE ? 1u : 0u};
   DevScene S = dev_scene(s);
   const bool quad = o->traversal == TH_TRAVERSE_WIDE && s->node_count > 0;
   const uint32_t block = 64;
   uint32_t depth = quad ? 3u * s->depth4 + 2u : std::max<uint32_t>(s->max_depth + 1u, 2u);
   size_t lds = (size_t)depth * (quad ? block / 4 : block) * sizeof(uint2);
-  // persistent grids: enough waves to fill the device, grid-stride over the queues
-  uint32_t grid_trace = std::min<uint32_t>((n * (quad ? 4u : 1u) + block - 1) / block, 8192u);
-  uint32_t grid_scalar = std::min<uint32_t>((n + block - 1) / block, 4096u);
-  uint32_t grid_shadow = std::min<uint32_t>((2u * n * (quad ? 4u : 1u) + block - 1) / block, 8192u);
+  // persistent grids: enough waves to fill the device, grid-stride over the queues; each a
+  // multiple of WF_NSEG (block b serves segment b % WF_NSEG)
+  auto segs = [](uint32_t g) { return (g + WF_NSEG - 1u) / WF_NSEG * WF_NSEG; };
+  uint32_t grid_trace = segs(std::min<uint32_t>((n * (quad ? 4u : 1u) + block - 1) / block, 8192u));
+  uint32_t grid_scalar = segs(std::min<uint32_t>((n + block - 1) / block, 4096u));
+  uint32_t grid_shadow = segs(std::min<uint32_t>((2u * n * (quad ? 4u : 1u) + block - 1) / block, 8192u));
   uint32_t per_sample_count = A.per_sample ? (p->samples + G - 1) / G : p->samples;
   uint32_t iters = per_sample_count * std::max<uint32_t>(p->max_bounces, 1u);
   hipLaunchKernelGGL(wf_reset, dim3(1), dim3(64), 0, ctx->stream, B);
