@@ -487,6 +487,13 @@ int32_t th_guide_set_params(th_guide* guide, const float* params, uint32_t count
 int32_t th_guide_set_hyper(th_guide* guide, float lr, float beta1, float beta2, float eps);
 /* Network outputs (count x 40) for host-supplied features (count x 48). */
 int32_t th_guide_forward(th_guide* guide, const float* features, uint32_t count, float* out);
+/* Network outputs (count x 40) of the guided renderer's fused inference kernel (the
+ * encoding computed on the device, then the four layers) for host-supplied diffuse
+ * vertices: count x 9 floats (position, normal, outgoing direction), with the position
+ * normalised to [0,1]^3 by the box [bmin, bmax] as the renderer normalises it by the
+ * scene bounds.  For checking that kernel; the renderer calls it on its own queues. */
+int32_t th_guide_infer_vertices(th_guide* guide, const float* vertices, uint32_t count,
+                                const float bmin[3], const float bmax[3], float* out);
 /* One Adam step (update != 0) or the gradient only on host-supplied samples: loss
  * -sum_b w_b log q_b(dir_b) / sum_b w_b; loss and gradient (14056 floats) optional. */
 int32_t th_guide_train_batch(th_guide* guide, const float* features, const float* dirs,

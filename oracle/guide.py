@@ -63,6 +63,32 @@ def encode(p01, n, wo):
     return np.stack(cols, 1)
 
 
+def encode_device_rounding(p01, n, wo):
+    """`encode` with the float32 roundings of the device's arguments (inputs taken as
+    float32): the angle f32(f32(6.2831855 * 2^k) * x), handed to sine and cosine in
+    revolutions as f32(angle * f32(1 / (2 pi))); the bin offset d = f32(f32(u - c) * 4) of
+    u = f32(0.5 * f32(c + 1)), and the exponent f32(f32(-0.5 * d) * d) handed to the base-2
+    exponential as f32(exponent * f32(log2 e)).  Each function is then evaluated exactly,
+    so the features differ from `encode` only by those argument roundings (a few 1e-6) --
+    which matters where a feature lies that close to a bfloat16 rounding boundary."""
+    f = np.float32
+    p01, n, wo = (np.atleast_2d(np.asarray(a, f)) for a in (p01, n, wo))
+    cols = []
+    for a in range(3):
+        for k in range(4):
+            ang = f(f(6.2831855) * f(2.0 ** k)) * p01[:, a]
+            rev = (ang * f(0.15915494)).astype(np.float64)
+            cols += [np.sin(2.0 * np.pi * rev), np.cos(2.0 * np.pi * rev)]
+    dc = np.concatenate([n, wo], 1)
+    for a in range(6):
+        u = f(0.5) * (dc[:, a] + f(1.0))
+        for j in range(4):
+            d = (u - f((j + 0.5) * 0.25)) * f(4.0)
+            x = (f(-0.5) * d) * d
+            cols.append(np.exp2((x * f(1.4426950408889634)).astype(np.float64)))
+    return np.stack(cols, 1)
+
+
 def forward(params, X, emulate_bf16=False):
     """Returns (O, cache).  With emulate_bf16 the inputs, weights and hidden activations
     are rounded to bfloat16 as on the device (accumulation stays exact here)."""

@@ -67,6 +67,40 @@ def test_forward_matches_oracle(gpu_ctx, n):
     g.close()
 
 
+@pytest.mark.parametrize("n", [1, 37, 4096, 70000])
+def test_fused_inference_matches_oracle(gpu_ctx, n):
+    """The guided renderer's inference kernel (encoding on the device, then the four
+    layers) on host-supplied vertices against the oracle's encoding and forward pass.
+    The oracle encodes with the device's float32 argument roundings
+    (encode_device_rounding); the transcendentals themselves are the hardware's, within a
+    few ulp, so a feature rarely rounds to the neighbouring BF16 value -- the same row
+    tolerance as the forward test covers it."""
+    rng = np.random.default_rng(500 + n)
+    g = Guide(gpu_ctx, seed=3)
+    p = rng.normal(0, 0.25, G.N_PARAMS).astype(np.float32)
+    g.set_params(p)
+    bmin = np.array([-4.0, -0.5, -2.5], np.float32)
+    bmax = np.array([4.0, 1.6, 2.0], np.float32)
+    pos = (bmin + rng.uniform(size=(n, 3)) * (bmax - bmin)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    wo = rng.normal(size=(n, 3))
+    wo = (wo / np.linalg.norm(wo, axis=1, keepdims=True)).astype(np.float32)
+    print(f"case fused inference n={n}", flush=True)
+    O = g.infer_vertices(pos, nrm, wo, bmin, bmax)
+    # the device's normalisation: (p - bmin) * f32(1 / (bmax - bmin))
+    scale = (np.float32(1.0) / (bmax - bmin)).astype(np.float32)
+    p01 = ((pos - bmin) * scale).astype(np.float32)
+    ref, _ = G.forward(p, G.encode_device_rounding(p01, nrm, wo), emulate_bf16=True)
+    err = np.abs(O - ref)
+    row_ok = np.all(err <= 1e-4 * (1 + np.abs(ref)), axis=1)
+    print(f"  rows within 1e-4: {row_ok.mean():.5f}, max err {err.max():.3g}")
+    assert row_ok.mean() >= 0.99
+    assert np.all(err <= 0.1 * (1 + np.abs(ref))), float(err.max())
+    assert float(err.mean()) < 1e-3
+    g.close()
+
+
 @pytest.mark.parametrize("n", [64, 5000, 262144])
 def test_gradient_and_adam_match_oracle(gpu_ctx, n):
     rng = np.random.default_rng(100 + n)

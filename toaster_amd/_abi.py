@@ -212,6 +212,7 @@ SIGNATURES = {
     "th_guide_set_params": (C.c_int32, [P, P, u32]),
     "th_guide_set_hyper": (C.c_int32, [P, f32, f32, f32, f32]),
     "th_guide_forward": (C.c_int32, [P, P, u32, P]),
+    "th_guide_infer_vertices": (C.c_int32, [P, P, u32, P, P, P]),
     "th_guide_train_batch": (C.c_int32, [P, P, P, P, u32, C.c_int32, P, P]),
     "th_guide_mfma_selftest": (C.c_int32, [P, P, P, P]),
     "th_bf16_selftest": (C.c_int32, [P, P, P]),

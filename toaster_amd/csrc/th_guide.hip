@@ -529,16 +529,101 @@ __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
 
 // Fused inference over a queue of deferred diffuse vertices, one wave per 16-row tile
 // and persistent over the queue.  Row i of the queue is the vertex (p, n, wo) stored in
-// queue order (vp, vn, vw).  Lane (r, g) encodes a quarter of row r's position sin/cos
-// pairs and a quarter of its direction bins into an LDS tile; the four layers then run on MFMA with every weight
-// fragment held in registers (loaded once per wave) and activations passed through the
-// same tile (BF16, row stride 72 elements = 144 bytes, 16-byte aligned rows); each row's
-// 40 outputs are written in FP32.
-#define TG_PAIRS_PER_GROUP (3 * TG_POS_OCT / 4)  // position sin/cos pairs per lane group
-#define TG_BINS_PER_GROUP (6 * TG_DIR_BINS / 4)  // direction bins per lane group
-static_assert(TG_PAIRS_PER_GROUP * 4 == 3 * TG_POS_OCT && TG_BINS_PER_GROUP * 4 == 6 * TG_DIR_BINS &&
-                  TG_BINS_PER_GROUP % 2 == 0,
-              "the encoding splits evenly over four lane groups");
+// queue order (vp, vn, vw).  The products run with the operands of k_gemm_rows swapped:
+// the A operand is 16 weight rows (out-major, every fragment held in registers, loaded
+// once per wave) and the B operand 32 inputs of the tile's 16 rows, so lane (r, g) ends
+// each 16-output block nb holding outputs nb*16 + 4g .. 4g + 3 of row r -- four
+// consecutive values of one row, written to the LDS activation tile with one 8-byte
+// store (BF16, row stride 72 elements) and to the output with one 16-byte store.  The
+// first layer's B fragment needs no tile: lane (r, g) holds inputs 8g .. 8g + 7 and
+// 32 + 8g .. 32 + 8g + 7 of row r, which it encodes itself -- lanes g < 3 the sin/cos
+// pairs of position axis g, lane g = 3 the bins of n.x and n.y, lanes g = 0 and 1 also
+// the bins of (n.z, wo.x) and (wo.y, wo.z), the rest zero padding -- with the same
+// feature functions as the training records.  Loads are clamped into the queue, and
+// stores cover whole tiles (segments hold a multiple of 64 rows) through a buffer
+// resource, the padding outputs 40..47 sent past its end (discarded by the hardware),
+// so the loop has no branch and the next tile's loads never wait for a store.
+struct InferNet {
+  tg_bf16x8 W[4][4][2];  // W[L][nb][kb]: weight rows nb*16 + r, inputs kb*32 + 8g .. 8g + 7
+  float4 bq[4][4];       // biases of outputs nb*16 + 4g .. 4g + 3
+};
+
+__device__ __forceinline__ void infer_tile(const InferNet& net, uint16_t* T, uint32_t r, uint32_t g,
+                                           uint32_t base, uint32_t count, float4 a, float4 b, float4 c,
+                                           float4 bmin, float4 bscale, __amdgpu_buffer_rsrc_t orsrc) {
+  typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const bool in = base + r < count;
+  const v3 p01 = in ? th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z)
+                    : th::mk(0.5f, 0.5f, 0.5f);
+  const v3 n = in ? th::mk(b.x, b.y, b.z) : th::mk(0.0f, 1.0f, 0.0f);
+  const v3 wo = in ? th::mk(c.x, c.y, c.z) : n;
+  // position axis g (lanes g < 3) and the two direction components whose bins this
+  // lane encodes
+  const float xg = g == 0u ? p01.x : (g == 1u ? p01.y : p01.z);
+  const float u0 = g == 0u ? n.z : (g == 1u ? wo.y : n.x);
+  const float u1 = g == 0u ? wo.x : (g == 1u ? wo.z : n.y);
+  const v3 px = th::mk(xg, 0.0f, 0.0f), nu0 = th::mk(u0, 0.0f, 0.0f), nu1 = th::mk(u1, 0.0f, 0.0f);
+  uint32_t pos[4], bins[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float s, cs;
+    tg::encode_pos_pair(2 * t, px, &s, &cs);  // axis 0 of px = axis g, octave t
+    pos[t] = tg::f2bf2(s, cs);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {  // feature 24 + b reads component 0 of its normal argument
+    bins[t] = tg::f2bf2(tg::encode_feature(6 * TG_POS_OCT + 2 * t, p01, nu0, wo),
+                        tg::encode_feature(6 * TG_POS_OCT + 2 * t + 1, p01, nu0, wo));
+    bins[2 + t] = tg::f2bf2(tg::encode_feature(6 * TG_POS_OCT + 2 * t, p01, nu1, wo),
+                            tg::encode_feature(6 * TG_POS_OCT + 2 * t + 1, p01, nu1, wo));
+  }
+  v4u32 x0, x1;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    x0[t] = g < 3u ? pos[t] : bins[t];
+    x1[t] = g < 2u ? bins[t] : 0u;
+  }
+  tg_bf16x8 b0 = __builtin_bit_cast(tg_bf16x8, x0), b1 = __builtin_bit_cast(tg_bf16x8, x1);
+#pragma unroll
+  for (int L = 0; L < 4; ++L) {
+    if (L > 0) {
+      b0 = lds8(T + r * 72 + g * 8);
+      b1 = lds8(T + r * 72 + 32 + g * 8);
+    }
+    const int nbn = L < 3 ? 4 : 3;  // the last layer has 40 outputs: three 16-wide blocks
+    tg_f32x4 acc[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      if (nb < nbn) {
+        acc[nb] = mfma(net.W[L][nb][0], b0, tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        acc[nb] = mfma(net.W[L][nb][1], b1, acc[nb]);
+      }
+    }
+    if (L < 3) {
+      __syncthreads();  // every lane has read the tile before it is overwritten
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        const float v0 = acc[nb][0] + net.bq[L][nb].x, v1 = acc[nb][1] + net.bq[L][nb].y;
+        const float v2 = acc[nb][2] + net.bq[L][nb].z, v3 = acc[nb][3] + net.bq[L][nb].w;
+        *(uint2*)(T + r * 72 + nb * 16 + 4 * g) =
+            make_uint2(tg::f2bf2(v0 > 0.0f ? v0 : 0.0f, v1 > 0.0f ? v1 : 0.0f),
+                       tg::f2bf2(v2 > 0.0f ? v2 : 0.0f, v3 > 0.0f ? v3 : 0.0f));
+      }
+      __syncthreads();
+    } else {
+      const uint32_t ob = ((base + r) * TG_OUT + 4u * g) * 4u;
+#pragma unroll
+      for (int nb = 0; nb < 3; ++nb) {
+        const v4f v = {acc[nb][0] + net.bq[3][nb].x, acc[nb][1] + net.bq[3][nb].y,
+                       acc[nb][2] + net.bq[3][nb].z, acc[nb][3] + net.bq[3][nb].w};
+        const uint32_t off = (nb < 2 || g < 2u) ? ob + (uint32_t)nb * 64u : 0x80000000u;
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orsrc, off, 0, 0);
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__ woi,
                                                     const float* __restrict__ bias,
                                                     const uint32_t* __restrict__ count_ptr,
@@ -556,91 +641,38 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
   vn += (size_t)sg * seg_cap;
   vw += (size_t)sg * seg_cap;
   O += (size_t)sg * seg_cap * TG_OUT;
+  // this segment's rows of the output (seg_cap * TG_OUT floats, below 2^32 bytes)
+  const __amdgpu_buffer_rsrc_t orsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)O, (short)0, (int)(seg_cap * (uint32_t)TG_OUT * 4u), 0x00020000);
   const uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
-  tg_bf16x8 W[4][4][2];
-  float bv[4][4];
+  InferNet net;
 #pragma unroll
   for (int L = 0; L < 4; ++L)
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
-      bv[L][nb] = bias[L * 64 + nb * 16 + r];
+      if (L < 3 || nb < 3) net.bq[L][nb] = *(const float4*)(bias + L * 64 + nb * 16 + 4 * g);
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb) W[L][nb][kb] = ld8(woi + L * 4096 + (nb * 16 + r) * 64 + kb * 32 + g * 8);
+      for (int kb = 0; kb < 2; ++kb)
+        net.W[L][nb][kb] = ld8(woi + L * 4096 + (nb * 16 + r) * 64 + kb * 32 + g * 8);
     }
-  // the next tile's vertex rows are loaded while the current tile runs its layers
-  const float4 zero4 = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
-  uint32_t base = jb * 16u;
-  float4 na = zero4, nbv = zero4, nc = zero4;
-  if (base + r < count) {
-    na = vp[base + r];
-    nbv = vn[base + r];
-    nc = vw[base + r];
-  }
-  for (; base < count; base += nbs * 16u) {
-    const uint32_t row = base + r;
-    const float4 a = na, b = nbv, c = nc;
-    const uint32_t nrow = row + nbs * 16u;
-    if (nrow < count) {
-      na = vp[nrow];
-      nbv = vn[nrow];
-      nc = vw[nrow];
-    }
-    v3 p01 = th::mk(0.5f, 0.5f, 0.5f), n = th::mk(0.0f, 1.0f, 0.0f), wo = n;
-    if (row < count) {
-      p01 = th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z);
-      n = th::mk(b.x, b.y, b.z);
-      wo = th::mk(c.x, c.y, c.z);
-    }
-    // every lane group does the same work: a quarter of the position sin/cos pairs and a
-    // quarter of the direction bins (no divergence between the two kinds of feature)
-#pragma unroll
-    for (int t = 0; t < TG_PAIRS_PER_GROUP; ++t) {
-      const int j = 2 * (TG_PAIRS_PER_GROUP * (int)g + t);
-      float s, c;
-      tg::encode_pos_pair(j, p01, &s, &c);
-      *(uint32_t*)(T + r * 72 + j) = tg::f2bf2(s, c);
-    }
-#pragma unroll
-    for (int t = 0; t < TG_BINS_PER_GROUP; t += 2) {
-      const int j = 6 * TG_POS_OCT + TG_BINS_PER_GROUP * (int)g + t;
-      *(uint32_t*)(T + r * 72 + j) =
-          tg::f2bf2(tg::encode_feature(j, p01, n, wo), tg::encode_feature(j + 1, p01, n, wo));
-    }
-    *(uint2*)(T + r * 72 + 48 + 4 * g) = make_uint2(0u, 0u);
-    __syncthreads();
-#pragma unroll
-    for (int L = 0; L < 4; ++L) {
-      const tg_bf16x8 a0 = __builtin_bit_cast(tg_bf16x8, *(const uint4*)(T + r * 72 + g * 8));
-      const tg_bf16x8 a1 = __builtin_bit_cast(tg_bf16x8, *(const uint4*)(T + r * 72 + 32 + g * 8));
-      const int nbs = L < 3 ? 4 : 3;  // the last layer has 40 outputs: three 16-wide tiles
-      tg_f32x4 acc[4];
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        if (nb < nbs) {
-          acc[nb] = mfma(a0, W[L][nb][0], tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-          acc[nb] = mfma(a1, W[L][nb][1], acc[nb]);
-        }
-      }
-      __syncthreads();  // every lane has read the tile before it is overwritten
-      if (L < 3) {
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            float v = acc[nb][i] + bv[L][nb];
-            T[(4 * g + i) * 72 + nb * 16 + r] = tg::f2bf(v > 0.0f ? v : 0.0f);
-          }
-        __syncthreads();
-      } else {
-#pragma unroll
-        for (int nb = 0; nb < 3; ++nb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const uint32_t col = nb * 16 + r, rr = base + 4 * g + i;
-            if (col < (uint32_t)TG_OUT && rr < count) O[(size_t)rr * TG_OUT + col] = acc[nb][i] + bv[3][nb];
-          }
-      }
-    }
+  // two register sets of vertex rows, loaded one tile ahead and used in turn (a single
+  // loop-carried set would be copied at the back edge, which waits for its loads)
+  const uint32_t step = nbs * 16u, last = count - 1u;
+  uint32_t base = jb * 16u, lr = std::min(base + r, last);
+  float4 pa = vp[lr], na = vn[lr], wa = vw[lr];
+  for (;;) {
+    lr = std::min(base + step + r, last);
+    const float4 pb = vp[lr], nb = vn[lr], wb = vw[lr];
+    infer_tile(net, T, r, g, base, count, pa, na, wa, bmin, bscale, orsrc);
+    base += step;
+    if (base >= count) break;
+    lr = std::min(base + step + r, last);
+    pa = vp[lr];
+    na = vn[lr];
+    wa = vw[lr];
+    infer_tile(net, T, r, g, base, count, pb, nb, wb, bmin, bscale, orsrc);
+    base += step;
+    if (base >= count) break;
   }
 }
 
@@ -910,6 +942,45 @@ int32_t th_guide_forward(th_guide* g, const float* feats, uint32_t count, float*
   for (uint32_t b = 0; b < count; ++b)
     for (int j = 0; j < TG_OUT; ++j) out[(size_t)b * TG_OUT + j] = o[(size_t)b * TG_OUTP + j];
   return TH_OK;
+}
+
+int32_t th_guide_infer_vertices(th_guide* g, const float* vertices, uint32_t count, const float bmin[3],
+                                const float bmax[3], float* out) {
+  if (!g || !vertices || !bmin || !bmax || !out || count == 0) return th_set_error(TH_E_INVALID, "null argument");
+  if (count > (1u << 24)) return th_set_error(TH_E_INVALID, "too many vertices");
+  // one queue segment holding the rows, padded to a multiple of 64 as the renderer's are
+  const uint32_t cap = round_up(count, 64);
+  std::vector<float4> hp(cap, make_float4(0.0f, 0.0f, 0.0f, 0.0f)), hn(hp), hw(hp);
+  for (uint32_t i = 0; i < count; ++i) {
+    const float* v = vertices + (size_t)i * 9;
+    hp[i] = make_float4(v[0], v[1], v[2], 0.0f);
+    hn[i] = make_float4(v[3], v[4], v[5], 0.0f);
+    hw[i] = make_float4(v[6], v[7], v[8], 0.0f);
+  }
+  hipStream_t st = g->ctx->stream;
+  float4 *dp = nullptr, *dn = nullptr, *dw = nullptr;
+  float* dO = nullptr;
+  uint32_t* dc = nullptr;
+  int32_t rc = TH_OK;
+  const size_t vb = (size_t)cap * sizeof(float4);
+  if (hipMalloc((void**)&dp, vb) != hipSuccess || hipMalloc((void**)&dn, vb) != hipSuccess ||
+      hipMalloc((void**)&dw, vb) != hipSuccess || hipMalloc((void**)&dO, (size_t)cap * TG_OUT * 4) != hipSuccess ||
+      hipMalloc((void**)&dc, 4) != hipSuccess) {
+    rc = th_set_error(TH_E_DEVICE, "inference workspace allocation failed");
+  } else if (hipMemcpyAsync(dp, hp.data(), vb, hipMemcpyHostToDevice, st) != hipSuccess ||
+             hipMemcpyAsync(dn, hn.data(), vb, hipMemcpyHostToDevice, st) != hipSuccess ||
+             hipMemcpyAsync(dw, hw.data(), vb, hipMemcpyHostToDevice, st) != hipSuccess ||
+             hipMemcpyAsync(dc, &count, 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+    rc = th_set_error(TH_E_DEVICE, "inference upload failed");
+  } else if ((rc = guide_infer(g, dc, 1u, 1u, cap, dp, dn, dw, bmin, bmax, dO, st)) == TH_OK) {
+    if (hipMemcpyAsync(out, dO, (size_t)count * TG_OUT * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = th_set_error(TH_E_DEVICE, "inference readback failed");
+  }
+  (void)hipStreamSynchronize(st);
+  for (void* p : {(void*)dp, (void*)dn, (void*)dw, (void*)dO, (void*)dc})
+    if (p) (void)hipFree(p);
+  return rc;
 }
 
 int32_t th_guide_train_batch(th_guide* g, const float* feats, const float* dirs, const float* weights,

@@ -49,6 +49,19 @@ class Guide:
         A.check(A.lib().th_guide_forward(self._h, _ptr(f), f.shape[0], _ptr(out)))
         return out
 
+    def infer_vertices(self, pos, nrm, wo, bmin, bmax):
+        """Outputs (count x 40) of the renderer's fused inference kernel for diffuse
+        vertices (position, normal, outgoing direction; position normalised by the box
+        [bmin, bmax] on the device)."""
+        v = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1, 3)
+                                                 for a in (pos, nrm, wo)], 1), np.float32)
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        assert lo.shape == (3,) and hi.shape == (3,)
+        out = np.empty((v.shape[0], A.TH_GUIDE_OUTPUTS), np.float32)
+        A.check(A.lib().th_guide_infer_vertices(self._h, _ptr(v), v.shape[0], _ptr(lo), _ptr(hi), _ptr(out)))
+        return out
+
     def train_batch(self, features, dirs, weights, update=True):
         """One Adam step (or the gradient only); returns (loss, gradient)."""
         f = np.ascontiguousarray(features, np.float32)
