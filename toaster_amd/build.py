@@ -29,6 +29,9 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wext
               "-Wno-unused-function"]
 
 HIP_SOURCES = ["th_runtime.hip", "th_wavefront.hip", "th_batch.hip", "th_guide.hip", "th_guided.hip"]
+# Per-source extra flags.  The network kernels keep MFMA results in ordinary VGPRs (no
+# accumulation-register copies: one v_accvgpr_read per result element otherwise).
+HIP_SOURCE_FLAGS = {"th_guide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HOST_SOURCES = ["th_pack.cpp"]
 
 
@@ -54,7 +57,8 @@ def build(verbose=False, defines=(), lib_path=LIB, jobs=None):
         objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        cmds.append([HIPCC] + HIP_FLAGS + dflags + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + HIP_SOURCE_FLAGS.get(src, []) + dflags +
+                    ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     workers = jobs or min(len(cmds), max(1, (os.cpu_count() or 2) // 2), 8)
     with ThreadPoolExecutor(max_workers=workers) as pool:

@@ -527,32 +527,55 @@ __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
   for (int i = 0; i < 4; ++i) C[(g * 4 + i) * 16 + r] = acc[i];
 }
 
-// Fused inference over a queue of deferred diffuse vertices, one wave per 16-row tile
-// and persistent over the queue.  Row i of the queue is the vertex (p, n, wo) stored in
-// queue order (vp, vn, vw).  The products run with the operands of k_gemm_rows swapped:
-// the A operand is 16 weight rows (out-major, every fragment held in registers, loaded
-// once per wave) and the B operand 32 inputs of the tile's 16 rows, so lane (r, g) ends
-// each 16-output block nb holding outputs nb*16 + 4g .. 4g + 3 of row r -- four
-// consecutive values of one row, written to the LDS activation tile with one 8-byte
-// store (BF16, row stride 72 elements) and to the output with one 16-byte store.  The
-// first layer's B fragment needs no tile: lane (r, g) holds inputs 8g .. 8g + 7 and
-// 32 + 8g .. 32 + 8g + 7 of row r, which it encodes itself -- lanes g < 3 the sin/cos
-// pairs of position axis g, lane g = 3 the bins of n.x and n.y, lanes g = 0 and 1 also
-// the bins of (n.z, wo.x) and (wo.y, wo.z), the rest zero padding -- with the same
-// feature functions as the training records.  Loads are clamped into the queue, and
-// stores cover whole tiles (segments hold a multiple of 64 rows) through a buffer
-// resource, the padding outputs 40..47 sent past its end (discarded by the hardware),
-// so the loop has no branch and the next tile's loads never wait for a store.
-struct InferNet {
-  tg_bf16x8 W[4][4][2];  // W[L][nb][kb]: weight rows nb*16 + r, inputs kb*32 + 8g .. 8g + 7
-  float4 bq[4][4];       // biases of outputs nb*16 + 4g .. 4g + 3
-};
+// Fused inference over a queue of deferred diffuse vertices.  Row i of the queue is the
+// vertex (p, n, wo) stored in queue order (vp, vn, vw).  A block of TG_INFER_WAVES waves
+// copies the network into LDS once, every weight fragment in the order the MFMA reads
+// it (fragment f, lane l at 16 * (64 f + l): one conflict-free 16-byte load per
+// fragment), then each wave walks its own 16-row tiles of the queue.  The products run
+// with the operands of k_gemm_rows swapped: the A operand is 16 weight rows and the B
+// operand 32 inputs of the tile's 16 rows, so lane (r, g) ends each 16-output block nb
+// with outputs nb*16 + 4g .. 4g + 3 of row r -- four consecutive values of one row,
+// written to the wave's LDS activation tile with one 8-byte store (BF16, row stride 72
+// elements) and to the output with one 16-byte store.  The biases (in registers) enter as
+// the accumulator's initial value.  The first layer's B fragment needs no tile: lane (r, g)
+// holds inputs 8g .. 8g + 7 and 32 + 8g .. 32 + 8g + 7 of row r, which it encodes itself
+// -- lanes g < 3 the sin/cos pairs of position axis g, lane g = 3 the bins of n.x and
+// n.y, lanes g = 0 and 1 also the bins of (n.z, wo.x) and (wo.y, wo.z), the rest zero
+// padding -- with the same feature functions as the training records.  ReLU is applied
+// to the BF16 pairs as 16-bit integer maxima with zero (a BF16 value and its bit pattern
+// as an int16 have the same sign, and max(bf16(v), 0) = bf16(max(v, 0)) for every
+// non-NaN v).  Loads are clamped into the queue, and stores cover whole tiles (segments
+// hold a multiple of 64 rows) through a buffer resource, the padding outputs 40..47 sent
+// past its end (discarded by the hardware), so the loop has no branch and the next
+// tile's loads never wait for a store.
+#ifndef TG_INFER_WAVES
+#define TG_INFER_WAVES 4
+#endif
+#define TG_INFER_FRAGS 30  // 4 x 2 per hidden-output layer (three), 3 x 2 for the 40 outputs
+#define TG_TILE_LD 72      // activation tile row stride (BF16 elements; 16-byte aligned rows)
+__device__ __forceinline__ int infer_frag(int L, int nb, int kb) {
+  return L < 3 ? (L * 4 + nb) * 2 + kb : 24 + nb * 2 + kb;
+}
 
-__device__ __forceinline__ void infer_tile(const InferNet& net, uint16_t* T, uint32_t r, uint32_t g,
-                                           uint32_t base, uint32_t count, float4 a, float4 b, float4 c,
-                                           float4 bmin, float4 bscale, __amdgpu_buffer_rsrc_t orsrc) {
+// LDS ordering between the lanes of one wave (its own tile: no other wave touches it)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+  typedef short s16x2 __attribute__((ext_vector_type(2)));
+  const s16x2 z = {0, 0};
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), z));
+}
+
+__device__ __forceinline__ void infer_tile(const uint4* __restrict__ WF, const float4 (&bq)[4][4],
+                                           uint16_t* T, uint32_t l, uint32_t base, uint32_t count, float4 a,
+                                           float4 b, float4 c, float4 bmin, float4 bscale,
+                                           __amdgpu_buffer_rsrc_t orsrc) {
   typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
-  typedef float v4f __attribute__((ext_vector_type(4)));
+  const uint32_t r = l & 15u, g = l >> 4;
   const bool in = base + r < count;
   const v3 p01 = in ? th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z)
                     : th::mk(0.5f, 0.5f, 0.5f);
@@ -588,89 +611,90 @@ __device__ __forceinline__ void infer_tile(const InferNet& net, uint16_t* T, uin
 #pragma unroll
   for (int L = 0; L < 4; ++L) {
     if (L > 0) {
-      b0 = lds8(T + r * 72 + g * 8);
-      b1 = lds8(T + r * 72 + 32 + g * 8);
+      b0 = lds8(T + r * TG_TILE_LD + g * 8);
+      b1 = lds8(T + r * TG_TILE_LD + 32 + g * 8);
     }
     const int nbn = L < 3 ? 4 : 3;  // the last layer has 40 outputs: three 16-wide blocks
     tg_f32x4 acc[4];
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
       if (nb < nbn) {
-        acc[nb] = mfma(net.W[L][nb][0], b0, tg_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-        acc[nb] = mfma(net.W[L][nb][1], b1, acc[nb]);
+        const tg_f32x4 c0 = {bq[L][nb].x, bq[L][nb].y, bq[L][nb].z, bq[L][nb].w};
+        acc[nb] = mfma(__builtin_bit_cast(tg_bf16x8, WF[infer_frag(L, nb, 0) * 64 + l]), b0, c0);
+        acc[nb] = mfma(__builtin_bit_cast(tg_bf16x8, WF[infer_frag(L, nb, 1) * 64 + l]), b1, acc[nb]);
       }
     }
     if (L < 3) {
-      __syncthreads();  // every lane has read the tile before it is overwritten
+      wave_lds_sync();  // every lane has read the tile before it is overwritten
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        const float v0 = acc[nb][0] + net.bq[L][nb].x, v1 = acc[nb][1] + net.bq[L][nb].y;
-        const float v2 = acc[nb][2] + net.bq[L][nb].z, v3 = acc[nb][3] + net.bq[L][nb].w;
-        *(uint2*)(T + r * 72 + nb * 16 + 4 * g) =
-            make_uint2(tg::f2bf2(v0 > 0.0f ? v0 : 0.0f, v1 > 0.0f ? v1 : 0.0f),
-                       tg::f2bf2(v2 > 0.0f ? v2 : 0.0f, v3 > 0.0f ? v3 : 0.0f));
-      }
-      __syncthreads();
+      for (int nb = 0; nb < 4; ++nb)
+        *(uint2*)(T + r * TG_TILE_LD + nb * 16 + 4 * g) =
+            make_uint2(relu_bf16x2(tg::f2bf2(acc[nb][0], acc[nb][1])),
+                       relu_bf16x2(tg::f2bf2(acc[nb][2], acc[nb][3])));
+      wave_lds_sync();
     } else {
       const uint32_t ob = ((base + r) * TG_OUT + 4u * g) * 4u;
 #pragma unroll
       for (int nb = 0; nb < 3; ++nb) {
-        const v4f v = {acc[nb][0] + net.bq[3][nb].x, acc[nb][1] + net.bq[3][nb].y,
-                       acc[nb][2] + net.bq[3][nb].z, acc[nb][3] + net.bq[3][nb].w};
         const uint32_t off = (nb < 2 || g < 2u) ? ob + (uint32_t)nb * 64u : 0x80000000u;
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orsrc, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, acc[nb]), orsrc, off, 0, 0);
       }
     }
   }
 }
 
-__global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__ woi,
-                                                    const float* __restrict__ bias,
-                                                    const uint32_t* __restrict__ count_ptr,
-                                                    uint32_t count_stride, uint32_t nseg, uint32_t seg_cap,
-                                                    const float4* __restrict__ vp,
-                                                    const float4* __restrict__ vn,
-                                                    const float4* __restrict__ vw, float4 bmin,
-                                                    float4 bscale, float* __restrict__ O) {
-  __shared__ __align__(16) uint16_t T[16 * 72];
-  // block b walks queue segment b % nseg (the grid is a multiple of nseg)
+__global__ void __launch_bounds__(64 * TG_INFER_WAVES) k_guide_infer(
+    const uint16_t* __restrict__ woi, const float* __restrict__ bias, const uint32_t* __restrict__ count_ptr,
+    uint32_t count_stride, uint32_t nseg, uint32_t seg_cap, const float4* __restrict__ vp,
+    const float4* __restrict__ vn, const float4* __restrict__ vw, float4 bmin, float4 bscale,
+    float* __restrict__ O) {
+  __shared__ uint4 WF[TG_INFER_FRAGS * 64];
+  __shared__ __align__(16) uint16_t TT[TG_INFER_WAVES][16 * TG_TILE_LD];
+  // block b serves queue segment b % nseg (the grid is a multiple of nseg); wave w of the
+  // block takes tiles jb * TG_INFER_WAVES + w, then every nbs * TG_INFER_WAVES-th
   const uint32_t sg = blockIdx.x % nseg, jb = blockIdx.x / nseg, nbs = gridDim.x / nseg;
   const uint32_t count = count_ptr[(size_t)sg * count_stride];
-  if (jb * 16u >= count) return;  // one wave per block: uniform exit
-  vp += (size_t)sg * seg_cap;
-  vn += (size_t)sg * seg_cap;
-  vw += (size_t)sg * seg_cap;
-  O += (size_t)sg * seg_cap * TG_OUT;
-  // this segment's rows of the output (seg_cap * TG_OUT floats, below 2^32 bytes)
-  const __amdgpu_buffer_rsrc_t orsrc =
-      __builtin_amdgcn_make_buffer_rsrc((void*)O, (short)0, (int)(seg_cap * (uint32_t)TG_OUT * 4u), 0x00020000);
-  const uint32_t l = threadIdx.x, r = l & 15u, g = l >> 4;
-  InferNet net;
+  if (jb * TG_INFER_WAVES * 16u >= count) return;  // uniform per block
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < TG_INFER_FRAGS * 64u; i += 64u * TG_INFER_WAVES) {
+    const uint32_t f = i >> 6, l = i & 63u, r = l & 15u, g = l >> 4;
+    const uint32_t L = f < 24u ? f >> 3 : 3u, nb = f < 24u ? (f >> 1) & 3u : (f - 24u) >> 1, kb = f & 1u;
+    WF[i] = *(const uint4*)(woi + L * 4096u + (nb * 16u + r) * 64u + kb * 32u + g * 8u);
+  }
+  __syncthreads();
+  const uint32_t w = t >> 6, l = t & 63u;
+  // biases of this lane's outputs nb*16 + 4g .. 4g + 3, held in registers
+  float4 bq[4][4];
 #pragma unroll
   for (int L = 0; L < 4; ++L)
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      if (L < 3 || nb < 3) net.bq[L][nb] = *(const float4*)(bias + L * 64 + nb * 16 + 4 * g);
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb)
-        net.W[L][nb][kb] = ld8(woi + L * 4096 + (nb * 16 + r) * 64 + kb * 32 + g * 8);
-    }
+    for (int nb = 0; nb < 4; ++nb)
+      if (L < 3 || nb < 3) bq[L][nb] = *(const float4*)(bias + L * 64 + nb * 16 + 4 * (l >> 4));
+  const uint32_t step = nbs * TG_INFER_WAVES * 16u, last = count - 1u;
+  uint32_t base = (jb * TG_INFER_WAVES + w) * 16u;
+  if (base >= count) return;  // no block barrier follows
+  vp += (size_t)sg * seg_cap;
+  vn += (size_t)sg * seg_cap;
+  vw += (size_t)sg * seg_cap;
+  // this segment's rows of the output (seg_cap * TG_OUT floats, below 2^32 bytes)
+  const __amdgpu_buffer_rsrc_t orsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(O + (size_t)sg * seg_cap * TG_OUT), (short)0, (int)(seg_cap * (uint32_t)TG_OUT * 4u), 0x00020000);
+  uint16_t* T = TT[w];
   // two register sets of vertex rows, loaded one tile ahead and used in turn (a single
   // loop-carried set would be copied at the back edge, which waits for its loads)
-  const uint32_t step = nbs * 16u, last = count - 1u;
-  uint32_t base = jb * 16u, lr = std::min(base + r, last);
+  uint32_t lr = std::min(base + (l & 15u), last);
   float4 pa = vp[lr], na = vn[lr], wa = vw[lr];
   for (;;) {
-    lr = std::min(base + step + r, last);
+    lr = std::min(base + step + (l & 15u), last);
     const float4 pb = vp[lr], nb = vn[lr], wb = vw[lr];
-    infer_tile(net, T, r, g, base, count, pa, na, wa, bmin, bscale, orsrc);
+    infer_tile(WF, bq, T, l, base, count, pa, na, wa, bmin, bscale, orsrc);
     base += step;
     if (base >= count) break;
-    lr = std::min(base + step + r, last);
+    lr = std::min(base + step + (l & 15u), last);
     pa = vp[lr];
     na = vn[lr];
     wa = vw[lr];
-    infer_tile(net, T, r, g, base, count, pb, nb, wb, bmin, bscale, orsrc);
+    infer_tile(WF, bq, T, l, base, count, pb, nb, wb, bmin, bscale, orsrc);
     base += step;
     if (base >= count) break;
   }
@@ -678,8 +702,8 @@ __global__ void __launch_bounds__(64) k_guide_infer(const uint16_t* __restrict__
 
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
-#ifndef TG_INFER_WAVES_PER_CU
-#define TG_INFER_WAVES_PER_CU 8u
+#ifndef TG_INFER_BLOCKS_PER_CU
+#define TG_INFER_BLOCKS_PER_CU 4u
 #endif
 
 }  // namespace
@@ -780,12 +804,12 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride,
                     uint32_t seg_cap, const float4* vp, const float4* vn, const float4* vw,
                     const float* bmin, const float* bmax, float* O, hipStream_t st) {
   if (nseg == 0u) return th_set_error(TH_E_INVALID, "guide queue without segments");
-  // persistent: as many one-wave blocks as stay resident (2 per SIMD at this kernel's
-  // register count), so each wave loads the weights once and walks many tiles; a
-  // multiple of nseg, so every segment has the same number of blocks
-  const uint32_t tiles = (seg_cap + 15u) / 16u;
+  // persistent: as many blocks as stay resident (TG_INFER_BLOCKS_PER_CU: the LDS of
+  // four fills a compute unit), so each block loads the network once and its waves walk
+  // many tiles; a multiple of nseg, so every segment has the same number of blocks
+  const uint32_t tiles = (seg_cap + 16u * TG_INFER_WAVES - 1u) / (16u * TG_INFER_WAVES);
   uint32_t per_seg = std::min<uint32_t>(
-      tiles, (TG_INFER_WAVES_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1) + nseg - 1u) / nseg);
+      tiles, (TG_INFER_BLOCKS_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1) + nseg - 1u) / nseg);
   const uint32_t grid = per_seg * nseg;
   if (grid == 0) return TH_OK;
   float4 lo = make_float4(bmin[0], bmin[1], bmin[2], 0.0f), sc;
@@ -793,7 +817,7 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride,
   for (int a = 0; a < 3; ++a) e[a] = bmax[a] - bmin[a];
   sc = make_float4(e[0] > 0.0f ? 1.0f / e[0] : 0.0f, e[1] > 0.0f ? 1.0f / e[1] : 0.0f,
                    e[2] > 0.0f ? 1.0f / e[2] : 0.0f, 0.0f);
-  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, st, g->d_woi, g->d_bias, d_count, count_stride,
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64 * TG_INFER_WAVES), 0, st, g->d_woi, g->d_bias, d_count, count_stride,
                      nseg, seg_cap, vp, vn, vw, lo, sc, O);
   TH_CHECK(hipGetLastError());
   return TH_OK;
