@@ -84,14 +84,46 @@ __global__ void __launch_bounds__(256) k_gemm_rows(const uint16_t* __restrict__ 
   }
 }
 
+// A training step reads its sample count from device memory, so the renderer can queue
+// steps on record counts it has not read back.  n = min(*count, cap) samples; fewer than
+// min_count make the step a no-op (no partials, no update, the step counter unchanged).
+// `step` is the device's Adam step counter (bias correction), advanced by k_weight_partial
+// when the step applies and updates; `stats` (optional) accumulates samples offered,
+// samples trained and steps applied.
+struct StepCtl {
+  const uint32_t* count;
+  uint32_t cap, min_count;
+  uint32_t* step;
+  uint32_t* stats;
+};
+__device__ __forceinline__ uint32_t step_samples(const StepCtl& c) {
+  const uint32_t n = min(*c.count, c.cap);
+  return n < c.min_count ? 0u : n;
+}
+// 64-sample chunks over at most 256 blocks (one partial slot each)
+__device__ __forceinline__ uint32_t step_rows_per(uint32_t n) { return ((n + 255u) / 256u + 63u) / 64u * 64u; }
+__device__ __forceinline__ uint32_t step_blocks(uint32_t n) {
+  return n == 0u ? 0u : (n + step_rows_per(n) - 1u) / step_rows_per(n);
+}
+
 // Loss gradient of one batch: L = -sum_b w_b log q_b(dir_b) / sum_b w_b.  The weight sum
 // is reduced in a fixed order (256 block partials here, then the same 256-to-1 tree in
 // every wave of the training kernel), so gradients are reproducible call to call.
-// Block 0 also clears the step's scalars (sums[1] later receives the loss from k_adam).
-__global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict__ w, uint32_t n,
+// Block 0 also clears the step's scalars (sums[1] later receives the loss from k_adam),
+// advances the step counter and the statistics.
+__global__ void __launch_bounds__(256) k_weight_partial(const float* __restrict__ w, StepCtl c, uint32_t update,
                                                         float* __restrict__ part, float* __restrict__ sums) {
   __shared__ float red[256];
+  const uint32_t n = step_samples(c);
   if (blockIdx.x == 0 && threadIdx.x < 16u) sums[threadIdx.x] = 0.0f;
+  if (blockIdx.x == 0 && threadIdx.x == 0u) {
+    if (n > 0u && update) *c.step += 1u;
+    if (c.stats) {
+      c.stats[0] += min(*c.count, c.cap);
+      c.stats[1] += n;
+      c.stats[2] += n > 0u ? 1u : 0u;
+    }
+  }
   float s = 0.0f;
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) s += w[i];
   red[threadIdx.x] = s;
@@ -164,8 +196,8 @@ __device__ __forceinline__ float oct_max(float v) {
 #define TF_THREADS 512
 __global__ void __launch_bounds__(TF_THREADS) k_train_fused(const uint16_t* __restrict__ X,
                                                             const float* __restrict__ dirs,
-                                                            const float* __restrict__ w, uint32_t count,
-                                                            uint32_t rows_per, const float* __restrict__ wpart,
+                                                            const float* __restrict__ w, StepCtl ctl,
+                                                            const float* __restrict__ wpart,
                                                             const uint16_t* __restrict__ woi,
                                                             const uint16_t* __restrict__ wio,
                                                             const float* __restrict__ bias,
@@ -179,6 +211,8 @@ __global__ void __launch_bounds__(TF_THREADS) k_train_fused(const uint16_t* __re
   __shared__ float lred[TF_THREADS / 64];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, wv = wave & 3u, h = wave >> 2;
   const uint32_t l = tid & 63u, r = l & 15u, g = l >> 4;
+  const uint32_t count = step_samples(ctl), rows_per = step_rows_per(count);
+  if (blockIdx.x >= step_blocks(count)) return;  // uniform per block (none when the step is skipped)
   const uint32_t r0 = blockIdx.x * rows_per, r1 = min(r0 + rows_per, count);
   const uint32_t col = 16u * wv + r;  // this lane's feature column in the forward / dZ passes
   const uint32_t rr = tid >> 3, lobe = tid & 7u;  // the loss gradient's sample and lobe
@@ -406,11 +440,18 @@ __global__ void __launch_bounds__(TF_THREADS) k_train_fused(const uint16_t* __re
 // (64 independent loads per lane, partials TG_PART floats apart), wave 0 adds the 16
 // group sums in order and applies the update.
 __global__ void __launch_bounds__(256) k_adam(float* __restrict__ master, float* __restrict__ m1,
-                       float* __restrict__ m2, const float* __restrict__ partials, float lr, float b1,
-                       float b2, float eps, float bc1, float bc2, float* __restrict__ grad_out,
+                       float* __restrict__ m2, const float* __restrict__ partials, StepCtl ctl, float lr,
+                       float b1, float b2, float eps, float* __restrict__ grad_out,
                        uint16_t* __restrict__ woi, uint16_t* __restrict__ wio, float* __restrict__ bias,
-                       const float* __restrict__ loss_part, uint32_t nblocks, float* __restrict__ loss) {
+                       const float* __restrict__ loss_part, float* __restrict__ loss) {
   __shared__ float gs[TG_GROUPS][64];
+  // ctl.count null: refresh the BF16 copies and biases from the master weights only
+  const bool refresh = ctl.count == nullptr;
+  const uint32_t nblocks = refresh ? 0u : step_blocks(step_samples(ctl));
+  if (!refresh && nblocks == 0u) return;  // skipped step: parameters, moments and gradient untouched
+  // Adam's bias corrections at the step counter k_weight_partial advanced
+  const float tstep = refresh ? 1.0f : (float)*ctl.step;
+  const float bc1 = lr > 0.0f ? 1.0f - powf(b1, tstep) : 1.0f, bc2 = lr > 0.0f ? 1.0f - powf(b2, tstep) : 1.0f;
   const uint32_t lane = threadIdx.x & 63u, q = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * 64u + lane;
   if (loss != nullptr && blockIdx.x == 0 && q == 0u) {  // the step's loss: fixed-order sum of the block partials
@@ -713,7 +754,10 @@ struct th_guide {
   th_context* ctx = nullptr;
   float *d_master = nullptr, *d_m1 = nullptr, *d_m2 = nullptr, *d_bias = nullptr, *d_grad = nullptr;
   uint16_t *d_woi = nullptr, *d_wio = nullptr;
-  uint32_t step = 0;
+  // d_ctl[0]: the host API's sample count, [1]: Adam's step counter (steps applied)
+  uint32_t* d_ctl = nullptr;
+  uint32_t host_count = 0;  // source of the count's upload (stays valid until it lands)
+  uint32_t step = 0;        // Adam steps applied, as far as the host knows (the guided schedule's count)
   float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
   // batch workspace
   uint32_t cap = 0;
@@ -774,27 +818,18 @@ static int32_t th_guide_forward_rows(th_guide* g, const uint16_t* X, const uint3
 // One training step on `count` samples already on the device: BF16 features X (64 wide,
 // zero beyond 48; rows up to count rounded to 32 must be finite), directions and
 // weights.  lr = 0 computes the gradient only.  Workspace must hold count rows.
-static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, const float* w,
-                          uint32_t count, float lr) {
+static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, const float* w, const StepCtl& c,
+                          float lr) {
   hipStream_t st = g->ctx->stream;
-  hipLaunchKernelGGL(k_weight_partial, dim3(256), dim3(256), 0, st, w, count, g->wpart, g->scal);
-  // 64-sample chunks, at most kWgradBlocks blocks (one partial slot each)
-  const uint32_t rows_per = round_up((count + kWgradBlocks - 1) / kWgradBlocks, 64);
-  const uint32_t nb = (count + rows_per - 1) / rows_per;
+  hipLaunchKernelGGL(k_weight_partial, dim3(256), dim3(256), 0, st, w, c, lr > 0.0f ? 1u : 0u, g->wpart, g->scal);
+  // every partial slot's block is launched; those beyond the step's sample count return
   float* P = g->partials;
-  hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(TF_THREADS), 0, st, X, dirs, w, count, rows_per, g->wpart, g->d_woi,
+  hipLaunchKernelGGL(k_train_fused, dim3(kWgradBlocks), dim3(TF_THREADS), 0, st, X, dirs, w, c, g->wpart, g->d_woi,
                      g->d_wio, g->d_bias, P, g->lpart);
   TH_CHECK(hipGetLastError());
-  float bc1 = 1.0f, bc2 = 1.0f;
-  if (lr > 0.0f) {
-    g->step += 1;
-    bc1 = 1.0f - powf(g->beta1, (float)g->step);
-    bc2 = 1.0f - powf(g->beta2, (float)g->step);
-  }
-  // partials of layer l: kWgradBlocks slots of TG_PART floats; the first nb are summed
-  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, st, g->d_master,
-                     g->d_m1, g->d_m2, P, lr, g->beta1, g->beta2,
-                     g->eps, bc1, bc2, g->d_grad, g->d_woi, g->d_wio, g->d_bias, g->lpart, nb, g->scal + 1);
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, st, g->d_master, g->d_m1, g->d_m2, P, c,
+                     lr, g->beta1, g->beta2, g->eps, g->d_grad, g->d_woi, g->d_wio, g->d_bias, g->lpart,
+                     g->scal + 1);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -823,10 +858,12 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride,
   return TH_OK;
 }
 
-int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count) {
-  int32_t rc = guide_reserve(g, count);
+int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, const uint32_t* d_count,
+                    uint32_t cap, uint32_t min_count, uint32_t* d_stats) {
+  int32_t rc = guide_reserve(g, cap);
   if (rc) return rc;
-  return guide_step(g, X, dirs, w, count, g->lr);
+  const StepCtl c{d_count, cap, min_count, g->d_ctl + 1, d_stats};
+  return guide_step(g, X, dirs, w, c, g->lr);
 }
 
 // The last step's loss, copied once (a copy per step would stall the stream each time).
@@ -836,6 +873,7 @@ int32_t guide_last_loss(th_guide* g, float* out) {
 }
 
 uint32_t guide_steps(const th_guide* g) { return g->step; }
+void guide_set_steps(th_guide* g, uint32_t steps) { g->step = steps; }
 th_context* guide_context(const th_guide* g) { return g->ctx; }
 
 // ====================================================================== C ABI
@@ -868,9 +906,10 @@ static void host_init_params(uint32_t seed, std::vector<float>& p) {
 }
 
 static int32_t guide_refresh(th_guide* g) {  // bf16 copies from the master weights
+  const StepCtl refresh{nullptr, 0u, 0u, nullptr, nullptr};
   hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, g->ctx->stream, g->d_master,
-                     g->d_m1, g->d_m2, g->partials, 0.0f, g->beta1, g->beta2, g->eps, 1.0f, 1.0f,
-                     (float*)nullptr, g->d_woi, g->d_wio, g->d_bias, (const float*)nullptr, 0u, (float*)nullptr);
+                     g->d_m1, g->d_m2, g->partials, refresh, 0.0f, g->beta1, g->beta2, g->eps, (float*)nullptr,
+                     g->d_woi, g->d_wio, g->d_bias, (const float*)nullptr, (float*)nullptr);
   TH_CHECK(hipGetLastError());
   return TH_OK;
 }
@@ -889,7 +928,7 @@ int32_t th_guide_create(th_context* ctx, uint32_t seed, th_guide** out) {
   if (!al((void**)&g->d_master, TG_NPARAMS * 4) || !al((void**)&g->d_m1, TG_NPARAMS * 4) ||
       !al((void**)&g->d_m2, TG_NPARAMS * 4) || !al((void**)&g->d_grad, TG_NPARAMS * 4) ||
       !al((void**)&g->d_bias, 4 * 64 * 4) || !al((void**)&g->d_woi, 4 * 4096 * 2) ||
-      !al((void**)&g->d_wio, 4 * 4096 * 2)) {
+      !al((void**)&g->d_wio, 4 * 4096 * 2) || !al((void**)&g->d_ctl, 16 * 4)) {
     th_guide_destroy(g);
     return th_set_error(TH_E_DEVICE, "guide allocation failed");
   }
@@ -914,7 +953,7 @@ void th_guide_destroy(th_guide* g) {
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
   for (void* p : g->ws) (void)hipFree(p);
-  void* ps[] = {g->d_master, g->d_m1, g->d_m2, g->d_grad, g->d_bias, g->d_woi, g->d_wio};
+  void* ps[] = {g->d_master, g->d_m1, g->d_m2, g->d_grad, g->d_bias, g->d_woi, g->d_wio, g->d_ctl};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete g;
@@ -933,6 +972,7 @@ int32_t th_guide_set_params(th_guide* g, const float* in, uint32_t n) {
   TH_CHECK(hipMemcpy(g->d_master, in, TG_NPARAMS * 4, hipMemcpyHostToDevice));
   TH_CHECK(hipMemset(g->d_m1, 0, TG_NPARAMS * 4));
   TH_CHECK(hipMemset(g->d_m2, 0, TG_NPARAMS * 4));
+  TH_CHECK(hipMemset(g->d_ctl, 0, 16 * 4));
   g->step = 0;
   int32_t rc = guide_refresh(g);
   if (rc) return rc;
@@ -1018,7 +1058,11 @@ int32_t th_guide_train_batch(th_guide* g, const float* feats, const float* dirs,
   TH_CHECK(hipMemcpyAsync(g->dirs, dirs, (size_t)count * 12, hipMemcpyHostToDevice, st));
   TH_CHECK(hipMemcpyAsync(g->w, weights, (size_t)count * 4, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_encode_host, dim3((rows * 8u + 255) / 256), dim3(256), 0, st, g->feat, count, rows, g->X0);
-  if ((rc = guide_step(g, g->X0, g->dirs, g->w, count, update ? g->lr : 0.0f))) return rc;
+  g->host_count = count;
+  TH_CHECK(hipMemcpyAsync(g->d_ctl, &g->host_count, 4, hipMemcpyHostToDevice, st));
+  const StepCtl c{g->d_ctl, count, 1u, g->d_ctl + 1, nullptr};
+  if ((rc = guide_step(g, g->X0, g->dirs, g->w, c, update ? g->lr : 0.0f))) return rc;
+  if (update && g->lr > 0.0f) g->step += 1;  // count >= 1: the step applies
   if (loss) TH_CHECK(hipMemcpyAsync(loss, g->scal + 1, 4, hipMemcpyDeviceToHost, st));
   if (grad) TH_CHECK(hipMemcpyAsync(grad, g->d_grad, TG_NPARAMS * 4, hipMemcpyDeviceToHost, st));
   TH_CHECK(hipStreamSynchronize(st));

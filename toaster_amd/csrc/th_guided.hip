@@ -66,6 +66,13 @@ struct GdBufs {
 };
 
 #define GD_MAX_CHUNKS 16u
+// scalar counter lines: 0-3 queues, 5 diffuse vertices, 6 network rows, 8..23 chunk totals,
+// then the record-rate state and the training statistics
+#define GD_DPP_PREV 24u    // diffuse-vertex count at the last training pass's end
+#define GD_DPP 26u         // diffuse vertices of the last training pass (persists across renders)
+#define GD_STATS 27u       // three words: samples offered, samples trained, steps applied
+#define GD_REC_ALL 30u     // this pass records every eligible vertex
+#define GD_REC_THRESH 31u  // ... or those whose hash is below this
 #define GD_C(c) ((c) * 32u)  // queue counters 128 bytes apart: no two share a cache line
 // Queues in segments.  A wave that appends to a queue takes its slots with one atomic on
 // a counter, and with one counter for the whole queue those same-address atomics
@@ -92,8 +99,6 @@ struct GdArgs {
   float alpha;          // guide probability (0: cosine lobe only)
   uint32_t defer;       // diffuse vertices go through the guide queue
   uint32_t record;      // record training samples
-  uint32_t rec_thresh;  // record an eligible vertex when its hash < rec_thresh
-  uint32_t rec_all;     // record every eligible vertex
   uint32_t env_nee;     // environment sample + shadow ray at diffuse vertices
   uint32_t nchunks;     // records are spread uniformly at random over this many chunks
   uint32_t chunk_cap;   // capacity of one chunk (chunk c holds rows c * chunk_cap ...), in
@@ -134,10 +139,22 @@ __device__ __forceinline__ void gd_push2(bool la, uint32_t a, bool lb, uint32_t 
   if (lb) queue[base + na + (uint32_t)__popcll(mb & lt)] = b;
 }
 
-__global__ void gd_reset(GdBufs B, uint32_t all) {
+// Counters of a pass (and, with `all`, of the render: everything but the record-rate
+// estimate GD_DPP, which persists with the workspace).  Thread 0 then sets the pass's
+// record rate: a vertex is recorded with probability target / (the last training pass's
+// diffuse vertices), `init_dpp` standing in before any pass has counted them.
+__global__ void gd_reset(GdBufs B, uint32_t all, uint32_t target, uint32_t init_dpp) {
   const uint32_t t = threadIdx.x;
-  if (t < 4u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u)) B.cnt[GD_C(t)] = 0u;
+  if (t < 4u || (t >= 8u && t < 8u + GD_MAX_CHUNKS) || (all && t < 32u && t != GD_DPP)) B.cnt[GD_C(t)] = 0u;
+  if (all && t < 3u) B.cnt[GD_C(GD_STATS) + t] = 0u;
   for (uint32_t c = t; c < (4u + GD_MAX_CHUNKS) * GD_NSEG; c += blockDim.x) B.cnt[GD_C(32u + c)] = 0u;
+  if (t == 0u) {
+    uint32_t dpp = B.cnt[GD_C(GD_DPP)];
+    if (dpp == 0u) B.cnt[GD_C(GD_DPP)] = dpp = init_dpp;
+    const double pr = (double)target / (double)(dpp > 0u ? dpp : 1u);
+    B.cnt[GD_C(GD_REC_ALL)] = pr >= 1.0 ? 1u : 0u;
+    B.cnt[GD_C(GD_REC_THRESH)] = (uint32_t)fmin(pr * 4294967296.0, 4294967295.0);
+  }
 }
 
 // ---------------------------------------------------------------- generate
@@ -526,7 +543,7 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       if (A.record && !done) {
         uint32_t h = th::pcg_hash((k * 0x9E3779B9u) ^ (A.bounce * 0x85EBCA6Bu) ^
                                   (A.smp * 0xC2B2AE35u + P.frame_index));
-        rec_ok = A.rec_all || h < A.rec_thresh;
+        rec_ok = B.cnt[GD_C(GD_REC_ALL)] != 0u || h < B.cnt[GD_C(GD_REC_THRESH)];
       }
       uint32_t f = GD_PREV_DIFFUSE | (done ? GD_DONE : 0u) | (sh_light ? GD_PEND_LIGHT : 0u) |
                    (sh_env ? GD_PEND_ENV : 0u);
@@ -615,10 +632,16 @@ __global__ void __launch_bounds__(256) gd_retire(GdBufs B, uint32_t n) {
 // Weight of each record of the pass: lum((L - L_v) / beta_v), zero where not finite.
 // The chunk's records, in GD_NSEG pieces (one per queue segment), are gathered into
 // contiguous rows of tX, tdir and rw (piece 0's first, then piece 1's, ...), and the
-// chunk's total is left in GD_C(8 + c) for the host.  One thread per destination row.
+// chunk's total is left in GD_C(8 + c) for the training step.  One thread per destination
+// row.
 __global__ void __launch_bounds__(256) gd_weights(GdBufs B, uint32_t nchunks, uint32_t chunk_cap) {
   __shared__ uint32_t pre[GD_MAX_CHUNKS][GD_NSEG + 1];  // pieces' first rows; [GD_NSEG]: total
   const uint32_t piece = chunk_cap / GD_NSEG;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the record-rate estimate for the next passes
+    const uint32_t total = B.cnt[GD_C(5)], d = total - B.cnt[GD_C(GD_DPP_PREV)];
+    B.cnt[GD_C(GD_DPP_PREV)] = total;
+    B.cnt[GD_C(GD_DPP)] = d > 0u ? d : 1u;
+  }
   for (uint32_t c = threadIdx.x; c < nchunks; c += blockDim.x) {
     uint32_t acc = 0;
     for (uint32_t j = 0; j < GD_NSEG; ++j) {
@@ -676,7 +699,6 @@ __global__ void __launch_bounds__(256) gd_resolve(GdBufs B, GdArgs A, TH_GpuRend
 
 struct th_guided_ws {
   uint32_t capacity = 0, rcap = 0;
-  uint64_t diffuse_per_pass = 0;  // last pass's diffuse vertices (record-rate estimate)
   GdBufs b;
   std::vector<void*> allocs;
 };
@@ -793,8 +815,9 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   }
   th_guided_stats st;
   memset(&st, 0, sizeof st);
-  uint64_t diffuse_before = 0;
-  if (w->diffuse_per_pass == 0) w->diffuse_per_pass = (uint64_t)npix * std::max<uint32_t>(p->max_bounces, 2u) / 2u;
+  // record-rate estimate before any training pass has counted diffuse vertices
+  const uint32_t init_dpp = (uint32_t)std::min<uint64_t>(
+      (uint64_t)npix * std::max<uint32_t>(p->max_bounces, 2u) / 2u, 0xffffffffu);
   std::pair<hipEvent_t, hipEvent_t> evp;
   if ((rc = begin_timed(ctx, &evp))) return rc;
   hipStream_t sm = ctx->stream;
@@ -820,9 +843,10 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     v.clear();
     return total;
   };
-  uint64_t trained = 0;
-  hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 1u);
+  hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 1u, 0u, init_dpp);
   uint32_t cnt_host[GD_C(32)];
+  const uint32_t steps0 = guide_steps(g);  // steps applied before this call
+  uint32_t applied_host = 0;
   for (uint32_t smp = 0; smp < p->samples; ++smp) {
     const uint32_t taken = guide_steps(g);
     const bool warm = taken < opt->warmup_steps;
@@ -842,12 +866,8 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     // environment sampling at diffuse vertices as the caller asks, whatever alpha is, so
     // the guide's gain can be measured against the same estimator without it
     A.env_nee = (opt->flags & TH_GUIDED_NO_ENV_NEE) ? 0u : 1u;
-    if (train_pass) {
-      double pr = (double)target / (double)std::max<uint64_t>(w->diffuse_per_pass, 1);
-      A.rec_all = pr >= 1.0 ? 1u : 0u;
-      A.rec_thresh = (uint32_t)std::min(pr * 4294967296.0, 4294967295.0);
-    }
-    hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 0u);
+    hipLaunchKernelGGL(gd_reset, dim3(1), dim3(64), 0, sm, B, 0u,
+                       train_pass ? (uint32_t)std::min<uint64_t>(target, 0xffffffffu) : 0u, init_dpp);
     hipLaunchKernelGGL(gd_generate, dim3((npix + 255) / 256), dim3(256), 0, sm, B, A, *p, *cam);
     for (uint32_t b = 0; b < p->max_bounces; ++b) {
       const uint32_t li = b & 1u;
@@ -878,18 +898,16 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
     if (train_pass) {
       hipLaunchKernelGGL(gd_weights, dim3(std::min<uint32_t>((pass_steps * chunk_cap + 255) / 256, 4096u)),
                          dim3(256), 0, sm, B, pass_steps, chunk_cap);
-      TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
-      TH_CHECK(hipStreamSynchronize(sm));
-      const uint64_t diffuse_total = cnt_host[GD_C(5)];
-      w->diffuse_per_pass = std::max<uint64_t>(diffuse_total - diffuse_before, 1);
-      diffuse_before = diffuse_total;
+      // the steps read their chunk's record count on the device (a step on fewer than
+      // 1024 records is skipped there), so the host queues the next pass while this one
+      // runs -- except while the schedule (warm-up rate, alpha ramp, the untrained
+      // network's alpha 0) still depends on how many steps have applied: then it reads
+      // that count back after the pass
       for (uint32_t j = 0; j < pass_steps; ++j) {
-        const uint32_t cnt = std::min(cnt_host[GD_C(8 + j)], chunk_cap);
-        st.records += cnt;
-        if (cnt < 1024u) continue;  // too few samples for a step
         const size_t off = (size_t)j * chunk_cap;
         mark(ev_train);
-        rc = guide_train(g, B.tX + off * 64u, B.tdir + off * 3u, B.rw + off, cnt);
+        rc = guide_train(g, B.tX + off * 64u, B.tdir + off * 3u, B.rw + off, B.cnt + GD_C(8u + j), chunk_cap,
+                         1024u, B.cnt + GD_C(GD_STATS));
         mark(ev_train);
         if (rc) {
           (void)end_timed(ctx, evp);
@@ -897,8 +915,14 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
           (void)span_ms(ev_train);
           return rc;
         }
-        st.train_steps++;
-        trained += cnt;
+      }
+      const bool settled = taken > 0u && taken >= opt->warmup_steps && taken >= opt->alpha_ramp;
+      if (settled) {
+        guide_set_steps(g, taken + pass_steps);  // the schedule no longer depends on it
+      } else {
+        TH_CHECK(hipMemcpyAsync(&applied_host, B.cnt + GD_C(GD_STATS) + 2u, 4, hipMemcpyDeviceToHost, sm));
+        TH_CHECK(hipStreamSynchronize(sm));
+        guide_set_steps(g, steps0 + applied_host);
       }
     }
   }
@@ -908,19 +932,24 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   if (le != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("guided launch: ") + hipGetErrorString(le));
   ctx->counters_pending = false;
   ctx->stats.paths = npix64 * p->samples;
-  if (st.train_steps > 0 && (rc = guide_last_loss(g, &st.last_loss))) return rc;
-  if (train) TH_CHECK(hipStreamSynchronize(sm));  // the last loss has landed in `st`
+  if (train || stats) {
+    TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
+    if (train && (rc = guide_last_loss(g, &st.last_loss))) return rc;
+    TH_CHECK(hipStreamSynchronize(sm));
+    if (cnt_host[GD_C(GD_STATS) + 2u] == 0u) st.last_loss = 0.0f;  // no step applied in this render
+    guide_set_steps(g, steps0 + cnt_host[GD_C(GD_STATS) + 2u]);
+  }
   if (time_net) {
     TH_CHECK(hipStreamSynchronize(sm));
     st.infer_ms = span_ms(ev_infer);
     st.train_ms = span_ms(ev_train);
   }
   if (stats) {
-    TH_CHECK(hipMemcpyAsync(cnt_host, B.cnt, sizeof cnt_host, hipMemcpyDeviceToHost, sm));
-    TH_CHECK(hipStreamSynchronize(sm));
     st.diffuse_vertices = cnt_host[GD_C(5)];
     st.inferred_rows = cnt_host[GD_C(6)];
-    st.trained_samples = trained;
+    st.records = cnt_host[GD_C(GD_STATS)];
+    st.trained_samples = cnt_host[GD_C(GD_STATS) + 1u];
+    st.train_steps = cnt_host[GD_C(GD_STATS) + 2u];
     *stats = st;
   }
   return TH_OK;

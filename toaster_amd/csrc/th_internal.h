@@ -107,6 +107,10 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride,
                     uint32_t seg_cap, const float4* vp, const float4* vn, const float4* vw,
                     const float* bmin, const float* bmax, float* O, hipStream_t st);
 int32_t guide_last_loss(th_guide* g, float* out);
-int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, uint32_t count);
+// One training step on min(*d_count, cap) rows (a no-op below min_count) with the count
+// read on the device; d_stats (optional) accumulates samples offered, trained, steps applied.
+int32_t guide_train(th_guide* g, const uint16_t* X, const float* dirs, const float* w, const uint32_t* d_count,
+                    uint32_t cap, uint32_t min_count, uint32_t* d_stats);
 uint32_t guide_steps(const th_guide* g);
+void guide_set_steps(th_guide* g, uint32_t steps);
 th_context* guide_context(const th_guide* g);
