@@ -834,8 +834,10 @@ float host_bits_to_float(uint32_t u) {
   return f;
 }
 
+// `box` (optional): the box of each reference node for the 4-wide records, 6 floats per
+// node (min, max), NaN in box[6i] for a node with nothing left to test (left out).
 void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out, uint32_t* root4,
-                uint32_t* depth4, std::vector<uint32_t>* src) {
+                uint32_t* depth4, std::vector<uint32_t>* src, const std::vector<float>* box) {
   out.clear();
   src->clear();
   *depth4 = 0;
@@ -864,7 +866,8 @@ void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out
     }
     for (uint32_t k = 0; k < 4; ++k) {
       Child4 ch;
-      if (k < nk) {
+      const float* bx = (k < nk && box) ? &(*box)[6 * (size_t)kids[k]] : nullptr;
+      if (k < nk && !(bx && bx[0] != bx[0])) {
         const TH_GpuBvhNode& g = nodes[kids[k]];
         uint32_t entry;
         if (g.primitive_count == 0) {
@@ -873,8 +876,9 @@ void build_wide(const TH_GpuBvhNode* nodes, uint32_t n, std::vector<Child4>& out
         } else {
           entry = th_entry_of(g, kids[k]);
         }
-        ch.a = make_float4(g.min[0], g.min[1], g.min[2], host_bits_to_float(entry));
-        ch.b = make_float4(g.max[0], g.max[1], g.max[2], 0.0f);
+        ch.a = bx ? make_float4(bx[0], bx[1], bx[2], host_bits_to_float(entry))
+                  : make_float4(g.min[0], g.min[1], g.min[2], host_bits_to_float(entry));
+        ch.b = bx ? make_float4(bx[3], bx[4], bx[5], 0.0f) : make_float4(g.max[0], g.max[1], g.max[2], 0.0f);
         src->push_back(kids[k]);
       } else {
         ch.a = make_float4(0, 0, 0, host_bits_to_float(THD_NO_CHILD));
@@ -911,6 +915,108 @@ int32_t prepare_env(th_scene* s, const th_scene_desc* d) {
   return TH_OK;
 }
 
+// Primitives tested before the 4-wide tree.  A primitive whose box is large against the
+// scene sits deep in a median-split tree (the split follows centroids) and inflates the
+// box of every ancestor: a roof or ground quad over a height field makes nearly every ray
+// enter those boxes, all the way down.  Candidates are the (at most 8) largest primitives
+// whose box surface area exceeds 1/64 of the root's; their removal is kept when the boxes
+// of the tree without them shrink by at least 2 root areas per candidate in total (the
+// expected node visits a ray saves, to first order, against one more primitive test per
+// candidate).  The 4-wide records then carry, per reference node, the box of what remains
+// below it; a node with nothing left is left out.  The candidates stay in their leaves
+// (where a second test finds the same distance and rank, so closest hits and their ties
+// are unchanged).  Boxes are folded from each primitive's packer box (triangle vertex
+// min/max -+ 1e-5, sphere center -+ radius).
+static void select_extracted(th_scene* s, const th_scene_desc* d, std::vector<float>& box) {
+  s->ext_n = 0;
+  box.clear();
+  const uint32_t nn = d->bvh_node_count, np = d->bvh_primitive_count;
+  if (nn < 3 || np == 0) return;
+  auto sa = [](const float* mn, const float* mx) {
+    const double a = (double)mx[0] - mn[0], b = (double)mx[1] - mn[1], c = (double)mx[2] - mn[2];
+    return 2.0 * (a * b + b * c + a * c);
+  };
+  std::vector<float> pb(6 * (size_t)np);
+  std::vector<double> psa(np);
+  for (uint32_t q = 0; q < np; ++q) {
+    const TH_GpuPrimitiveRef& r = d->bvh_primitives[q];
+    float* b = &pb[6 * (size_t)q];
+    if (r.kind == 1u && r.index < d->triangle_count) {
+      const TH_GpuTriangle& t = d->triangles[r.index];
+      for (int a = 0; a < 3; ++a) {
+        b[a] = std::min(std::min(t.v0[a], t.v1[a]), t.v2[a]) - 1e-5f;
+        b[3 + a] = std::max(std::max(t.v0[a], t.v1[a]), t.v2[a]) + 1e-5f;
+      }
+    } else if (r.kind == 0u && r.index < d->sphere_count) {
+      const TH_GpuSphere& sp = d->spheres[r.index];
+      for (int a = 0; a < 3; ++a) {
+        b[a] = sp.center_radius[a] - sp.center_radius[3];
+        b[3 + a] = sp.center_radius[a] + sp.center_radius[3];
+      }
+    } else {
+      return;  // unknown reference: keep the tree as it is
+    }
+    psa[q] = sa(b, b + 3);
+  }
+  const double root = sa(d->bvh_nodes[0].min, d->bvh_nodes[0].max);
+  if (!(root > 0.0)) return;
+  std::vector<uint32_t> cand;
+  for (uint32_t q = 0; q < np; ++q)
+    if (psa[q] > root / 64.0) cand.push_back(q);
+  std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return psa[a] > psa[b]; });
+  if (cand.size() > 8) cand.resize(8);
+  if (cand.empty()) return;
+  std::vector<uint8_t> ext(np, 0);
+  for (uint32_t q : cand) ext[q] = 1;
+  // boxes without the candidates, children before parents (pre-order: reverse index order)
+  const float NaN = __builtin_nanf("");
+  box.assign(6 * (size_t)nn, NaN);
+  double gain = 0.0;
+  for (uint32_t i = nn; i-- > 0;) {
+    const TH_GpuBvhNode& n = d->bvh_nodes[i];
+    float* b = &box[6 * (size_t)i];
+    bool any = false;
+    auto fold = [&](const float* c) {
+      for (int a = 0; a < 3; ++a) {
+        b[a] = any ? std::min(b[a], c[a]) : c[a];
+        b[3 + a] = any ? std::max(b[3 + a], c[3 + a]) : c[3 + a];
+      }
+      any = true;
+    };
+    if (n.primitive_count > 0) {
+      for (uint32_t k = 0; k < n.primitive_count; ++k) {
+        const uint32_t q = n.first_or_right + k;
+        if (q < np && !ext[q]) fold(&pb[6 * (size_t)q]);
+      }
+    } else {
+      for (uint32_t c : {i + 1u, n.first_or_right}) {
+        const float* cb = &box[6 * (size_t)c];
+        if (cb[0] == cb[0]) fold(cb);
+      }
+    }
+    gain += sa(n.min, n.max) - (any ? sa(b, b + 3) : 0.0);
+  }
+  if (gain / root < 2.0 * (double)cand.size()) {
+    box.clear();
+    return;
+  }
+  for (uint32_t i = 0; i < nn; ++i) {
+    const TH_GpuBvhNode& n = d->bvh_nodes[i];
+    if (n.primitive_count == 0) continue;
+    for (uint32_t k = 0; k < n.primitive_count; ++k) {
+      const uint32_t q = n.first_or_right + k;
+      if (q < np && ext[q]) {
+        for (uint32_t j = 0; j < cand.size(); ++j)
+          if (cand[j] == q) {
+            s->ext_slot[j] = q;
+            s->ext_leaf[j] = n.first_or_right;
+          }
+      }
+    }
+  }
+  s->ext_n = (uint32_t)cand.size();
+}
+
 int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
   th_context* ctx = s->ctx;
   s->node_count = d->bvh_node_count;
@@ -925,7 +1031,9 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
     s->max_depth = bvh_depth(d->bvh_nodes, d->bvh_node_count);
     std::vector<Child4> w;
     std::vector<uint32_t> wsrc;
-    build_wide(d->bvh_nodes, d->bvh_node_count, w, &s->root4, &s->depth4, &wsrc);
+    std::vector<float> rbox;
+    select_extracted(s, d, rbox);
+    build_wide(d->bvh_nodes, d->bvh_node_count, w, &s->root4, &s->depth4, &wsrc, rbox.empty() ? nullptr : &rbox);
     s->node4_count = (uint32_t)(w.size() / 4);
     if (!w.empty()) {
       TH_CHECK(hipMemcpy(s->d_nodes4, w.data(), w.size() * sizeof(Child4), hipMemcpyHostToDevice));
@@ -1053,6 +1161,11 @@ DevScene dev_scene(const th_scene* s) {
   for (int k = 0; k < 3; ++k) {
     S.root_min[k] = s->root_min[k];
     S.root_max[k] = s->root_max[k];
+  }
+  S.ext_n = s->ext_n;
+  for (int k = 0; k < 8; ++k) {
+    S.ext_slot[k] = s->ext_slot[k];
+    S.ext_leaf[k] = s->ext_leaf[k];
   }
   return S;
 }

@@ -2,10 +2,12 @@
 //
 // A path is owned by a quad of lanes (lanes 4k..4k+3 of a wave).  The quad walks a
 // 4-wide hierarchy obtained by collapsing every interior node of the reference's
-// median-split binary tree with its interior children (same boxes, same leaves, same
-// leaf primitive order): at an interior node lane j tests child j's box, at a leaf lane
-// j tests primitive j, and the quad combines results with DPP quad permutes (no LDS
-// round trip).  Children are visited nearest first; the farther hit children go on a
+// median-split binary tree with its interior children (same leaves, same leaf primitive
+// order): at an interior node lane j tests child j's box, at a leaf lane j tests
+// primitive j, and the quad combines results with DPP quad permutes (no LDS round trip).
+// A scene may name up to 8 extracted primitives (DevScene::ext_*: large ones that would
+// inflate every box above them); the quad tests those first, and the 4-wide boxes are
+// the boxes of what remains below each node (the reference's boxes otherwise).  Children are visited nearest first; the farther hit children go on a
 // per-quad LDS stack with their entry distance, re-checked against the current best
 // when popped.  The closest hit is resolved exactly as the reference resolves it:
 // smallest distance, ties to the reference's first-found primitive (leaves in
@@ -135,6 +137,17 @@ __device__ __forceinline__ bool box_bf(float mnx, float mny, float mnz, float mx
 }
 
 // ------------------------------------------------------------------ traversal
+// a[B + lq] with compile-time indices (a run-time index into an array passed by value
+// would go through scratch memory)
+template <int B>
+__device__ __forceinline__ uint32_t quad_pick(const uint32_t (&a)[8], uint32_t lq) {
+  uint32_t v = a[B];
+  v = lq == 1u ? a[B + 1] : v;
+  v = lq == 2u ? a[B + 2] : v;
+  v = lq == 3u ? a[B + 3] : v;
+  return v;
+}
+
 // anyhit false: closest hit (reference tie-breaking).  anyhit true: any hit with
 // t < limit.  Every value that steers control flow is identical in the four lanes of the
 // quad; different quads of a wave may run different modes in the same loop.  A quad
@@ -154,9 +167,50 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
   *best_t = TH_FLT_MAX;
   *best_slot = THD_NO_SLOT;
   const __amdgpu_buffer_rsrc_t nrs = node_rsrc(nodes4, S.node4_bytes);
-  if (!active || !box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0],
-                         S.root_max[1], S.root_max[2], r, best, &te))
-    return false;
+  if (!active) return false;
+  // the scene's extracted primitives first (the 4-wide boxes leave them out), lane j of
+  // the quad testing primitives j and j + 4; candidates merged in lane order with the
+  // reference tie rule
+#pragma unroll
+  for (int base = 0; base < 8; base += 4) {
+    if ((uint32_t)base < S.ext_n) {
+      const uint32_t k = (uint32_t)base + lq;
+      const uint32_t slot = base == 0 ? quad_pick<0>(S.ext_slot, lq) : quad_pick<4>(S.ext_slot, lq);
+      const uint32_t leaf = base == 0 ? quad_pick<0>(S.ext_leaf, lq) : quad_pick<4>(S.ext_leaf, lq);
+      float t = INF;
+      if (k < S.ext_n) {
+        uint32_t tag;
+        t = slot_t(S, slot, o, d, &tag);
+      }
+      if (anyhit) {
+        const bool h = t < limit;
+        if (qbcast_u<0>(h) | qbcast_u<1>(h) | qbcast_u<2>(h) | qbcast_u<3>(h)) {
+          *best_t = 0.0f;
+          *best_slot = slot;
+          return true;
+        }
+      } else {
+        const float tq[4] = {qbcast_f<0>(t), qbcast_f<1>(t), qbcast_f<2>(t), qbcast_f<3>(t)};
+        const uint32_t sq[4] = {qbcast_u<0>(slot), qbcast_u<1>(slot), qbcast_u<2>(slot), qbcast_u<3>(slot)};
+        const uint32_t lfq[4] = {qbcast_u<0>(leaf), qbcast_u<1>(leaf), qbcast_u<2>(leaf), qbcast_u<3>(leaf)};
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+          if (tq[w] < TH_FLT_MAX &&
+              (tq[w] < best || (tq[w] == best && (lfq[w] > bleaf || (lfq[w] == bleaf && sq[w] < bslot))))) {
+            best = tq[w];
+            bslot = sq[w];
+            bleaf = lfq[w];
+          }
+      }
+    }
+  }
+  if (!box_bf(S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1], S.root_max[2], r,
+              best, &te)) {
+    if (anyhit) return false;
+    *best_t = best;
+    *best_slot = bslot;
+    return bslot != THD_NO_SLOT;
+  }
   st.sp = 0;
   uint32_t e = root4;
   for (;;) {
