@@ -362,6 +362,11 @@ int32_t th_scene_update(th_scene* scene, uint32_t change_mask, const th_scene_de
  * reference, which rebuilds every frame; a caller rebuilds (TH_CHANGE_BVH) once it passes
  * a bound of its choice. */
 int32_t th_scene_refit_quality(const th_scene* scene, float* ratio);
+/* The 4-wide traversal structure derived from the uploaded tree: *extracted = primitives
+ * tested before the tree by every 4-wide kernel (large primitives whose boxes would
+ * inflate their ancestors'; 0-8), *nodes4 = 4-wide nodes.  Either pointer may be null.
+ * No counterpart in the reference (its shader walks the uploaded binary tree). */
+int32_t th_scene_traversal_info(const th_scene* scene, uint32_t* extracted, uint32_t* nodes4);
 void th_scene_destroy(th_scene* scene);
 
 /* One dispatch: params.samples samples per pixel folded into the progressive

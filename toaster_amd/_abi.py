@@ -176,6 +176,7 @@ SIGNATURES = {
     "th_device_count": (C.c_int32, [C.POINTER(C.c_int32)]),
     "th_device_info": (C.c_int32, [C.c_int32, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]),
     "th_scene_refit_quality": (C.c_int32, [P, C.POINTER(f32)]),
+    "th_scene_traversal_info": (C.c_int32, [P, C.POINTER(u32), C.POINTER(u32)]),
     "th_context_create": (C.c_int32, [C.c_int32, C.POINTER(P)]),
     "th_context_destroy": (None, [P]),
     "th_last_error": (C.c_char_p, []),

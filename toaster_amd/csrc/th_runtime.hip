@@ -1454,6 +1454,13 @@ int32_t th_scene_refit_quality(const th_scene* s, float* ratio) {
   return TH_OK;
 }
 
+int32_t th_scene_traversal_info(const th_scene* s, uint32_t* extracted, uint32_t* nodes4) {
+  if (!s) return th_set_error(TH_E_INVALID, "null argument");
+  if (extracted) *extracted = s->ext_n;
+  if (nodes4) *nodes4 = s->node4_count;
+  return TH_OK;
+}
+
 void th_scene_destroy(th_scene* s) {
   if (!s) return;
   (void)hipSetDevice(s->ctx->device);

@@ -153,6 +153,12 @@ class GpuScene:
         A.check(self._lib.th_scene_refit_quality(self._h, C.byref(r)))
         return float(r.value)
 
+    def traversal_info(self):
+        """(primitives the 4-wide kernels test before the tree, 4-wide nodes)."""
+        e, n = C.c_uint32(), C.c_uint32()
+        A.check(self._lib.th_scene_traversal_info(self._h, C.byref(e), C.byref(n)))
+        return int(e.value), int(n.value)
+
     def params(self, samples=None, frame_index=None, accumulated_samples=None, max_bounces=None):
         p = A.GpuRenderParams.from_buffer_copy(self.packed.params)
         if samples is not None:
