@@ -64,10 +64,12 @@ struct DevScene {
   // the graph advances between replays; null for an ordinary launch (params rule)
   const uint32_t* frame_state;
   // primitives the 4-wide traversal tests before the tree (their boxes are left out of
-  // the 4-wide boxes): slot and the first slot of its reference leaf (the tie rank)
+  // the 4-wide boxes): slot, the first slot of its reference leaf (the tie rank) and that
+  // leaf's reference node (whose box the reference must enter to reach the primitive)
   uint32_t ext_n;
   uint32_t ext_slot[8];
   uint32_t ext_leaf[8];
+  uint32_t ext_node[8];
 };
 
 // The frame seed and accumulated sample count of a launch: from the params, or (FS, a

@@ -61,6 +61,7 @@ struct th_scene {
   uint32_t root4 = 0, depth4 = 0, node4_count = 0;
   // primitives the 4-wide traversal tests before the tree (see prepare_bvh)
   uint32_t ext_n = 0, ext_slot[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ext_leaf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t ext_node[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float4* d_slots = nullptr;
   float4* d_accum = nullptr;
   uint32_t node_count = 0, prim_count = 0, env_count = 0, tex_count = 0;

@@ -925,7 +925,11 @@ int32_t prepare_env(th_scene* s, const th_scene_desc* d) {
 // candidate).  The 4-wide records then carry, per reference node, the box of what remains
 // below it; a node with nothing left is left out.  The candidates stay in their leaves
 // (where a second test finds the same distance and rank, so closest hits and their ties
-// are unchanged).  Boxes are folded from each primitive's packer box (triangle vertex
+// are unchanged).  Before a candidate's hit counts, the traversal tests the box of its
+// reference leaf (as the reference's slab test computes it, without a far bound): the
+// reference reaches the primitive only through that box and its ancestors, whose slab
+// intervals contain the leaf's, so a ray the leaf's rounding turns away misses the
+// primitive in both.  Boxes are folded from each primitive's packer box (triangle vertex
 // min/max -+ 1e-5, sphere center -+ radius).
 static void select_extracted(th_scene* s, const th_scene_desc* d, std::vector<float>& box) {
   s->ext_n = 0;
@@ -1010,6 +1014,7 @@ static void select_extracted(th_scene* s, const th_scene_desc* d, std::vector<fl
           if (cand[j] == q) {
             s->ext_slot[j] = q;
             s->ext_leaf[j] = n.first_or_right;
+            s->ext_node[j] = i;
           }
       }
     }
@@ -1166,6 +1171,7 @@ DevScene dev_scene(const th_scene* s) {
   for (int k = 0; k < 8; ++k) {
     S.ext_slot[k] = s->ext_slot[k];
     S.ext_leaf[k] = s->ext_leaf[k];
+    S.ext_node[k] = s->ext_node[k];
   }
   return S;
 }

@@ -177,10 +177,17 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
       const uint32_t k = (uint32_t)base + lq;
       const uint32_t slot = base == 0 ? quad_pick<0>(S.ext_slot, lq) : quad_pick<4>(S.ext_slot, lq);
       const uint32_t leaf = base == 0 ? quad_pick<0>(S.ext_leaf, lq) : quad_pick<4>(S.ext_leaf, lq);
+      const uint32_t node = base == 0 ? quad_pick<0>(S.ext_node, lq) : quad_pick<4>(S.ext_node, lq);
       float t = INF;
       if (k < S.ext_n) {
+        // the reference reaches the primitive only through its leaf's box (and the
+        // ancestors', whose slab intervals contain the leaf's)
+        const float4* nb = (const float4*)(S.nodes + node);
+        const float4 mn = nb[0], mx = nb[1];
         uint32_t tag;
-        t = slot_t(S, slot, o, d, &tag);
+        const float tp = slot_t(S, slot, o, d, &tag);
+        float tl;
+        if (box_bf(mn.x, mn.y, mn.z, mx.x, mx.y, mx.z, r, TH_FLT_MAX, &tl)) t = tp;
       }
       if (anyhit) {
         const bool h = t < limit;
