@@ -1026,6 +1026,7 @@ int32_t prepare_bvh(th_scene* s, const th_scene_desc* d) {
   th_context* ctx = s->ctx;
   s->node_count = d->bvh_node_count;
   s->prim_count = d->bvh_primitive_count;
+  s->ext_n = 0;  // chosen again below for a new tree
   if (s->node_count) {
     const TH_GpuBvhNode& r = d->bvh_nodes[0];
     s->root_entry = th_entry_of(r, 0u);
