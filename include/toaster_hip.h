@@ -492,8 +492,9 @@ int32_t th_guide_set_params(th_guide* guide, const float* params, uint32_t count
 int32_t th_guide_set_hyper(th_guide* guide, float lr, float beta1, float beta2, float eps);
 /* Network outputs (count x 40) for host-supplied features (count x 48). */
 int32_t th_guide_forward(th_guide* guide, const float* features, uint32_t count, float* out);
-/* Network outputs (count x 40) of the guided renderer's fused inference kernel (the
- * encoding computed on the device, then the four layers) for host-supplied diffuse
+/* Network outputs (count x 40, rounded to binary16 as the renderer stores them) of the
+ * guided renderer's fused inference kernel (the encoding computed on the device, then the
+ * four layers) for host-supplied diffuse
  * vertices: count x 9 floats (position, normal, outgoing direction), with the position
  * normalised to [0,1]^3 by the box [bmin, bmax] as the renderer normalises it by the
  * scene bounds.  For checking that kernel; the renderer calls it on its own queues. */
@@ -564,6 +565,10 @@ int32_t th_render_guided(th_context* ctx, th_scene* scene, th_guide* guide,
                          const TH_GpuRenderParams* params, const TH_GpuCamera* camera,
                          const th_guided_options* options, th_guided_stats* stats);
 int32_t th_guide_mfma_selftest(th_context* ctx, const float* a, const float* b, float* c);
+/* One v_mfma_f32_32x32x16_bf16 with the fragment mapping of the renderer's inference
+ * kernel: C (32x32) = A (32x16) B (16x32), inputs rounded to BF16.  No counterpart in the
+ * reference. */
+int32_t th_guide_mfma32_selftest(th_context* ctx, const float* a, const float* b, float* c);
 /* The device's float -> BF16 rounding (the gfx950 conversion instruction, used by every
  * network kernel) against integer round-to-nearest-even over all 2^32 float patterns:
  * *mismatches counts non-NaN inputs that differ (expected 0), *nan_mismatches (may be

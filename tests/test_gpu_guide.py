@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import guide as G
-from toaster_amd.guide import Guide, bf16_selftest, mfma_selftest
+from toaster_amd.guide import Guide, bf16_selftest, mfma32_selftest, mfma_selftest
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,18 @@ def test_mfma_fragment_mapping(gpu_ctx):
     b = rng.normal(size=(32, 16)).astype(np.float32)
     print("case mfma selftest", flush=True)
     c = mfma_selftest(gpu_ctx, a, b)
+    ref = G.bf16(a) @ G.bf16(b)
+    assert np.allclose(c, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_mfma32_fragment_mapping(gpu_ctx):
+    """The 32x32x16 mapping the inference kernel relies on (lane s + 32 h holds
+    A[s][8h..8h+7] and B[8h..8h+7][s]; accumulator r of that lane is D[8 (r/4) + 4 h + r % 4][s])."""
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(32, 16)).astype(np.float32)
+    b = rng.normal(size=(16, 32)).astype(np.float32)
+    print("case mfma32 selftest", flush=True)
+    c = mfma32_selftest(gpu_ctx, a, b)
     ref = G.bf16(a) @ G.bf16(b)
     assert np.allclose(c, ref, rtol=1e-5, atol=1e-5)
 
@@ -73,8 +85,10 @@ def test_fused_inference_matches_oracle(gpu_ctx, n):
     layers) on host-supplied vertices against the oracle's encoding and forward pass.
     The oracle encodes with the device's float32 argument roundings
     (encode_device_rounding); the transcendentals themselves are the hardware's, within a
-    few ulp, so a feature rarely rounds to the neighbouring BF16 value -- the same row
-    tolerance as the forward test covers it."""
+    few ulp, so a feature rarely rounds to the neighbouring BF16 value -- the forward
+    test's row tolerance covers it.  The kernel stores its outputs as binary16, so each
+    output may also differ by one binary16 ulp (2^-10 of its magnitude) from the oracle's
+    exact value."""
     rng = np.random.default_rng(500 + n)
     g = Guide(gpu_ctx, seed=3)
     p = rng.normal(0, 0.25, G.N_PARAMS).astype(np.float32)
@@ -93,7 +107,7 @@ def test_fused_inference_matches_oracle(gpu_ctx, n):
     p01 = ((pos - bmin) * scale).astype(np.float32)
     ref, _ = G.forward(p, G.encode_device_rounding(p01, nrm, wo), emulate_bf16=True)
     err = np.abs(O - ref)
-    row_ok = np.all(err <= 1e-4 * (1 + np.abs(ref)), axis=1)
+    row_ok = np.all(err <= 1e-4 * (1 + np.abs(ref)) + 2.0 ** -10 * np.abs(ref), axis=1)
     print(f"  rows within 1e-4: {row_ok.mean():.5f}, max err {err.max():.3g}")
     assert row_ok.mean() >= 0.99
     assert np.all(err <= 0.1 * (1 + np.abs(ref))), float(err.max())

@@ -216,6 +216,7 @@ SIGNATURES = {
     "th_guide_infer_vertices": (C.c_int32, [P, P, u32, P, P, P]),
     "th_guide_train_batch": (C.c_int32, [P, P, P, P, u32, C.c_int32, P, P]),
     "th_guide_mfma_selftest": (C.c_int32, [P, P, P, P]),
+    "th_guide_mfma32_selftest": (C.c_int32, [P, P, P, P]),
     "th_bf16_selftest": (C.c_int32, [P, P, P]),
     "th_render_guided": (C.c_int32, [P, P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
                                      C.POINTER(GuidedOptions), C.POINTER(GuidedStats)]),

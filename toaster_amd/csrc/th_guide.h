@@ -57,6 +57,25 @@ TH_HD uint32_t f2bf2(float lo, float hi) {
   return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
 #endif
 }
+// IEEE binary16 bits -> float (the network's stored outputs)
+TH_HD float h2f(uint16_t h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (float)__builtin_bit_cast(_Float16, h);
+#else
+  const uint32_t sgn = (uint32_t)(h & 0x8000u) << 16, ex = (h >> 10) & 0x1fu, man = h & 0x3ffu;
+  union { float f; uint32_t u; } v;
+  if (ex == 0x1fu) {
+    v.u = sgn | 0x7f800000u | (man << 13);
+  } else if (ex != 0u) {
+    v.u = sgn | ((ex + 112u) << 23) | (man << 13);
+  } else {  // zero or subnormal: man * 2^-24
+    v.f = (float)man * 5.9604645e-8f;
+    v.u |= sgn;
+  }
+  return v.f;
+#endif
+}
+
 TH_HD float bf2f(uint16_t b) {
   union { float f; uint32_t u; } v;
   v.u = (uint32_t)b << 16;

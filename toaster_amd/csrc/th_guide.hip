@@ -568,41 +568,66 @@ __global__ void k_mfma_selftest(const float* A, const float* B, float* C) {
   for (int i = 0; i < 4; ++i) C[(g * 4 + i) * 16 + r] = acc[i];
 }
 
+// One v_mfma_f32_32x32x16_bf16 (gfx950): lane l = s + 32 h holds A[s][8h .. 8h + 7] and
+// B[8h .. 8h + 7][s]; accumulator element r of lane l is D[8 (r / 4) + 4 h + r % 4][s]
+// (checked on the device by th_guide_mfma32_selftest against a host product).
+typedef float tg_f32x16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ tg_f32x16 mfma32(tg_bf16x8 a, tg_bf16x8 b, tg_f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__global__ void k_mfma32_selftest(const float* A, const float* B, float* C) {
+  // A 32x16 and B 16x32 row-major FP32 (rounded to BF16 here); C 32x32
+  const uint32_t l = threadIdx.x, s = l & 31u, h = l >> 5;
+  uint16_t av[8], bv[8];
+  for (int e = 0; e < 8; ++e) {
+    av[e] = tg::f2bf(A[s * 16 + h * 8 + e]);
+    bv[e] = tg::f2bf(B[(h * 8 + e) * 32 + s]);
+  }
+  tg_f32x16 acc = {};
+  acc = mfma32(__builtin_bit_cast(tg_bf16x8, *(uint4*)av), __builtin_bit_cast(tg_bf16x8, *(uint4*)bv), acc);
+  for (int r = 0; r < 16; ++r) C[(8 * (r / 4) + 4 * h + r % 4) * 32 + s] = acc[r];
+}
+
 // Fused inference over a queue of deferred diffuse vertices.  Row i of the queue is the
-// vertex (p, n, wo) stored in queue order (vp, vn, vw).  A block of TG_INFER_WAVES waves
-// copies the network into LDS once, every weight fragment in the order the MFMA reads
-// it (fragment f, lane l at 16 * (64 f + l): one conflict-free 16-byte load per
-// fragment), then each wave walks its own 16-row tiles of the queue.  The products run
-// with the operands of k_gemm_rows swapped: the A operand is 16 weight rows and the B
-// operand 32 inputs of the tile's 16 rows, so lane (r, g) ends each 16-output block nb
-// with outputs nb*16 + 4g .. 4g + 3 of row r -- four consecutive values of one row,
-// written to the wave's LDS activation tile with one 8-byte store (BF16, row stride 72
-// elements) and to the output with one 16-byte store.  The biases (in registers) enter as
-// the accumulator's initial value.  The first layer's B fragment needs no tile: lane (r, g)
-// holds inputs 8g .. 8g + 7 and 32 + 8g .. 32 + 8g + 7 of row r, which it encodes itself
-// -- lanes g < 3 the sin/cos pairs of position axis g, lane g = 3 the bins of n.x and
-// n.y, lanes g = 0 and 1 also the bins of (n.z, wo.x) and (wo.y, wo.z), the rest zero
-// padding -- with the same feature functions as the training records.  ReLU is applied
-// to the BF16 pairs as 16-bit integer maxima with zero (a BF16 value and its bit pattern
-// as an int16 have the same sign, and max(bf16(v), 0) = bf16(max(v, 0)) for every
-// non-NaN v).  Loads are clamped into the queue, and stores cover whole tiles (segments
-// hold a multiple of 64 rows) through a buffer resource, the padding outputs 40..47 sent
-// past its end (discarded by the hardware), so the loop has no branch and the next
-// tile's loads never wait for a store.
+// vertex (p, n, wo) stored in queue order (vp, vn, vw).  A wave takes 32 rows at a time
+// on v_mfma_f32_32x32x16_bf16, lane (s, h) = s + 32 h serving row s: each layer is
+// D[32 outputs x 32 rows] = W[32 x 16] X^T[16 x 32] per output block mb (two) and input
+// block kb, the weights as the A operand (fragments in LDS, copied there once per block of
+// TG_INFER_WAVES waves in the order the MFMA reads them: one conflict-free 16-byte load
+// each) and the inputs as the B operand, so lane (s, h) ends with outputs
+// 32 mb + 8 a + 4 h + c (a, c < 4) of row s.  The next layer's B fragment for input block
+// kb' needs outputs 16 kb' + 8 h .. + 7 of row s: four of them are the lane's own and four
+// its partner's (lane s + 32 (1 - h)), exchanged by one v_permlane32_swap per packed BF16
+// pair -- no LDS round trip between layers.  Biases enter through an extra input block
+// holding 1 in its first three inputs against the bias split into three BF16 parts
+// (hi + mid + lo is the FP32 bias exactly).  ReLU runs on the packed BF16 pairs as 16-bit
+// integer maxima with zero (a BF16 value and its bit pattern as an int16 have the same
+// sign, and max(bf16(v), 0) = bf16(max(v, 0)) for every non-NaN v).  The first layer's B
+// fragments are encoded by the lane itself with the training records' feature
+// functions: input block 0 the sin/cos pairs of position axis h, block 1 those of the z
+// axis (h = 0) or the bins of n.x and n.y (h = 1), block 2 the bins of (n.z, wo.x) or
+// (wo.y, wo.z).  Loads are clamped into the queue and prefetched a tile ahead into two
+// register sets used in turn; stores cover whole tiles (segments hold a multiple of 64
+// rows) through a buffer resource, so the loop has no branch and the next tile's loads
+// never wait for a store.  Outputs are stored as binary16 (relative rounding 2^-11, below
+// the BF16 rounding of every hidden activation): the rows the shading kernel reads back
+// are 80 bytes instead of 160, and the kernel's time is mostly these stores.
 #ifndef TG_INFER_WAVES
 #define TG_INFER_WAVES 4
 #endif
-#define TG_INFER_FRAGS 30  // 4 x 2 per hidden-output layer (three), 3 x 2 for the 40 outputs
-#define TG_TILE_LD 72      // activation tile row stride (BF16 elements; 16-byte aligned rows)
-__device__ __forceinline__ int infer_frag(int L, int nb, int kb) {
-  return L < 3 ? (L * 4 + nb) * 2 + kb : 24 + nb * 2 + kb;
+#define TG_INFER_FRAGS 38  // per layer and output block: 3 or 4 input blocks + the bias block
+__device__ __forceinline__ int infer_frag(int L, int mb, int kb) {  // kb == nkb: the bias block
+  return L == 0 ? mb * 4 + kb : 8 + (L - 1) * 10 + mb * 5 + kb;
 }
 
-// LDS ordering between the lanes of one wave (its own tile: no other wave touches it)
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Two floats -> packed binary16 (round to nearest even), clamped to +-65504 first so a
+// large output stays finite.
+__device__ __forceinline__ uint32_t f2h2(float x, float y) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef _Float16 hv2 __attribute__((ext_vector_type(2)));
+  const f2 v = {__builtin_amdgcn_fmed3f(x, -65504.0f, 65504.0f), __builtin_amdgcn_fmed3f(y, -65504.0f, 65504.0f)};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, hv2));
 }
 
 __device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
@@ -611,74 +636,95 @@ __device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
   return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), z));
 }
 
-__device__ __forceinline__ void infer_tile(const uint4* __restrict__ WF, const float4 (&bq)[4][4],
-                                           uint16_t* T, uint32_t l, uint32_t base, uint32_t count, float4 a,
-                                           float4 b, float4 c, float4 bmin, float4 bscale,
-                                           __amdgpu_buffer_rsrc_t orsrc) {
+__device__ __forceinline__ void infer_tile(const uint4* __restrict__ WF, uint32_t l, uint32_t base,
+                                           uint32_t count, float4 a, float4 b, float4 c, float4 bmin,
+                                           float4 bscale, __amdgpu_buffer_rsrc_t orsrc) {
   typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
-  const uint32_t r = l & 15u, g = l >> 4;
-  const bool in = base + r < count;
+  const uint32_t s = l & 31u, h = l >> 5;
+  const bool in = base + s < count;
   const v3 p01 = in ? th::mk((a.x - bmin.x) * bscale.x, (a.y - bmin.y) * bscale.y, (a.z - bmin.z) * bscale.z)
                     : th::mk(0.5f, 0.5f, 0.5f);
   const v3 n = in ? th::mk(b.x, b.y, b.z) : th::mk(0.0f, 1.0f, 0.0f);
   const v3 wo = in ? th::mk(c.x, c.y, c.z) : n;
-  // position axis g (lanes g < 3) and the two direction components whose bins this
-  // lane encodes
-  const float xg = g == 0u ? p01.x : (g == 1u ? p01.y : p01.z);
-  const float u0 = g == 0u ? n.z : (g == 1u ? wo.y : n.x);
-  const float u1 = g == 0u ? wo.x : (g == 1u ? wo.z : n.y);
-  const v3 px = th::mk(xg, 0.0f, 0.0f), nu0 = th::mk(u0, 0.0f, 0.0f), nu1 = th::mk(u1, 0.0f, 0.0f);
-  uint32_t pos[4], bins[4];
+  // ---- layer 0 inputs: three blocks of 16 (features 16 kb + 8 h + e)
+  const v3 wz = th::mk(0.0f, 0.0f, 0.0f);
+  auto sincos4 = [&](float x, v4u32& out) {  // the four octaves of axis value x
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    float s, cs;
-    tg::encode_pos_pair(2 * t, px, &s, &cs);  // axis 0 of px = axis g, octave t
-    pos[t] = tg::f2bf2(s, cs);
-  }
+    for (int t = 0; t < 4; ++t) {
+      float sn, cs;
+      tg::encode_pos_pair(2 * t, th::mk(x, 0.0f, 0.0f), &sn, &cs);
+      out[t] = tg::f2bf2(sn, cs);
+    }
+  };
+  auto bins2 = [&](float u0, float u1, v4u32& out) {  // four bins of u0, then of u1
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {  // feature 24 + b reads component 0 of its normal argument
-    bins[t] = tg::f2bf2(tg::encode_feature(6 * TG_POS_OCT + 2 * t, p01, nu0, wo),
-                        tg::encode_feature(6 * TG_POS_OCT + 2 * t + 1, p01, nu0, wo));
-    bins[2 + t] = tg::f2bf2(tg::encode_feature(6 * TG_POS_OCT + 2 * t, p01, nu1, wo),
-                            tg::encode_feature(6 * TG_POS_OCT + 2 * t + 1, p01, nu1, wo));
-  }
-  v4u32 x0, x1;
+    for (int t = 0; t < 2; ++t) {
+      out[t] = tg::f2bf2(tg::encode_feature(6 * TG_POS_OCT + 2 * t, p01, th::mk(u0, 0.0f, 0.0f), wz),
+                         tg::encode_feature(6 * TG_POS_OCT + 2 * t + 1, p01, th::mk(u0, 0.0f, 0.0f), wz));
+      out[2 + t] = tg::f2bf2(tg::encode_feature(6 * TG_POS_OCT + 2 * t, p01, th::mk(u1, 0.0f, 0.0f), wz),
+                             tg::encode_feature(6 * TG_POS_OCT + 2 * t + 1, p01, th::mk(u1, 0.0f, 0.0f), wz));
+    }
+  };
+  v4u32 x0, x1, x2, zs, zb;
+  sincos4(h ? p01.y : p01.x, x0);
+  sincos4(p01.z, zs);
+  bins2(n.x, n.y, zb);
+  bins2(h ? wo.y : n.z, h ? wo.z : wo.x, x2);
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    x0[t] = g < 3u ? pos[t] : bins[t];
-    x1[t] = g < 2u ? bins[t] : 0u;
-  }
-  tg_bf16x8 b0 = __builtin_bit_cast(tg_bf16x8, x0), b1 = __builtin_bit_cast(tg_bf16x8, x1);
+  for (int t = 0; t < 4; ++t) x1[t] = h ? zb[t] : zs[t];
+  tg_bf16x8 B[4];
+  B[0] = __builtin_bit_cast(tg_bf16x8, x0);
+  B[1] = __builtin_bit_cast(tg_bf16x8, x1);
+  B[2] = __builtin_bit_cast(tg_bf16x8, x2);
+  B[3] = B[2];  // unused by layer 0
+  // the bias block's inputs: 1, 1, 1 in the first three (lanes h = 0), zero elsewhere
+  const v4u32 one3 = {h ? 0u : 0x3F803F80u, h ? 0u : 0x00003F80u, 0u, 0u};
+  const tg_bf16x8 Bb = __builtin_bit_cast(tg_bf16x8, one3);
+  // the weight fragments are read from LDS for every tile: an opaque copy of the lane
+  // index keeps the compiler from hoisting all 38 loads out of the tile loop into
+  // registers (152 VGPRs, two waves per SIMD instead of four)
+  uint32_t li = l;
+  __asm__ volatile("" : "+v"(li));
 #pragma unroll
   for (int L = 0; L < 4; ++L) {
-    if (L > 0) {
-      b0 = lds8(T + r * TG_TILE_LD + g * 8);
-      b1 = lds8(T + r * TG_TILE_LD + 32 + g * 8);
-    }
-    const int nbn = L < 3 ? 4 : 3;  // the last layer has 40 outputs: three 16-wide blocks
-    tg_f32x4 acc[4];
+    const int nkb = L == 0 ? 3 : 4;
+    tg_f32x16 acc[2];
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      if (nb < nbn) {
-        const tg_f32x4 c0 = {bq[L][nb].x, bq[L][nb].y, bq[L][nb].z, bq[L][nb].w};
-        acc[nb] = mfma(__builtin_bit_cast(tg_bf16x8, WF[infer_frag(L, nb, 0) * 64 + l]), b0, c0);
-        acc[nb] = mfma(__builtin_bit_cast(tg_bf16x8, WF[infer_frag(L, nb, 1) * 64 + l]), b1, acc[nb]);
-      }
+    for (int mb = 0; mb < 2; ++mb) {
+      acc[mb] = mfma32(__builtin_bit_cast(tg_bf16x8, WF[infer_frag(L, mb, nkb) * 64 + li]), Bb, tg_f32x16{});
+#pragma unroll
+      for (int kb = 0; kb < nkb; ++kb)
+        acc[mb] = mfma32(__builtin_bit_cast(tg_bf16x8, WF[infer_frag(L, mb, kb) * 64 + li]), B[kb], acc[mb]);
     }
     if (L < 3) {
-      wave_lds_sync();  // every lane has read the tile before it is overwritten
+      // B[kb'] (features 16 kb' + 8 h + e) from output block kb' / 2, rows a0 = 2 (kb' % 2)
+      // and a0 + 1 of the accumulator groups: X = this lane's group a0, Y = its group a0 + 1;
+      // swapping X of the upper half with Y of the lower gives the lower lanes (X, partner X)
+      // = features 0..7 of the block and the upper lanes (partner Y, Y) = features 8..15
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb)
-        *(uint2*)(T + r * TG_TILE_LD + nb * 16 + 4 * g) =
-            make_uint2(relu_bf16x2(tg::f2bf2(acc[nb][0], acc[nb][1])),
-                       relu_bf16x2(tg::f2bf2(acc[nb][2], acc[nb][3])));
-      wave_lds_sync();
+      for (int kb = 0; kb < 4; ++kb) {
+        const tg_f32x16& q = acc[kb >> 1];
+        const int a0 = 2 * (kb & 1);
+        uint32_t X0 = relu_bf16x2(tg::f2bf2(q[4 * a0 + 0], q[4 * a0 + 1]));
+        uint32_t X1 = relu_bf16x2(tg::f2bf2(q[4 * a0 + 2], q[4 * a0 + 3]));
+        uint32_t Y0 = relu_bf16x2(tg::f2bf2(q[4 * a0 + 4], q[4 * a0 + 5]));
+        uint32_t Y1 = relu_bf16x2(tg::f2bf2(q[4 * a0 + 6], q[4 * a0 + 7]));
+        const auto s0 = __builtin_amdgcn_permlane32_swap(X0, Y0, false, false);
+        const auto s1 = __builtin_amdgcn_permlane32_swap(X1, Y1, false, false);
+        const v4u32 v = {s0[0], s1[0], s0[1], s1[1]};
+        B[kb] = __builtin_bit_cast(tg_bf16x8, v);
+      }
     } else {
-      const uint32_t ob = ((base + r) * TG_OUT + 4u * g) * 4u;
+      // outputs 8 a + 4 h + c of row s (block 0) and 32 + 4 h + c (block 1, a = 0), rounded
+      // to binary16 (clamped to its range) and stored four at a time
+      typedef unsigned v2u32 __attribute__((ext_vector_type(2)));
+      const uint32_t ob = ((base + s) * TG_OUT + 4u * h) * 2u;
 #pragma unroll
-      for (int nb = 0; nb < 3; ++nb) {
-        const uint32_t off = (nb < 2 || g < 2u) ? ob + (uint32_t)nb * 64u : 0x80000000u;
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, acc[nb]), orsrc, off, 0, 0);
+      for (int a = 0; a < 5; ++a) {
+        const tg_f32x16& q = acc[a >> 2];
+        const int g = a & 3;
+        const v2u32 v = {f2h2(q[4 * g], q[4 * g + 1]), f2h2(q[4 * g + 2], q[4 * g + 3])};
+        __builtin_amdgcn_raw_buffer_store_b64(v, orsrc, ob + 16u * (uint32_t)a, 0, 0);
       }
     }
   }
@@ -688,54 +734,60 @@ __global__ void __launch_bounds__(64 * TG_INFER_WAVES) k_guide_infer(
     const uint16_t* __restrict__ woi, const float* __restrict__ bias, const uint32_t* __restrict__ count_ptr,
     uint32_t count_stride, uint32_t nseg, uint32_t seg_cap, const float4* __restrict__ vp,
     const float4* __restrict__ vn, const float4* __restrict__ vw, float4 bmin, float4 bscale,
-    float* __restrict__ O) {
+    uint16_t* __restrict__ O) {
   __shared__ uint4 WF[TG_INFER_FRAGS * 64];
-  __shared__ __align__(16) uint16_t TT[TG_INFER_WAVES][16 * TG_TILE_LD];
   // block b serves queue segment b % nseg (the grid is a multiple of nseg); wave w of the
-  // block takes tiles jb * TG_INFER_WAVES + w, then every nbs * TG_INFER_WAVES-th
+  // block takes 32-row tiles jb * TG_INFER_WAVES + w, then every nbs * TG_INFER_WAVES-th
   const uint32_t sg = blockIdx.x % nseg, jb = blockIdx.x / nseg, nbs = gridDim.x / nseg;
   const uint32_t count = count_ptr[(size_t)sg * count_stride];
-  if (jb * TG_INFER_WAVES * 16u >= count) return;  // uniform per block
+  if (jb * TG_INFER_WAVES * 32u >= count) return;  // uniform per block
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < TG_INFER_FRAGS * 64u; i += 64u * TG_INFER_WAVES) {
-    const uint32_t f = i >> 6, l = i & 63u, r = l & 15u, g = l >> 4;
-    const uint32_t L = f < 24u ? f >> 3 : 3u, nb = f < 24u ? (f >> 1) & 3u : (f - 24u) >> 1, kb = f & 1u;
-    WF[i] = *(const uint4*)(woi + L * 4096u + (nb * 16u + r) * 64u + kb * 32u + g * 8u);
+    const uint32_t f = i >> 6, l = i & 63u, o = l & 31u, h = l >> 5;
+    const uint32_t L = f < 8u ? 0u : 1u + (f - 8u) / 10u;
+    const uint32_t r = f < 8u ? f : (f - 8u) % 10u;
+    const uint32_t nkb = L == 0u ? 3u : 4u, mb = r / (nkb + 1u), kb = r % (nkb + 1u);
+    const uint32_t row = mb * 32u + o;
+    uint4 v;
+    if (kb < nkb) {
+      v = *(const uint4*)(woi + L * 4096u + row * 64u + kb * 16u + h * 8u);
+    } else {  // bias block: hi, mid, lo BF16 parts in the first three inputs of lanes h = 0
+      const float bv = bias[L * 64u + row];
+      const uint16_t hi = tg::f2bf(bv);
+      const float r1 = bv - tg::bf2f(hi);
+      const uint16_t mid = tg::f2bf(r1);
+      const uint16_t lo = tg::f2bf(r1 - tg::bf2f(mid));
+      v = h ? make_uint4(0u, 0u, 0u, 0u)
+            : make_uint4((uint32_t)hi | ((uint32_t)mid << 16), (uint32_t)lo, 0u, 0u);
+    }
+    WF[i] = v;
   }
   __syncthreads();
-  const uint32_t w = t >> 6, l = t & 63u;
-  // biases of this lane's outputs nb*16 + 4g .. 4g + 3, held in registers
-  float4 bq[4][4];
-#pragma unroll
-  for (int L = 0; L < 4; ++L)
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb)
-      if (L < 3 || nb < 3) bq[L][nb] = *(const float4*)(bias + L * 64 + nb * 16 + 4 * (l >> 4));
-  const uint32_t step = nbs * TG_INFER_WAVES * 16u, last = count - 1u;
-  uint32_t base = (jb * TG_INFER_WAVES + w) * 16u;
+  const uint32_t w = t >> 6, l = t & 63u, s = l & 31u;
+  const uint32_t step = nbs * TG_INFER_WAVES * 32u, last = count - 1u;
+  uint32_t base = (jb * TG_INFER_WAVES + w) * 32u;
   if (base >= count) return;  // no block barrier follows
   vp += (size_t)sg * seg_cap;
   vn += (size_t)sg * seg_cap;
   vw += (size_t)sg * seg_cap;
-  // this segment's rows of the output (seg_cap * TG_OUT floats, below 2^32 bytes)
+  // this segment's rows of the output (seg_cap * TG_OUT binary16 values, below 2^31 bytes)
   const __amdgpu_buffer_rsrc_t orsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(O + (size_t)sg * seg_cap * TG_OUT), (short)0, (int)(seg_cap * (uint32_t)TG_OUT * 4u), 0x00020000);
-  uint16_t* T = TT[w];
+      (void*)(O + (size_t)sg * seg_cap * TG_OUT), (short)0, (int)(seg_cap * (uint32_t)TG_OUT * 2u), 0x00020000);
   // two register sets of vertex rows, loaded one tile ahead and used in turn (a single
   // loop-carried set would be copied at the back edge, which waits for its loads)
-  uint32_t lr = std::min(base + (l & 15u), last);
+  uint32_t lr = std::min(base + s, last);
   float4 pa = vp[lr], na = vn[lr], wa = vw[lr];
   for (;;) {
-    lr = std::min(base + step + (l & 15u), last);
+    lr = std::min(base + step + s, last);
     const float4 pb = vp[lr], nb = vn[lr], wb = vw[lr];
-    infer_tile(WF, bq, T, l, base, count, pa, na, wa, bmin, bscale, orsrc);
+    infer_tile(WF, l, base, count, pa, na, wa, bmin, bscale, orsrc);
     base += step;
     if (base >= count) break;
-    lr = std::min(base + step + (l & 15u), last);
+    lr = std::min(base + step + s, last);
     pa = vp[lr];
     na = vn[lr];
     wa = vw[lr];
-    infer_tile(WF, bq, T, l, base, count, pb, nb, wb, bmin, bscale, orsrc);
+    infer_tile(WF, l, base, count, pb, nb, wb, bmin, bscale, orsrc);
     base += step;
     if (base >= count) break;
   }
@@ -837,12 +889,12 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
 // ====================================================================== renderer interface
 int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride, uint32_t nseg,
                     uint32_t seg_cap, const float4* vp, const float4* vn, const float4* vw,
-                    const float* bmin, const float* bmax, float* O, hipStream_t st) {
+                    const float* bmin, const float* bmax, uint16_t* O, hipStream_t st) {
   if (nseg == 0u) return th_set_error(TH_E_INVALID, "guide queue without segments");
   // persistent: as many blocks as stay resident (TG_INFER_BLOCKS_PER_CU: the LDS of
   // four fills a compute unit), so each block loads the network once and its waves walk
   // many tiles; a multiple of nseg, so every segment has the same number of blocks
-  const uint32_t tiles = (seg_cap + 16u * TG_INFER_WAVES - 1u) / (16u * TG_INFER_WAVES);
+  const uint32_t tiles = (seg_cap + 32u * TG_INFER_WAVES - 1u) / (32u * TG_INFER_WAVES);
   uint32_t per_seg = std::min<uint32_t>(
       tiles, (TG_INFER_BLOCKS_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1) + nseg - 1u) / nseg);
   const uint32_t grid = per_seg * nseg;
@@ -1023,12 +1075,12 @@ int32_t th_guide_infer_vertices(th_guide* g, const float* vertices, uint32_t cou
   }
   hipStream_t st = g->ctx->stream;
   float4 *dp = nullptr, *dn = nullptr, *dw = nullptr;
-  float* dO = nullptr;
+  uint16_t* dO = nullptr;
   uint32_t* dc = nullptr;
   int32_t rc = TH_OK;
   const size_t vb = (size_t)cap * sizeof(float4);
   if (hipMalloc((void**)&dp, vb) != hipSuccess || hipMalloc((void**)&dn, vb) != hipSuccess ||
-      hipMalloc((void**)&dw, vb) != hipSuccess || hipMalloc((void**)&dO, (size_t)cap * TG_OUT * 4) != hipSuccess ||
+      hipMalloc((void**)&dw, vb) != hipSuccess || hipMalloc((void**)&dO, (size_t)cap * TG_OUT * 2) != hipSuccess ||
       hipMalloc((void**)&dc, 4) != hipSuccess) {
     rc = th_set_error(TH_E_DEVICE, "inference workspace allocation failed");
   } else if (hipMemcpyAsync(dp, hp.data(), vb, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -1037,9 +1089,13 @@ int32_t th_guide_infer_vertices(th_guide* g, const float* vertices, uint32_t cou
              hipMemcpyAsync(dc, &count, 4, hipMemcpyHostToDevice, st) != hipSuccess) {
     rc = th_set_error(TH_E_DEVICE, "inference upload failed");
   } else if ((rc = guide_infer(g, dc, 1u, 1u, cap, dp, dn, dw, bmin, bmax, dO, st)) == TH_OK) {
-    if (hipMemcpyAsync(out, dO, (size_t)count * TG_OUT * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
+    std::vector<uint16_t> ho((size_t)count * TG_OUT);
+    if (hipMemcpyAsync(ho.data(), dO, ho.size() * 2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
       rc = th_set_error(TH_E_DEVICE, "inference readback failed");
+    } else {
+      for (size_t i = 0; i < ho.size(); ++i) out[i] = tg::h2f(ho[i]);
+    }
   }
   (void)hipStreamSynchronize(st);
   for (void* p : {(void*)dp, (void*)dn, (void*)dw, (void*)dO, (void*)dc})
@@ -1088,6 +1144,28 @@ int32_t th_guide_mfma_selftest(th_context* ctx, const float* A, const float* B, 
   (void)hipFree(dB);
   (void)hipFree(dC);
   if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("mfma selftest: ") + hipGetErrorString(e));
+  return TH_OK;
+}
+
+int32_t th_guide_mfma32_selftest(th_context* ctx, const float* A, const float* B, float* C) {
+  if (!ctx || !A || !B || !C) return th_set_error(TH_E_INVALID, "null argument");
+  TH_CHECK(hipSetDevice(ctx->device));
+  float *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  hipError_t e = hipMalloc((void**)&dA, 32 * 16 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&dB, 16 * 32 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&dC, 32 * 32 * 4);
+  if (e == hipSuccess) e = hipMemcpy(dA, A, 32 * 16 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dB, B, 16 * 32 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma32_selftest, dim3(1), dim3(64), 0, ctx->stream, dA, dB, dC);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(C, dC, 32 * 32 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  if (e != hipSuccess) return th_set_error(TH_E_DEVICE, std::string("mfma32 selftest: ") + hipGetErrorString(e));
   return TH_OK;
 }
 

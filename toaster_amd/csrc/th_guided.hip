@@ -55,7 +55,7 @@ struct GdBufs {
   uint32_t* cnt;  // GD_C: [0] paths at the first bounce, [5] diffuse total, [6] rows the
                   // network evaluated, [8 + c] records of chunk c; GD_QC: queue segments
   uint32_t seg_cap;  // paths per segment of q0, q1, qg (and its rows); qs holds twice that
-  float* O;       // guide outputs, TG_OUT per guide-queue row
+  uint16_t* O;    // guide outputs, TG_OUT binary16 values per guide-queue row
   uint16_t* rX;   // training records: BF16 features, 64 wide (zero beyond 48)
   float *rdir, *rw;
   uint16_t* tX;   // the records of each chunk made contiguous for training (rows of rX,
@@ -523,14 +523,16 @@ __global__ void __launch_bounds__(64) gd_shade_b(GdBufs B, GdArgs A, DevScene S,
       tg::Mixture mix;
       if (use_guide) {
         float o[TG_OUT];
-        const float4* o4 = (const float4*)(B.O + (size_t)i * TG_OUT);  // 160-byte rows
+        const uint4* o8 = (const uint4*)(B.O + (size_t)i * TG_OUT);  // 80-byte rows, 8 values per load
 #pragma unroll
-        for (int j = 0; j < TG_OUT / 4; ++j) {
-          const float4 v = o4[j];
-          o[4 * j] = v.x;
-          o[4 * j + 1] = v.y;
-          o[4 * j + 2] = v.z;
-          o[4 * j + 3] = v.w;
+        for (int j = 0; j < TG_OUT / 8; ++j) {
+          const uint4 v = o8[j];
+          const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            o[8 * j + 2 * q] = tg::h2f((uint16_t)(w[q] & 0xffffu));
+            o[8 * j + 2 * q + 1] = tg::h2f((uint16_t)(w[q] >> 16));
+          }
         }
         tg::decode(o, &mix);
       }
@@ -745,7 +747,7 @@ static int32_t gd_reserve(th_scene* s, uint32_t n, uint32_t rcap) {
       (rc = gd_alloc(w, (void**)&b.q1, qu)) || (rc = gd_alloc(w, (void**)&b.qs, 2 * qu)) ||
       (rc = gd_alloc(w, (void**)&b.qg, qu)) ||
       (rc = gd_alloc(w, (void**)&b.cnt, GD_C(GD_CNT_LINES) * sizeof(uint32_t))) ||
-      (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(float))) ||
+      (rc = gd_alloc(w, (void**)&b.O, rows * TG_OUT * sizeof(uint16_t))) ||
       (rc = gd_alloc(w, (void**)&b.rX, (size_t)rcap * 64 * sizeof(uint16_t))) ||
       (rc = gd_alloc(w, (void**)&b.rdir, (size_t)rcap * 3 * sizeof(float))) ||
       (rc = gd_alloc(w, (void**)&b.tX, (size_t)rcap * 64 * sizeof(uint16_t))) ||

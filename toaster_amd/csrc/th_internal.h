@@ -105,10 +105,10 @@ int32_t th_check_render_params(const th_scene* s, const TH_GpuRenderParams* p);
 // Guided wavefront (th_guided.hip) and the network kernels it drives (th_guide.hip).
 void th_guided_free(th_guided_ws* w);
 // Inference over a queue in `nseg` segments: segment s holds d_count[s * count_stride]
-// rows starting at row s * seg_cap of vp, vn, vw and O.
+// rows starting at row s * seg_cap of vp, vn, vw and O (TG_OUT binary16 outputs per row).
 int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride, uint32_t nseg,
                     uint32_t seg_cap, const float4* vp, const float4* vn, const float4* vw,
-                    const float* bmin, const float* bmax, float* O, hipStream_t st);
+                    const float* bmin, const float* bmax, uint16_t* O, hipStream_t st);
 int32_t guide_last_loss(th_guide* g, float* out);
 // One training step on min(*d_count, cap) rows (a no-op below min_count) with the count
 // read on the device; d_stats (optional) accumulates samples offered, trained, steps applied.

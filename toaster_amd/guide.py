@@ -87,6 +87,17 @@ class Guide:
             pass
 
 
+def mfma32_selftest(ctx: GpuContext, a, b):
+    """C = A (32x16) B (16x32) through one v_mfma_f32_32x32x16_bf16 with the inference
+    kernel's fragment mapping (inputs rounded to BF16)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == (32, 16) and b.shape == (16, 32)
+    c = np.empty((32, 32), np.float32)
+    A.check(A.lib().th_guide_mfma32_selftest(ctx.handle, _ptr(a), _ptr(b), _ptr(c)))
+    return c
+
+
 def mfma_selftest(ctx: GpuContext, a, b):
     """C = A (16x32) B (32x16) through one v_mfma_f32_16x16x32_bf16 with this library's
     fragment mapping (inputs rounded to BF16)."""
