@@ -614,7 +614,7 @@ __global__ void k_mfma32_selftest(const float* A, const float* B, float* C) {
 // the BF16 rounding of every hidden activation): the rows the shading kernel reads back
 // are 80 bytes instead of 160, and the kernel's time is mostly these stores.
 #ifndef TG_INFER_WAVES
-#define TG_INFER_WAVES 4
+#define TG_INFER_WAVES 16
 #endif
 #define TG_INFER_FRAGS 38  // per layer and output block: 3 or 4 input blocks + the bias block
 __device__ __forceinline__ int infer_frag(int L, int mb, int kb) {  // kb == nkb: the bias block
@@ -796,7 +796,7 @@ __global__ void __launch_bounds__(64 * TG_INFER_WAVES) k_guide_infer(
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
 #ifndef TG_INFER_BLOCKS_PER_CU
-#define TG_INFER_BLOCKS_PER_CU 4u
+#define TG_INFER_BLOCKS_PER_CU 1u
 #endif
 
 }  // namespace
@@ -891,9 +891,10 @@ int32_t guide_infer(th_guide* g, const uint32_t* d_count, uint32_t count_stride,
                     uint32_t seg_cap, const float4* vp, const float4* vn, const float4* vw,
                     const float* bmin, const float* bmax, uint16_t* O, hipStream_t st) {
   if (nseg == 0u) return th_set_error(TH_E_INVALID, "guide queue without segments");
-  // persistent: as many blocks as stay resident (TG_INFER_BLOCKS_PER_CU: the LDS of
-  // four fills a compute unit), so each block loads the network once and its waves walk
-  // many tiles; a multiple of nseg, so every segment has the same number of blocks
+  // persistent: one block of 16 waves per compute unit (four per SIMD), so each compute
+  // unit copies the network into LDS once and its waves walk many tiles (four blocks of
+  // four waves, each with its own copy, measured 7 % slower); a multiple of nseg, so every
+  // segment has the same number of blocks
   const uint32_t tiles = (seg_cap + 32u * TG_INFER_WAVES - 1u) / (32u * TG_INFER_WAVES);
   uint32_t per_seg = std::min<uint32_t>(
       tiles, (TG_INFER_BLOCKS_PER_CU * (uint32_t)std::max(g->ctx->cu_count, 1) + nseg - 1u) / nseg);
