@@ -436,10 +436,15 @@ __global__ void __launch_bounds__(TF_THREADS) k_train_fused(const uint16_t* __re
 
 // Adam on the FP32 master parameters (gradient = the fixed-order sum of the `nblocks`
 // block partials, or only the gradient when lr == 0), then both padded BF16 copies and
-// biases.  A block takes 64 consecutive parameters; wave q sums groups 4q..4q+3 of each
-// (64 independent loads per lane, partials TG_PART floats apart), wave 0 adds the 16
-// group sums in order and applies the update.
-__global__ void __launch_bounds__(256) k_adam(float* __restrict__ master, float* __restrict__ m1,
+// biases.  A block takes 64 consecutive parameters; its TG_ADAM_WAVES waves share the 16
+// groups (wave q sums groups q * 16 / TG_ADAM_WAVES ..., 16 loads per group and lane,
+// partials TG_PART floats apart), wave 0 adds the 16 group sums in order and applies the
+// update.
+#ifndef TG_ADAM_WAVES
+#define TG_ADAM_WAVES 16u
+#endif
+#define TG_ADAM_GPW (TG_GROUPS / TG_ADAM_WAVES)  // groups per wave
+__global__ void __launch_bounds__(64 * TG_ADAM_WAVES) k_adam(float* __restrict__ master, float* __restrict__ m1,
                        float* __restrict__ m2, const float* __restrict__ partials, StepCtl ctl, float lr,
                        float b1, float b2, float eps, float* __restrict__ grad_out,
                        uint16_t* __restrict__ woi, uint16_t* __restrict__ wio, float* __restrict__ bias,
@@ -475,21 +480,21 @@ __global__ void __launch_bounds__(256) k_adam(float* __restrict__ master, float*
   if (valid) {
     const float* base =
         partials + (size_t)l * 256u * TG_PART + (is_bias ? (size_t)64 * 64 + n : (size_t)n * 64 + k);
-    float v[4][256 / TG_GROUPS];
+    float v[TG_ADAM_GPW][256 / TG_GROUPS];
 #pragma unroll
-    for (uint32_t jj = 0; jj < 4; ++jj)
+    for (uint32_t jj = 0; jj < TG_ADAM_GPW; ++jj)
 #pragma unroll
       for (uint32_t t = 0; t < 256 / TG_GROUPS; ++t) {
-        const uint32_t b = 4u * q + jj + t * TG_GROUPS;
+        const uint32_t b = TG_ADAM_GPW * q + jj + t * TG_GROUPS;
         v[jj][t] = b < nblocks ? base[(size_t)b * TG_PART] : 0.0f;
       }
 #pragma unroll
-    for (uint32_t jj = 0; jj < 4; ++jj) {
+    for (uint32_t jj = 0; jj < TG_ADAM_GPW; ++jj) {
       float s = 0.0f;
 #pragma unroll
       for (uint32_t t = 0; t < 256 / TG_GROUPS; ++t)
-        if (4u * q + jj + t * TG_GROUPS < nblocks) s += v[jj][t];
-      gs[4u * q + jj][lane] = s;
+        if (TG_ADAM_GPW * q + jj + t * TG_GROUPS < nblocks) s += v[jj][t];
+      gs[TG_ADAM_GPW * q + jj][lane] = s;
     }
   }
   __syncthreads();
@@ -879,7 +884,7 @@ static int32_t guide_step(th_guide* g, const uint16_t* X, const float* dirs, con
   hipLaunchKernelGGL(k_train_fused, dim3(kWgradBlocks), dim3(TF_THREADS), 0, st, X, dirs, w, c, g->wpart, g->d_woi,
                      g->d_wio, g->d_bias, P, g->lpart);
   TH_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, st, g->d_master, g->d_m1, g->d_m2, P, c,
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(64 * TG_ADAM_WAVES), 0, st, g->d_master, g->d_m1, g->d_m2, P, c,
                      lr, g->beta1, g->beta2, g->eps, g->d_grad, g->d_woi, g->d_wio, g->d_bias, g->lpart,
                      g->scal + 1);
   TH_CHECK(hipGetLastError());
@@ -960,7 +965,7 @@ static void host_init_params(uint32_t seed, std::vector<float>& p) {
 
 static int32_t guide_refresh(th_guide* g) {  // bf16 copies from the master weights
   const StepCtl refresh{nullptr, 0u, 0u, nullptr, nullptr};
-  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(256), 0, g->ctx->stream, g->d_master,
+  hipLaunchKernelGGL(k_adam, dim3((TG_NPARAMS + 63) / 64), dim3(64 * TG_ADAM_WAVES), 0, g->ctx->stream, g->d_master,
                      g->d_m1, g->d_m2, g->partials, refresh, 0.0f, g->beta1, g->beta2, g->eps, (float*)nullptr,
                      g->d_woi, g->d_wio, g->d_bias, (const float*)nullptr, (float*)nullptr);
   TH_CHECK(hipGetLastError());
