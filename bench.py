@@ -481,6 +481,20 @@ def batched_leg(runs=3):
             "one_launch_per_frame_gpaths_s": round(paths / dseq / 1e9, 3)}
 
 
+def measured_peaks(ctx):
+    """The device's reachable ceilings, beside the published peaks the roofline fractions
+    use: streaming copy and streaming read of 1 GiB (th_measure_copy; GB/s moved), and
+    dense BF16 MFMA throughput of the inference (32x32x16) and training (16x16x32) shapes
+    (th_measure_mfma; an MFMA result that is not exact reports null)."""
+    out = {"copy_gbs": round(ctx.measure_copy(1 << 30, 8), 1),
+           "read_gbs": round(ctx.measure_copy(1 << 30, 8, read_only=True), 1)}
+    for shape, waves, name in ((1, 4, "mfma_bf16_32x32x16_tflops"), (0, 8, "mfma_bf16_16x16x32_tflops")):
+        tf, exact = ctx.measure_mfma(shape, waves, 16384)
+        out[name] = round(tf, 1) if exact else None
+    out["published"] = {"hbm_gbs": HBM_PEAK_GBS, "bf16_dense_tflops": BF16_PEAK_TFLOPS}
+    return out
+
+
 def copy_calibration(mib=1024, copies=10):
     """A plain device-to-device copy of `mib` MiB, `copies` times (torch's copy kernel on
     device memory); each copy reads and writes exactly mib MiB, so the profiler's
@@ -653,6 +667,8 @@ def main():
         variants[name] = {"kernel_ms": round(kms, 4), "mpaths_s": round(sc.paths_per_frame / kms / 1e3, 2)}
     gs.close()
     scene_update = scene_update_leg(ctx, sc)
+    log("measured ceilings")
+    peaks = measured_peaks(ctx)
     ctx.close()
     dt = t1 - t0
     paths = sc.paths_per_frame
@@ -691,7 +707,15 @@ def main():
                                       "achieved": round(bytes_nf / (kernel_ms * 1e-3) / 1e9, 2),
                                       "frac": round(bytes_nf / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                                       "nodes_visited": ordered_counters["nodes_visited"],
-                                      "reference_order_nodes_visited": counters["nodes_visited"]}}
+                                      "reference_order_nodes_visited": counters["nodes_visited"]},
+                    # against the streaming read this device measurably reaches
+                    "measured_read_gbs": peaks["read_gbs"],
+                    "frac_of_measured_read": round(achieved / peaks["read_gbs"], 5) if peaks["read_gbs"] else None}
+    if equal_time is not None and "mfma" in equal_time:
+        for leg, shape in (("inference", "mfma_bf16_32x32x16_tflops"), ("training", "mfma_bf16_16x16x32_tflops")):
+            m = equal_time["mfma"].get(leg)
+            if m is not None and peaks.get(shape):
+                m["utilization_of_measured"] = round(m["tflops"] / peaks[shape], 5)
     line = {
         "metric": "Mpaths/s + equal-time relRMSE: guided vs plain HIP tracer, 1 MI355X",
         "value": round(args.steps * paths / dt / 1e6, 3),
@@ -707,6 +731,7 @@ def main():
         "rays_per_s": round(args.steps * rays_per_frame / dt, 1),
         "kernel_ms_mean": round(kernel_ms, 5),
         "roofline": roofline,
+        "measured_peaks": peaks,
         "variants": variants,
         "scene_update": scene_update,
         "equal_time": equal_time,

@@ -576,6 +576,21 @@ int32_t th_guide_mfma32_selftest(th_context* ctx, const float* a, const float* b
  * no network); a check of the rounding the guide's oracle emulates. */
 int32_t th_bf16_selftest(th_context* ctx, uint64_t* mismatches, uint64_t* nan_mismatches);
 
+/* ---- measured device ceilings (diagnostics; no counterpart in the reference, whose
+ * benchmarks report no roofline) ----
+ * th_measure_copy: a streaming copy of `bytes` (a nonzero multiple of 16) between two
+ * device buffers it allocates (read_only != 0: a streaming read of one), `reps` times
+ * after one untimed round; *gbs = bytes read + written per second / 1e9 (the HBM
+ * bandwidth a kernel can reach).
+ * th_measure_mfma: `iters` rounds of four independent BF16 MFMAs per wave on every SIMD
+ * (`waves_per_simd` waves each, 1..8), shape 0 = v_mfma_f32_16x16x32_bf16 (training
+ * kernels), 1 = v_mfma_f32_32x32x16_bf16 (inference kernel), after one untimed launch;
+ * *tflops = dense FLOP/s / 1e12; *exact (may be null) = 1 when every accumulator holds
+ * the exact all-ones product sum. */
+int32_t th_measure_copy(th_context* ctx, uint64_t bytes, uint32_t reps, uint32_t read_only, double* gbs);
+int32_t th_measure_mfma(th_context* ctx, uint32_t shape, uint32_t waves_per_simd, uint32_t iters, double* tflops,
+                        uint32_t* exact);
+
 /* Linear HDR float4 -> RGBA8 with exposure, ACES fitted and sRGB OETF
  * (image_output.rs:59-92, color.rs:23-75).  Host implementation. */
 int32_t th_display_rgba8(const float* rgba, uint32_t width, uint32_t height, float exposure_stops,

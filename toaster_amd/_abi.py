@@ -218,6 +218,8 @@ SIGNATURES = {
     "th_guide_mfma_selftest": (C.c_int32, [P, P, P, P]),
     "th_guide_mfma32_selftest": (C.c_int32, [P, P, P, P]),
     "th_bf16_selftest": (C.c_int32, [P, P, P]),
+    "th_measure_copy": (C.c_int32, [P, C.c_uint64, u32, u32, C.POINTER(C.c_double)]),
+    "th_measure_mfma": (C.c_int32, [P, u32, u32, u32, C.POINTER(C.c_double), C.POINTER(u32)]),
     "th_render_guided": (C.c_int32, [P, P, P, C.POINTER(GpuRenderParams), C.POINTER(GpuCamera),
                                      C.POINTER(GuidedOptions), C.POINTER(GuidedStats)]),
 }

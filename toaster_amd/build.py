@@ -28,7 +28,8 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
               "-Wno-unused-function"]
 
-HIP_SOURCES = ["th_runtime.hip", "th_wavefront.hip", "th_batch.hip", "th_guide.hip", "th_guided.hip"]
+HIP_SOURCES = ["th_runtime.hip", "th_wavefront.hip", "th_batch.hip", "th_guide.hip", "th_guided.hip",
+               "th_peaks.hip"]
 # Per-source extra flags.  The network kernels keep MFMA results in ordinary VGPRs (no
 # accumulation-register copies: one v_accvgpr_read per result element otherwise).
 HIP_SOURCE_FLAGS = {"th_guide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}

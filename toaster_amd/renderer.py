@@ -80,6 +80,21 @@ class GpuContext:
     def reset_stats(self):
         A.check(self._lib.th_reset_stats(self._h))
 
+    def measure_copy(self, nbytes=1 << 30, reps=8, read_only=False):
+        """Streaming-copy (or read) bandwidth of the device (th_measure_copy): GB/s read +
+        written."""
+        gbs = C.c_double()
+        A.check(self._lib.th_measure_copy(self._h, int(nbytes), int(reps), 1 if read_only else 0, C.byref(gbs)))
+        return gbs.value
+
+    def measure_mfma(self, shape=1, waves_per_simd=2, iters=16384):
+        """Dense BF16 MFMA throughput (th_measure_mfma): (TFLOP/s, every product exact).
+        shape 0: v_mfma_f32_16x16x32_bf16; 1: v_mfma_f32_32x32x16_bf16."""
+        tf, ok = C.c_double(), A.u32()
+        A.check(self._lib.th_measure_mfma(self._h, int(shape), int(waves_per_simd), int(iters), C.byref(tf),
+                                          C.byref(ok)))
+        return tf.value, bool(ok.value)
+
     def adopt(self, child):
         """Registers an object (scene, guide) that must be released before this context."""
         self._children.add(child)
