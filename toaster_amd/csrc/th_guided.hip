@@ -277,7 +277,9 @@ __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, fal
   {
     v3 so, dir, term;
     float limit;
-    if (it.light_candidate(p, n, albedo, &rng, &so, &dir, &limit, &term)) {
+    // a shadow ray an extracted primitive blocks adds nothing: resolved here, not queued
+    if (it.light_candidate(p, n, albedo, &rng, &so, &dir, &limit, &term) &&
+        !lane_extracted_anyhit(it.S, so, dir, limit)) {
       v3 c = th::mul(throughput, term);
       B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
       B.sh0[k] = make_float4(dir.x, dir.y, dir.z, limit);
@@ -293,10 +295,12 @@ __device__ __forceinline__ bool shade_diffuse(const GdBufs& B, Integrator<1, fal
       if (guided && !last) bp = alpha * tg::mixture_pdf(*m, es.dir) + (1.0f - alpha) * bp;
       v3 c = th::mul(throughput, it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, bp)));
       v3 so = th::add(p, th::muls(n, TH_MIN_DISTANCE));
-      B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
-      B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);
-      B.c1[k] = make_float4(c.x, c.y, c.z, 0.0f);
-      sh_env = true;
+      if (!lane_extracted_anyhit(it.S, so, es.dir, 10000.001f)) {
+        B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
+        B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);
+        B.c1[k] = make_float4(c.x, c.y, c.z, 0.0f);
+        sh_env = true;
+      }
     }
   }
   if (!guided) {

@@ -217,7 +217,9 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
           {
             v3 so, dir, term;
             float limit;
-            if (it.light_candidate(rec.p, rec.n, albedo, &rng, &so, &dir, &limit, &term)) {
+            // a shadow ray an extracted primitive blocks adds nothing: resolved here
+            if (it.light_candidate(rec.p, rec.n, albedo, &rng, &so, &dir, &limit, &term) &&
+                !lane_extracted_anyhit(S, so, dir, limit)) {
               v3 c = th::mul(throughput, term);
               B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
               B.sh0[k] = make_float4(dir.x, dir.y, dir.z, limit);
@@ -232,10 +234,12 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
             if (it.env_candidate(rec.n, &rng, &es, &sc)) {
               v3 c = th::mul(throughput, it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, sc / TH_PI)));
               v3 so = th::add(rec.p, th::muls(rec.n, TH_MIN_DISTANCE));
-              B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
-              B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);
-              B.c1[k] = make_float4(c.x, c.y, c.z, 0.0f);
-              sh_env = true;
+              if (!lane_extracted_anyhit(S, so, es.dir, 10000.001f)) {
+                B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
+                B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);
+                B.c1[k] = make_float4(c.x, c.y, c.z, 0.0f);
+                sh_env = true;
+              }
             }
           }
           v3 dir = th::add(rec.n, it.random_unit_vector(&rng));

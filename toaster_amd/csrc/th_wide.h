@@ -311,6 +311,26 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
   return bslot != THD_NO_SLOT;
 }
 
+// Any hit with t < limit among the scene's extracted primitives, tested by one lane:
+// the tests of traverse_quad_rt's first step in the same arithmetic (the primitive's
+// distance, counted only through its reference leaf's box), so a ray this finds blocked
+// is one the quad traversal reports occluded.  Shading kernels resolve such shadow rays
+// at once instead of queueing them.
+__device__ __forceinline__ bool lane_extracted_anyhit(const DevScene& S, const v3& o, const v3& d, float limit) {
+  if (S.ext_n == 0u) return false;
+  const RayPre r = make_ray(o, d);
+#pragma unroll 1
+  for (uint32_t k = 0; k < S.ext_n; ++k) {
+    const float4* nb = (const float4*)(S.nodes + S.ext_node[k]);
+    const float4 mn = nb[0], mx = nb[1];
+    uint32_t tag;
+    const float tp = slot_t(S, S.ext_slot[k], o, d, &tag);
+    float tl;
+    if (box_bf(mn.x, mn.y, mn.z, mx.x, mx.y, mx.z, r, TH_FLT_MAX, &tl) && tp < limit) return true;
+  }
+  return false;
+}
+
 // MODE 1: closest hit.  MODE 2: any hit with t < limit.
 template <int MODE>
 __device__ __forceinline__ bool traverse_quad(const DevScene& S, const Child4* __restrict__ nodes4,
