@@ -423,6 +423,40 @@ def equal_time_guided(reference_spp):
     }
 
 
+def workloads_leg(runs=3):
+    """Workloads 1 (Cornell box, 320x180, 4 spp, 4 bounces) and 3 (random triangles and
+    spheres on a ground quad under the 1024x512 sky + sun HDR, 1280x720, 64 spp, 8
+    bounces) at their full shapes: device ms per frame and Mpaths/s of the 4-wide kernels
+    (reference RNG: static, octet and speculative schedules, the wavefront) and of the
+    per-sample quad kernel; the mean of `runs` frames after one untimed frame."""
+    from toaster_amd import GpuContext, GpuScene, RenderOptions, _abi as A, scenes
+    W = A.TH_TRAVERSE_WIDE
+    legs = (("quad", RenderOptions(traversal=W)),
+            ("octet", RenderOptions(traversal=W, schedule=A.TH_SCHEDULE_OCTET)),
+            ("speculative", RenderOptions(traversal=W, schedule=A.TH_SCHEDULE_SPECULATIVE)),
+            ("wavefront", RenderOptions(traversal=W, kernel=A.TH_KERNEL_WAVEFRONT)),
+            ("quad_per_sample_rng", RenderOptions(traversal=W, rng_mode=A.TH_RNG_PER_SAMPLE)))
+    ctx = GpuContext(0)
+    out = {}
+    for name, sc in (("cornell_320x180_4spp", scenes.cornell()),
+                     ("mixed_environment_1280x720_64spp", scenes.mixed_environment())):
+        gs = GpuScene.from_scene(ctx, sc)
+        res = {"paths_per_frame": sc.paths_per_frame, "extracted_primitives": gs.traversal_info()[0]}
+        for lname, o in legs:
+            gs.render(options=o, frame_index=0, accumulated_samples=0)
+            ctx.synchronize()
+            ctx.reset_stats()
+            for _ in range(runs):
+                gs.render(options=o, frame_index=0, accumulated_samples=0)
+            st = ctx.stats()
+            kms = st["total_kernel_ms"] / max(1, st["launches"])
+            res[lname] = {"kernel_ms": round(kms, 4), "mpaths_s": round(sc.paths_per_frame / kms / 1e3, 2)}
+        gs.close()
+        out[name] = res
+    ctx.close()
+    return out
+
+
 def batched_leg(runs=3):
     """The batched workload: 16 scenes (128 - 262,144 triangles, 0 - 256 spheres, three
     resolutions) x 4 rigid-motion frames, 16 spp, 6 bounces, per-sample RNG, rendered by
@@ -574,6 +608,7 @@ def main():
     ap.add_argument("--no-guided", action="store_true")
     ap.add_argument("--no-suite", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true")
     ap.add_argument("--reference-spp", type=int, default=16384)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 10 times (a copy kernel of known size for the "
@@ -681,6 +716,8 @@ def main():
     suite = None if args.no_suite else reference_suite()
     log("batched leg")
     batched = None if args.no_batch else batched_leg()
+    log("workloads 1 and 3 leg")
+    workloads = None if args.no_workloads else workloads_leg()
     cpu, counters, parity = (None, None, None)
     if not args.no_cpu:
         log("cpu baseline leg (native oracle build, parity check)")
@@ -737,6 +774,7 @@ def main():
         "equal_time": equal_time,
         "reference_suite": suite,
         "batched": batched,
+        "workloads": workloads,
         "cpu_baseline": cpu,
         "parity": parity,
         "host": platform.node(),
