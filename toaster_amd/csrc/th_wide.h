@@ -278,7 +278,9 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
       const uint32_t MISS = 0x7f800000u;
       uint32_t key = (__float_as_uint(tE) & ~3u) | lq;
       uint32_t k0 = qbcast_u<0>(key), k1 = qbcast_u<1>(key), k2 = qbcast_u<2>(key), k3 = qbcast_u<3>(key);
-      uint32_t nh = (k0 < MISS) + (k1 < MISS) + (k2 < MISS) + (k3 < MISS);
+      // hit children of the quad: its four bits of the wave's hit mask
+      const uint64_t hm = __ballot(key < MISS);
+      const uint32_t nh = (uint32_t)__builtin_popcount((uint32_t)(hm >> (threadIdx.x & 60u)) & 0xfu);
       if (nh > 0u) {
         uint32_t rank = (k0 < key) + (k1 < key) + (k2 < key) + (k3 < key);
         uint32_t w = __builtin_elementwise_min(__builtin_elementwise_min(k0, k1),
