@@ -4,7 +4,8 @@
 // 4-wide hierarchy obtained by collapsing every interior node of the reference's
 // median-split binary tree with its interior children (same leaves, same leaf primitive
 // order): at an interior node lane j tests child j's box, at a leaf lane j tests
-// primitive j, and the quad combines results with DPP quad permutes (no LDS round trip).
+// primitive j, and the quad combines results with DPP quad permutes and, for the next
+// entry, one ds_bpermute (the LDS crossbar, no LDS memory access).
 // A scene may name up to 8 extracted primitives (DevScene::ext_*: large ones that would
 // inflate every box above them); the quad tests those first, and the 4-wide boxes are
 // the boxes of what remains below each node (the reference's boxes otherwise).  Children are visited nearest first; the farther hit children go on a
@@ -285,10 +286,10 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
         uint32_t rank = (k0 < key) + (k1 < key) + (k2 < key) + (k3 < key);
         uint32_t w = __builtin_elementwise_min(__builtin_elementwise_min(k0, k1),
                                                __builtin_elementwise_min(k2, k3)) & 3u;
-        uint32_t e0 = qbcast_u<0>(ce), e1 = qbcast_u<1>(ce), e2 = qbcast_u<2>(ce), e3 = qbcast_u<3>(ce);
-        // entry of lane w by its two bits (selects, no branches)
-        uint32_t e01 = (w & 1u) ? e1 : e0, e23 = (w & 1u) ? e3 : e2;
-        uint32_t ew = (w & 2u) ? e23 : e01;
+        // entry of lane w of the quad through the LDS crossbar (ds_bpermute: no memory
+        // access; one instruction in place of four broadcasts and three selects, which
+        // measured 2 % faster on the headline frame and the terrain passes)
+        const uint32_t ew = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(((threadIdx.x & 60u) | w) << 2), (int)ce);
         // the other hit children, farthest deepest
         if (key < MISS && rank > 0u) st.at(st.sp + (int)(nh - 1u - rank)) = make_uint2(ce, __float_as_uint(tE));
         st.sp += (int)nh - 1;
