@@ -161,8 +161,12 @@ __global__ void __launch_bounds__(64) wf_extend(WfBufs B, DevScene S, TH_GpuRend
 
 // ---------------------------------------------------------------- shade
 // One iteration of the reference loop body (wgsl:817-911) for the hit found by extend.
+// `pretest`: shadow rays an extracted primitive blocks are resolved here (the 4-wide
+// traversal's own first step in the same arithmetic); the scalar traversals keep every
+// shadow ray for their reference-order walk.
 __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRenderParams P,
-                                               const float* __restrict__ srgb_lut, uint32_t live_idx) {
+                                               const float* __restrict__ srgb_lut, uint32_t live_idx,
+                                               uint32_t pretest) {
   const uint32_t sg = blockIdx.x % WF_NSEG;
   const uint32_t* q = (live_idx ? B.q1 : B.q0) + (size_t)sg * B.seg_cap;
   uint32_t n = B.cnt[WF_QC(live_idx, sg)];
@@ -219,7 +223,7 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
             float limit;
             // a shadow ray an extracted primitive blocks adds nothing: resolved here
             if (it.light_candidate(rec.p, rec.n, albedo, &rng, &so, &dir, &limit, &term) &&
-                !lane_extracted_anyhit(S, so, dir, limit)) {
+                !(pretest && lane_extracted_anyhit(S, so, dir, limit))) {
               v3 c = th::mul(throughput, term);
               B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
               B.sh0[k] = make_float4(dir.x, dir.y, dir.z, limit);
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(64) wf_shade(WfBufs B, DevScene S, TH_GpuRende
             if (it.env_candidate(rec.n, &rng, &es, &sc)) {
               v3 c = th::mul(throughput, it.env_term(albedo, es, sc, it.power_heuristic(es.pdf, sc / TH_PI)));
               v3 so = th::add(rec.p, th::muls(rec.n, TH_MIN_DISTANCE));
-              if (!lane_extracted_anyhit(S, so, es.dir, 10000.001f)) {
+              if (!(pretest && lane_extracted_anyhit(S, so, es.dir, 10000.001f))) {
                 B.sho[k] = make_float4(so.x, so.y, so.z, 0.0f);
                 B.sh1[k] = make_float4(es.dir.x, es.dir.y, es.dir.z, 10000.001f);
                 B.c1[k] = make_float4(c.x, c.y, c.z, 0.0f);
@@ -605,7 +609,7 @@ static int32_t wf_launch(th_context* ctx, th_scene* s, const TH_GpuRenderParams*
     else
       hipLaunchKernelGGL(wf_extend<1>, dim3(grid_trace), dim3(block), lds, ctx->stream, B, S, *p, li, 0u);
     hipLaunchKernelGGL(wf_shade, dim3(grid_scalar), dim3(block), 0, ctx->stream, B, S, *p,
-                       (const float*)ctx->d_srgb_lut, li);
+                       (const float*)ctx->d_srgb_lut, li, quad ? 1u : 0u);
     if (quad)
       hipLaunchKernelGGL(wf_shadow<3>, dim3(grid_shadow), dim3(block), lds, ctx->stream, B, S, *p, 1u);
     else if (o->traversal == TH_TRAVERSE_REFERENCE)
