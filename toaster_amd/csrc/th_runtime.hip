@@ -483,13 +483,19 @@ __global__ void __launch_bounds__(256, 4) k_render_spec(DevScene S, TH_GpuRender
         // b = B - r diffuse vertices (ranks with b < 0 stay idle)
         uint32_t h = e;
         const uint32_t len = 2u + per_vertex * B;
+        if (G == 2) {
+          // the single candidate (b = B) starts where the walk ends
+          for (uint32_t t = 1u; t <= len; ++t) h = th::pcg_hash(h);
+          cand[0] = h;
+        } else {
 #pragma unroll
-        for (int r = 0; r < G - 1; ++r) cand[r] = 0u;
-        for (uint32_t t = 1u; t <= len; ++t) {
-          h = th::pcg_hash(h);
+          for (int r = 0; r < G - 1; ++r) cand[r] = 0u;
+          for (uint32_t t = 1u; t <= len; ++t) {
+            h = th::pcg_hash(h);
 #pragma unroll
-          for (int r = 0; r < G - 1; ++r)
-            if ((uint32_t)r <= B && t == 2u + per_vertex * (B - (uint32_t)r)) cand[r] = h;
+            for (int r = 0; r < G - 1; ++r)
+              if ((uint32_t)r <= B && t == 2u + per_vertex * (B - (uint32_t)r)) cand[r] = h;
+          }
         }
       }
       if (j == lq) {
