@@ -162,20 +162,21 @@ __global__ void __launch_bounds__(256) gd_generate(GdBufs B, GdArgs A, TH_GpuRen
                                                    TH_GpuCamera cam) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < A.n) {
-    float4 a = A.smp == 0u ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : B.acc[k];
     if (P.max_bounces == 0u) {  // ray_color returns zero (wgsl:536)
+      float4 a = A.smp == 0u ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : B.acc[k];
       a.w = a.w + 1.0f;
+      B.acc[k] = a;
     } else {
+      if (A.smp == 0u) B.acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       uint32_t rng = th::pcg_hash((k ^ (P.frame_index * 9781u)) + A.smp * 0x9E3779B9u);
       v3 o, d;
       camera_ray(cam, P, k % P.width, k / P.width, &rng, &o, &d);
       B.o[k] = make_float4(o.x, o.y, o.z, 0.0f);
       B.d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(GD_INCLUDE_EMISSIVE));
-      B.thr[k] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(GD_NO_RECORD));
-      B.rad[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       B.rng[k] = rng;
+      // throughput (1, 1, 1) with no record and zero radiance: the first bounce's shading
+      // starts from these constants instead of reading them back
     }
-    B.acc[k] = a;
   }
   // every path is live at the first bounce: that bounce's queue is the identity (path i
   // at position i), so only its length is stored
@@ -379,7 +380,9 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
     float4 pk_p, pk_n, pk_w;  // the parked vertex
     if (act) {
       k = first ? i : q[i];
-      float4 o4 = B.o[k], d4 = B.d[k], t4 = B.thr[k], r4 = B.rad[k];
+      float4 o4 = B.o[k], d4 = B.d[k];
+      float4 t4 = first ? make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(GD_NO_RECORD)) : B.thr[k];
+      float4 r4 = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : B.rad[k];
       apply_pending(B, k, __float_as_uint(d4.w), r4, t4);
       uint32_t rng = B.rng[k];
       uint2 h = B.hit[k];
@@ -475,6 +478,8 @@ __global__ void __launch_bounds__(64) gd_shade_a(GdBufs B, GdArgs A, DevScene S,
         B.thr[k] = make_float4(throughput.x, throughput.y, throughput.z, t4.w);
         B.rng[k] = rng;
         live = !done;
+      } else if (first) {
+        B.thr[k] = t4;  // the parked path's throughput, read by the second shading kernel
       }
       B.rad[k] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
     }
