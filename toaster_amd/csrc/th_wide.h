@@ -299,8 +299,9 @@ __device__ __forceinline__ bool traverse_quad_rt(const DevScene& S, const Child4
         // measured 2 % faster on the headline frame and the terrain passes)
         const uint32_t ew = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(((threadIdx.x & 60u) | w) << 2), (int)ce);
         // the other hit children, farthest deepest
-        if (key < MISS && rank > 0u) st.at(st.sp + (int)(nh - 1u - rank)) = make_uint2(ce, __float_as_uint(tE));
-        st.sp += (int)nh - 1;
+        const int nsp = st.sp + (int)nh - 1;  // the stack top after the pushes
+        if (key < MISS && rank > 0u) st.at(nsp - (int)rank) = make_uint2(ce, __float_as_uint(tE));
+        st.sp = nsp;
         e = ew;
         continue;
       }
