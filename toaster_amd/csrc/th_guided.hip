@@ -813,7 +813,13 @@ extern "C" int32_t th_render_guided(th_context* ctx, th_scene* s, th_guide* g, c
   const size_t lds = (size_t)depth * (block / 4) * sizeof(uint2);
   // every grid a multiple of GD_NSEG: block b serves queue segment b % GD_NSEG
   auto segs = [](uint32_t g) { return (g + GD_NSEG - 1u) / GD_NSEG * GD_NSEG; };
-  const uint32_t grid_trace = segs(std::min<uint32_t>((npix * 4u + block - 1) / block, 8192u));
+  // The closest-hit grid holds one wave per 16 paths, so on the first bounce each wave
+  // traces one round of rays and the dispatcher balances the waves' unequal costs: on the
+  // 1280x720 terrain 1.16x faster than 8192 persistent waves (2.55 -> 2.21 ms per plain
+  // pass).  Later bounces' surplus waves find their segment empty and exit at once.
+  // The shadow grid stays at 8192 waves (a larger one measured slower in the guided passes)
+  // and the shading grid at 4096 (8192 or 14,400 within noise).
+  const uint32_t grid_trace = segs((npix * 4u + block - 1) / block);
   const uint32_t grid_scalar = segs(std::min<uint32_t>((npix + block - 1) / block, 4096u));
   const uint32_t grid_shadow = segs(std::min<uint32_t>((2u * npix * 4u + block - 1) / block, 8192u));
   GdArgs A;

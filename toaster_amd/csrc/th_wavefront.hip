@@ -590,10 +590,12 @@ static int32_t wf_launch(th_context* ctx, th_scene* s, const TH_GpuRenderParams*
   const uint32_t block = 64;
   uint32_t depth = quad ? 3u * s->depth4 + 2u : std::max<uint32_t>(s->max_depth + 1u, 2u);
   size_t lds = (size_t)depth * (quad ? block / 4 : block) * sizeof(uint2);
-  // persistent grids: enough waves to fill the device, grid-stride over the queues; each a
-  // multiple of WF_NSEG (block b serves segment b % WF_NSEG)
+  // grids grid-stride over the queues, each a multiple of WF_NSEG (block b serves segment
+  // b % WF_NSEG).  The closest-hit grid holds one wave per queue round of the first bounce,
+  // so the dispatcher balances the waves' unequal costs (terrain 1280x720, 16 spp:
+  // 45.8 -> 39.1 ms against 8192 persistent waves); the others stay persistent.
   auto segs = [](uint32_t g) { return (g + WF_NSEG - 1u) / WF_NSEG * WF_NSEG; };
-  uint32_t grid_trace = segs(std::min<uint32_t>((n * (quad ? 4u : 1u) + block - 1) / block, 8192u));
+  uint32_t grid_trace = segs((n * (quad ? 4u : 1u) + block - 1) / block);
   uint32_t grid_scalar = segs(std::min<uint32_t>((n + block - 1) / block, 4096u));
   uint32_t grid_shadow = segs(std::min<uint32_t>((2u * n * (quad ? 4u : 1u) + block - 1) / block, 8192u));
   uint32_t per_sample_count = A.per_sample ? (p->samples + G - 1) / G : p->samples;
